@@ -1,0 +1,103 @@
+"""ctypes binding of gem_amd/_lib/libgem_hip.so (the C ABI in include/gem_hip.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C gem_amd/csrc``).  Loading fails loudly when it is missing: there is no
+CPU fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
+LIB_PATH = os.path.join(LIB_DIR, "libgem_hip.so")
+
+GEM_MAX_QV = 32
+GEM_MAX_QE = 256
+
+STATUS = {0: "GEM_OK", 1: "GEM_E_INVALID", 2: "GEM_E_HIP", 3: "GEM_E_NOMEM", 4: "GEM_E_QUERY",
+          5: "GEM_E_STATE", 6: "GEM_E_IO", 7: "GEM_E_LIMIT"}
+
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+
+
+class QueryView(C.Structure):
+    _fields_ = [("num_v", C.c_uint32), ("num_e", C.c_uint32), ("labels", u32p), ("edges", u32p)]
+
+
+class CsrView(C.Structure):
+    _fields_ = [("num_v", C.c_uint32), ("num_slots", C.c_uint64), ("offsets", u64p),
+                ("neighbors", u32p), ("labels", u32p)]
+
+
+class JoinResult(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("saturated", C.c_uint32), ("num_steps", C.c_uint32),
+                ("num_embeddings_written", C.c_uint64)]
+
+
+class StepLog(C.Structure):
+    _fields_ = [("edge", C.c_uint32 * GEM_MAX_QE), ("is_select", C.c_uint32 * GEM_MAX_QE),
+                ("rows", C.c_uint64 * GEM_MAX_QE), ("ms", C.c_float * GEM_MAX_QE)]
+
+
+class PhaseTimes(C.Structure):
+    _fields_ = [("filter_ms", C.c_float), ("index_ms", C.c_float), ("tables_ms", C.c_float),
+                ("join_ms", C.c_float)]
+
+
+# (name, restype, argtypes) for every function declared in include/gem_hip.h
+SIGNATURES = [
+    ("gem_graph_generate_er", C.c_int, [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("gem_graph_generate_rmat", C.c_int, [C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                          C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("gem_graph_from_csr", C.c_int, [C.c_uint32, u64p, u32p, u32p, C.POINTER(C.c_void_p)]),
+    ("gem_graph_load", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("gem_graph_save", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("gem_graph_num_vertices", C.c_uint32, [C.c_void_p]),
+    ("gem_graph_num_slots", C.c_uint64, [C.c_void_p]),
+    ("gem_graph_offsets", u64p, [C.c_void_p]),
+    ("gem_graph_neighbors", u32p, [C.c_void_p]),
+    ("gem_graph_labels", u32p, [C.c_void_p]),
+    ("gem_graph_free", None, [C.c_void_p]),
+    ("gem_query_random_walk", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, u32p, u32p, C.c_uint32, u32p]),
+    ("gem_query_load", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p]),
+    ("gem_query_save", C.c_int, [C.POINTER(QueryView), C.c_char_p]),
+    ("gem_plan_gem_order", C.c_int, [C.POINTER(QueryView), u64p, u32p]),
+    ("gem_hip_create", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    ("gem_hip_destroy", None, [C.c_void_p]),
+    ("gem_hip_last_error", C.c_char_p, [C.c_void_p]),
+    ("gem_hip_set_level_budget", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("gem_hip_load_data_csr", C.c_int, [C.c_void_p, C.POINTER(CsrView)]),
+    ("gem_hip_filter", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, u32p, u32p, C.c_uint64]),
+    ("gem_hip_position_lookup", C.c_int, [C.c_void_p, C.c_uint32, u32p, C.c_uint64, u32p]),
+    ("gem_hip_build_edge_tables", C.c_int, [C.c_void_p, u64p]),
+    ("gem_hip_read_edge_table", C.c_int, [C.c_void_p, C.c_uint32, u64p, u32p]),
+    ("gem_hip_join", C.c_int, [C.c_void_p, u32p, C.c_uint64, C.POINTER(JoinResult), C.POINTER(StepLog),
+                               u32p, C.c_uint64]),
+    ("gem_hip_phase_times", C.c_int, [C.c_void_p, C.POINTER(PhaseTimes)]),
+    ("gem_hip_match", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint64, C.POINTER(JoinResult), u32p]),
+]
+
+_lib = None
+
+
+def load():
+    """Load the built library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           f"or make -C gem_amd/csrc")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class GemError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status

@@ -1,0 +1,225 @@
+// C ABI entry points of the device library (declared in include/gem_hip.h).
+#include <cstring>
+#include <new>
+
+#include "ctx.h"
+
+using namespace gemd;
+
+namespace {
+
+template <class F>
+int guard(gem_ctx *c, F &&f) {
+  if (!c) return GEM_E_INVALID;
+  try {
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) throw Error{GEM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)};
+    f();
+    return GEM_OK;
+  } catch (const Error &e) {
+    c->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    c->err = "host allocation failed";
+    return GEM_E_NOMEM;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+    return GEM_E_INVALID;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gem_hip_create(int device, gem_ctx **out) {
+  if (!out) return GEM_E_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return GEM_E_HIP;
+  auto *c = new (std::nothrow) gem_ctx();
+  if (!c) return GEM_E_NOMEM;
+  c->device = device;
+  int st = guard(c, [&]() {
+    GEM_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &e : c->ev) GEM_HIP_CHECK(hipEventCreate(&e));
+    // keep freed pool memory for reuse instead of returning it to the driver
+    hipMemPool_t pool;
+    GEM_HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, device));
+    uint64_t thr = UINT64_MAX;
+    GEM_HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+  });
+  if (st) {
+    gem_hip_destroy(c);
+    return st;
+  }
+  *out = c;
+  return GEM_OK;
+}
+
+void gem_hip_destroy(gem_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->q = QueryDev();
+  c->g = GraphDev();
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *gem_hip_last_error(const gem_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int gem_hip_set_level_budget(gem_ctx *c, uint64_t bytes) {
+  if (!c || bytes < (1ull << 20)) return GEM_E_INVALID;
+  c->level_budget_bytes = bytes;
+  return GEM_OK;
+}
+
+int gem_hip_load_data_csr(gem_ctx *c, const gem_csr_view *g) {
+  return guard(c, [&]() {
+    if (!g || !g->offsets || !g->labels || (g->num_slots && !g->neighbors))
+      throw Error{GEM_E_INVALID, "null graph array"};
+    if (g->offsets[0] != 0 || g->offsets[g->num_v] != g->num_slots)
+      throw Error{GEM_E_INVALID, "offsets do not span num_slots"};
+    GraphDev &G = c->g;
+    c->q = QueryDev();
+    G = GraphDev();
+    G.n = g->num_v;
+    G.slots = g->num_slots;
+    for (uint32_t v = 0; v < g->num_v; ++v) {
+      if (g->offsets[v + 1] < g->offsets[v]) throw Error{GEM_E_INVALID, "offsets not monotone"};
+      G.max_deg = std::max<uint64_t>(G.max_deg, g->offsets[v + 1] - g->offsets[v]);
+    }
+    // dense label ids in first-seen order
+    std::vector<uint32_t> dense(g->num_v);
+    for (uint32_t v = 0; v < g->num_v; ++v) {
+      auto it = G.label_id.find(g->labels[v]);
+      uint32_t id;
+      if (it == G.label_id.end()) {
+        id = (uint32_t)G.label_id.size();
+        G.label_id.emplace(g->labels[v], id);
+        G.label_freq.push_back(0);
+      } else {
+        id = it->second;
+      }
+      G.label_freq[id]++;
+      dense[v] = id;
+    }
+    G.lab8 = G.label_freq.size() <= 255;
+    hipStream_t s = c->stream;
+    G.off.alloc(((uint64_t)G.n + 1) * 8, s);
+    G.nbr.alloc(std::max<uint64_t>(G.slots, 1) * 4, s);
+    GEM_HIP_CHECK(hipMemcpyAsync(G.off.p, g->offsets, ((uint64_t)G.n + 1) * 8, hipMemcpyHostToDevice, s));
+    if (G.slots) GEM_HIP_CHECK(hipMemcpyAsync(G.nbr.p, g->neighbors, G.slots * 4, hipMemcpyHostToDevice, s));
+    if (G.lab8) {
+      std::vector<uint8_t> l8(dense.begin(), dense.end());
+      G.lab.alloc(std::max<uint64_t>(G.n, 1), s);
+      GEM_HIP_CHECK(hipMemcpyAsync(G.lab.p, l8.data(), G.n, hipMemcpyHostToDevice, s));
+      GEM_HIP_CHECK(hipStreamSynchronize(s));
+    } else {
+      G.lab.alloc(std::max<uint64_t>(G.n, 1) * 4, s);
+      GEM_HIP_CHECK(hipMemcpyAsync(G.lab.p, dense.data(), (uint64_t)G.n * 4, hipMemcpyHostToDevice, s));
+      GEM_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    G.loaded = true;
+  });
+}
+
+int gem_hip_filter(gem_ctx *c, const gem_query_view *q, uint32_t threshold, uint32_t *h_num_can, uint32_t *h_can,
+                   uint64_t can_pitch) {
+  return guard(c, [&]() {
+    if (!h_num_can) throw Error{GEM_E_INVALID, "h_num_can is null"};
+    if (threshold != 0) throw Error{GEM_E_INVALID, "only threshold 0 is supported"};
+    run_filter(c, q, threshold);
+    const QueryDev &Qd = c->q;
+    for (uint32_t u = 0; u < Qd.Q.nv; ++u) h_num_can[u] = Qd.ncan[u];
+    if (h_can) {
+      for (uint32_t u = 0; u < Qd.Q.nv; ++u) {
+        if (Qd.ncan[u] > can_pitch) throw Error{GEM_E_INVALID, "can_pitch smaller than a candidate list"};
+        if (Qd.ncan[u])
+          GEM_HIP_CHECK(hipMemcpyAsync(h_can + (uint64_t)u * can_pitch, Qd.can.as<uint32_t>() + Qd.can_off[u],
+                                       (uint64_t)Qd.ncan[u] * 4, hipMemcpyDeviceToHost, c->stream));
+      }
+      GEM_HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+  });
+}
+
+int gem_hip_position_lookup(gem_ctx *c, uint32_t u, const uint32_t *vertices, uint64_t n, uint32_t *pos_out) {
+  return guard(c, [&]() {
+    if (n && (!vertices || !pos_out)) throw Error{GEM_E_INVALID, "null array"};
+    run_position_lookup(c, u, vertices, n, pos_out);
+  });
+}
+
+int gem_hip_build_edge_tables(gem_ctx *c, uint64_t *h_table_sizes) {
+  return guard(c, [&]() {
+    run_tables(c);
+    if (h_table_sizes)
+      for (uint32_t d = 0; d < 2 * c->q.Q.ne; ++d) h_table_sizes[d] = c->q.tab[d].present ? c->q.tab[d].size : 0;
+  });
+}
+
+int gem_hip_read_edge_table(gem_ctx *c, uint32_t d, uint64_t *offsets, uint32_t *values) {
+  return guard(c, [&]() {
+    const QueryDev &Qd = c->q;
+    if (!Qd.tabled) throw Error{GEM_E_STATE, "edge tables have not been built"};
+    if (d >= Qd.tab.size() || !Qd.tab[d].present) throw Error{GEM_E_INVALID, "no table for this directed edge"};
+    const TableDev &T = Qd.tab[d];
+    const uint64_t nrow = Qd.ncan[T.key_u];
+    if (offsets)
+      GEM_HIP_CHECK(hipMemcpyAsync(offsets, T.off.p, (nrow + 1) * 8, hipMemcpyDeviceToHost, c->stream));
+    if (values && T.size)
+      GEM_HIP_CHECK(hipMemcpyAsync(values, T.val.p, T.size * 4, hipMemcpyDeviceToHost, c->stream));
+    GEM_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int gem_hip_join(gem_ctx *c, const uint32_t *edge_order, uint64_t cap, gem_join_result *out, gem_step_log *log,
+                 uint32_t *emb_out, uint64_t emb_capacity) {
+  return guard(c, [&]() {
+    if (!edge_order || !out) throw Error{GEM_E_INVALID, "null argument"};
+    JoinOut jo;
+    run_join(c, edge_order, cap, &jo, log, emb_out, emb_capacity);
+    out->count = jo.count;
+    out->saturated = jo.saturated ? 1 : 0;
+    out->num_steps = c->q.Q.ne - 1;
+    out->num_embeddings_written = jo.emb_written;
+  });
+}
+
+int gem_hip_phase_times(const gem_ctx *c, gem_phase_times *out) {
+  if (!c || !out) return GEM_E_INVALID;
+  *out = c->times;
+  return GEM_OK;
+}
+
+int gem_hip_match(gem_ctx *c, const gem_query_view *q, uint64_t cap, gem_join_result *out, uint32_t *order_out) {
+  return guard(c, [&]() {
+    if (!out) throw Error{GEM_E_INVALID, "null result"};
+    run_filter(c, q, 0);
+    run_tables(c);
+    const gemh::QueryInfo &Q = c->q.Q;
+    std::vector<uint64_t> sz(Q.ne);
+    for (uint32_t e = 0; e < Q.ne; ++e) {
+      const TableDev &a = c->q.tab[2 * e], &b = c->q.tab[2 * e + 1];
+      sz[e] = a.present ? a.size : b.size;
+    }
+    std::vector<uint32_t> order(Q.ne);
+    int st = gemh::gem_order(Q, sz.data(), order.data());
+    if (st) throw Error{st, "GEM ordering failed"};
+    if (order_out) std::copy(order.begin(), order.end(), order_out);
+    JoinOut jo;
+    run_join(c, order.data(), cap, &jo, nullptr, nullptr, 0);
+    out->count = jo.count;
+    out->saturated = jo.saturated ? 1 : 0;
+    out->num_steps = Q.ne - 1;
+    out->num_embeddings_written = 0;
+  });
+}
+
+}  // extern "C"

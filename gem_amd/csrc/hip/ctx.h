@@ -1,0 +1,186 @@
+// Device context, per-query device state and shared helpers of the HIP
+// library.  gfx950 only (wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../host/planner.h"
+#include "gem_hip.h"
+
+namespace gemd {
+
+constexpr uint32_t WAVE = 64;
+constexpr uint32_t NO_SLOT = 0xFFu;
+constexpr uint32_t NOT_FOUND = 0xFFFFFFFFu;
+
+// Thrown inside the library, turned into a status + message at the C ABI.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define GEM_HIP_CHECK(call)                                                              \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw ::gemd::Error{GEM_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+// Launch-error check after a kernel launch.
+#define GEM_LAUNCH_CHECK(name)                                                            \
+  do {                                                                                    \
+    hipError_t e_ = hipGetLastError();                                                    \
+    if (e_ != hipSuccess)                                                                 \
+      throw ::gemd::Error{GEM_E_HIP, std::string("launch ") + name + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+// Stream-ordered device allocation (hipMallocAsync on the context stream; the
+// default pool keeps freed memory for reuse).
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  hipStream_t s = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), s(o.s) { o.p = nullptr; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p;
+      bytes = o.bytes;
+      s = o.s;
+      o.p = nullptr;
+    }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void alloc(size_t n, hipStream_t stream) {
+    reset();
+    s = stream;
+    bytes = n;
+    if (n == 0) return;
+    hipError_t e = hipMallocAsync(&p, n, stream);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error{GEM_E_NOMEM, "hipMallocAsync(" + std::to_string(n) + " bytes): " + hipGetErrorString(e)};
+    }
+  }
+  void reset() {
+    if (p) (void)hipFreeAsync(p, s);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+// Data graph resident in HBM.  Labels are replaced by dense label ids
+// (first-seen order over the vertex array) held as u8 when the graph has at
+// most 255 distinct labels, else u32.
+struct GraphDev {
+  bool loaded = false;
+  uint32_t n = 0;
+  uint64_t slots = 0;
+  uint64_t max_deg = 0;
+  DevBuf off;     // u64[n + 1]
+  DevBuf nbr;     // u32[slots]
+  DevBuf lab;     // u8[n] or u32[n] dense label ids
+  bool lab8 = true;
+  std::unordered_map<uint32_t, uint32_t> label_id;  // original -> dense
+  std::vector<uint64_t> label_freq;                 // dense id -> #vertices
+};
+
+// Query description consumed by the filter kernel (device copy).
+struct FilterQuery {
+  uint32_t nv, nslots, T, W;
+  uint32_t qdeg[GEM_MAX_QV];
+  uint32_t group[GEM_MAX_QV];   // slot -> mask of query vertices with that label
+  uint32_t nreq[GEM_MAX_QV];
+  uint8_t req_slot[GEM_MAX_QV][GEM_MAX_QV];
+  uint8_t req_cnt[GEM_MAX_QV][GEM_MAX_QV];
+};
+
+struct TableDev {
+  bool present = false;
+  uint32_t key_u = 0, val_u = 0;
+  uint64_t size = 0;
+  DevBuf off;  // u64[|Can(key)| + 1]
+  DevBuf val;  // u32[size], positions in Can(val_u)
+};
+
+struct QueryDev {
+  bool filtered = false, tabled = false;
+  gemh::QueryInfo Q;
+  uint32_t W = 0;                      // 64-bit words per candidate bitmap
+  std::vector<uint32_t> ncan;          // |Can(u)|
+  std::vector<uint64_t> can_off;       // start of Can(u) inside `can`
+  DevBuf can;                          // u32 candidate lists (sorted)
+  DevBuf bitmap;                       // u64[nv * W]
+  DevBuf rank;                         // u32[nv * (W + 1)]: exclusive popcount prefix
+  DevBuf fq;                           // FilterQuery
+  DevBuf slot_of;                      // u8[num data labels]: dense label -> query slot
+  std::vector<TableDev> tab;           // per directed edge
+};
+
+}  // namespace gemd
+
+struct gem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  std::string err;
+  gemd::GraphDev g;
+  gemd::QueryDev q;
+  gem_phase_times times{};
+  uint64_t level_budget_bytes = 1ull << 31;  // per join level buffer
+  void *pinned = nullptr;                   // small pinned staging area
+  size_t pinned_bytes = 0;
+};
+
+namespace gemd {
+
+// Pinned staging for small device->host reads.
+template <class T>
+T *pinned(gem_ctx *c, size_t n) {
+  size_t need = n * sizeof(T);
+  if (need > c->pinned_bytes) {
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    size_t sz = std::max<size_t>(need, 1 << 20);
+    GEM_HIP_CHECK(hipHostMalloc(&c->pinned, sz, hipHostMallocDefault));
+    c->pinned_bytes = sz;
+  }
+  return reinterpret_cast<T *>(c->pinned);
+}
+
+// ---------------------------------------------------------------- phases
+void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T);
+void run_tables(gem_ctx *c);
+void run_position_lookup(gem_ctx *c, uint32_t u, const uint32_t *h_vertices, uint64_t n, uint32_t *h_pos);
+
+struct JoinOut {
+  uint64_t count = 0;
+  bool saturated = false;
+  uint64_t emb_written = 0;
+};
+void run_join(gem_ctx *c, const uint32_t *order, uint64_t cap, JoinOut *out, gem_step_log *log, uint32_t *emb_out,
+              uint64_t emb_capacity);
+
+// ------------------------------------------------------------------ scans
+// Exclusive scan of n u64 values in place-free form: out[i] = sum(in[0..i)),
+// out[n] = total.  Returns nothing; caller reads out[n] if needed.
+void scan_u64(gem_ctx *c, const uint64_t *d_in, uint64_t *d_out, uint64_t n);
+// Segmented exclusive scan of u32 counts: nseg segments of length len each,
+// out has len + 1 entries per segment (last = segment total).
+void scan_segments_u32(gem_ctx *c, const uint32_t *d_in, uint32_t *d_out, uint32_t nseg, uint64_t len);
+
+}  // namespace gemd

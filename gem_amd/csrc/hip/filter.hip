@@ -1,0 +1,276 @@
+// Candidate filter and position index.
+//
+// Predicate (out-of-core/lib/filter/src/filter.cu:160-240; in-core
+// filter_kernel.cu:198-265): v is a candidate of query vertex u iff
+//   lab(v) == lab(u),  deg(u) <= deg(v) + T,
+//   sum over query labels l of max(0, nlc_u[l] - nlc_v[l]) <= T,
+// where nlc_x[l] counts neighbours of x with label l, over labels present in
+// the query.
+//
+// Layout: one wave owns a tile of 64 consecutive data vertices (lane = vertex)
+// and writes, for every query vertex u, one 64-bit word of u's candidate bitmap
+// (ballot of the lanes that pass) plus that word's popcount.  A per-u scan of
+// the popcounts gives rank[u][tile] = #candidates before the tile, so
+//   pos_u(v) = rank[u][v/64] + popcount(bitmap[u][v/64] & below(v % 64))
+// equals lower_bound(Can(u), v): the value the reference's cuckoo hash returns
+// (unifiedTrie.cuh:84-125, unifiedTrie.cu:54-101), in one cache line, with no
+// build retries.  Can(u) is then compacted from the bitmap in ascending order.
+//
+// Neighbour-label counts: a vertex of degree <= 64 is counted by its own lane
+// into a private LDS column (label-major, lane-minor: conflict-free); a
+// higher-degree vertex is counted by the whole wave, 64 neighbours per step,
+// one ballot + popcount per query label (no atomics).
+#include "ctx.h"
+#include "wave.h"
+
+namespace gemd {
+
+constexpr uint32_t FILTER_WAVES = 4;
+constexpr uint32_t LIGHT_DEG = 64;
+
+template <class LabT>
+__global__ void __launch_bounds__(FILTER_WAVES * 64)
+k_filter(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr, const LabT *__restrict__ lab, uint32_t n,
+         const FilterQuery *__restrict__ fqp, const uint8_t *__restrict__ slot_of, uint64_t *__restrict__ bitmap,
+         uint32_t *__restrict__ tile_count) {
+  __shared__ FilterQuery fq;
+  __shared__ uint32_t hist[FILTER_WAVES][GEM_MAX_QV][64];
+  __shared__ uint32_t heavy_cnt[FILTER_WAVES][GEM_MAX_QV];
+  {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(fqp);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&fq);
+    for (uint32_t i = threadIdx.x; i < sizeof(FilterQuery) / 4; i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const uint32_t tile = blockIdx.x * FILTER_WAVES + w;
+  if (tile >= fq.W) return;  // whole wave; no block barrier follows
+  const uint64_t v = (uint64_t)tile * 64 + l;
+  const uint32_t nslots = fq.nslots, T = fq.T;
+  uint32_t (*h)[64] = hist[w];
+
+  uint32_t slot = NO_SLOT;
+  uint64_t o0 = 0, deg = 0;
+  uint32_t cmask = 0;
+  if (v < n) {
+    slot = slot_of[lab[v]];
+    if (slot != NO_SLOT) {
+      o0 = off[v];
+      deg = off[v + 1] - o0;
+      uint32_t g = fq.group[slot];
+      while (g) {
+        uint32_t u = __builtin_ctz(g);
+        g &= g - 1;
+        if ((uint64_t)fq.qdeg[u] <= deg + T) cmask |= 1u << u;
+      }
+    }
+  }
+  for (uint32_t s = 0; s < nslots; ++s) h[s][l] = 0;
+  uint32_t pass = 0;
+  // light vertices: own lane, private LDS column
+  if (cmask && deg <= LIGHT_DEG) {
+    for (uint64_t j = 0; j < deg; ++j) {
+      uint32_t s = slot_of[lab[nbr[o0 + j]]];
+      if (s != NO_SLOT) h[s][l] += 1;
+    }
+    uint32_t g = cmask;
+    while (g) {
+      uint32_t u = __builtin_ctz(g);
+      g &= g - 1;
+      uint32_t deficit = 0;
+      for (uint32_t r = 0; r < fq.nreq[u]; ++r) {
+        uint32_t need = fq.req_cnt[u][r], have = h[fq.req_slot[u][r]][l];
+        deficit += need > have ? need - have : 0;
+      }
+      if (deficit <= T) pass |= 1u << u;
+    }
+  }
+  // heavy vertices: whole wave, ballot counting
+  uint64_t heavy = __ballot(cmask != 0 && deg > LIGHT_DEG);
+  while (heavy) {
+    const uint32_t hl = __ffsll((unsigned long long)heavy) - 1;
+    heavy &= heavy - 1;
+    const uint64_t ho0 = readlane_u64(o0, hl), hdeg = readlane_u64(deg, hl);
+    const uint32_t hmask = readlane_u32(cmask, hl);
+    uint32_t acc = 0;  // lane s holds the count of slot s
+    for (uint64_t j = 0; j < hdeg; j += 64) {
+      uint32_t s = NO_SLOT;
+      if (j + l < hdeg) s = slot_of[lab[nbr[ho0 + j + l]]];
+      for (uint32_t q = 0; q < nslots; ++q) {
+        uint32_t c = (uint32_t)__popcll(__ballot(s == q));
+        if (l == q) acc += c;
+      }
+    }
+    if (l < nslots) heavy_cnt[w][l] = acc;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // lane u decides query vertex u
+    bool ok = false;
+    if (l < fq.nv && (hmask >> l & 1u)) {
+      uint32_t deficit = 0;
+      for (uint32_t r = 0; r < fq.nreq[l]; ++r) {
+        uint32_t need = fq.req_cnt[l][r], have = heavy_cnt[w][fq.req_slot[l][r]];
+        deficit += need > have ? need - have : 0;
+      }
+      ok = deficit <= T;
+    }
+    uint32_t pm = (uint32_t)__ballot(ok);
+    if (l == hl) pass = pm;
+    __builtin_amdgcn_wave_barrier();
+  }
+  const uint32_t W = fq.W;
+  for (uint32_t u = 0; u < fq.nv; ++u) {
+    uint64_t m = __ballot((pass >> u) & 1u);
+    if (l == 0) {
+      bitmap[(uint64_t)u * W + tile] = m;
+      tile_count[(uint64_t)u * W + tile] = (uint32_t)__popcll(m);
+    }
+  }
+}
+
+// Compaction: Can(u) ascending from the bitmap and ranks.
+__global__ void __launch_bounds__(256) k_compact(const uint64_t *__restrict__ bitmap, const uint32_t *__restrict__ rank,
+                                               uint32_t nv, uint32_t W, const uint64_t *__restrict__ can_off,
+                                               uint32_t *__restrict__ can) {
+  const uint32_t tile = blockIdx.x * 4 + threadIdx.x / 64, l = threadIdx.x % 64;
+  if (tile >= W) return;
+  for (uint32_t u = 0; u < nv; ++u) {
+    const uint64_t m = bitmap[(uint64_t)u * W + tile];
+    if (!m) continue;
+    if (m >> l & 1ull) {
+      const uint32_t r = rank[(uint64_t)u * (W + 1) + tile] + rank_below(m);
+      can[can_off[u] + r] = tile * 64 + l;
+    }
+  }
+}
+
+__global__ void k_position_lookup(const uint64_t *__restrict__ bitmap, const uint32_t *__restrict__ rank, uint32_t W,
+                                  uint32_t n, uint32_t u, const uint32_t *__restrict__ vs, uint64_t cnt,
+                                  uint32_t *__restrict__ pos) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const uint32_t v = vs[i];
+  uint32_t p = NOT_FOUND;
+  if (v < n) {
+    const uint64_t m = bitmap[(uint64_t)u * W + (v >> 6)];
+    const uint32_t b = v & 63;
+    if (m >> b & 1ull) p = rank[(uint64_t)u * (W + 1) + (v >> 6)] + (uint32_t)__popcll(m & ((1ull << b) - 1));
+  }
+  pos[i] = p;
+}
+
+void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T) {
+  if (!c->g.loaded) throw Error{GEM_E_STATE, "no data graph loaded"};
+  QueryDev &Qd = c->q;
+  Qd = QueryDev();
+  std::string why;
+  int st = gemh::analyse_query(qv, &Qd.Q, &why);
+  if (st) throw Error{st, why};
+  const gemh::QueryInfo &Q = Qd.Q;
+  const GraphDev &G = c->g;
+  const uint32_t nv = Q.nv;
+  const uint32_t W = (G.n + 63) / 64;
+  Qd.W = W;
+
+  // query label slots in first-seen order (in-core/lib/io/src/io.cu:135-161)
+  FilterQuery fq{};
+  fq.nv = nv;
+  fq.T = T;
+  fq.W = W;
+  std::vector<uint32_t> slot(nv);
+  std::vector<uint32_t> slot_label;
+  for (uint32_t u = 0; u < nv; ++u) {
+    uint32_t s = 0;
+    while (s < slot_label.size() && slot_label[s] != Q.lab[u]) ++s;
+    if (s == slot_label.size()) slot_label.push_back(Q.lab[u]);
+    slot[u] = s;
+    fq.group[s] |= 1u << u;
+    fq.qdeg[u] = Q.deg[u];
+  }
+  fq.nslots = (uint32_t)slot_label.size();
+  for (uint32_t u = 0; u < nv; ++u) {
+    uint32_t cnt[GEM_MAX_QV] = {};
+    for (auto [w, e] : Q.adj[u]) cnt[slot[w]]++;
+    for (uint32_t s = 0; s < fq.nslots; ++s)
+      if (cnt[s]) {
+        fq.req_slot[u][fq.nreq[u]] = (uint8_t)s;
+        fq.req_cnt[u][fq.nreq[u]] = (uint8_t)cnt[s];
+        fq.nreq[u]++;
+      }
+  }
+  // dense data label -> query slot
+  const uint32_t nlab = (uint32_t)G.label_freq.size();
+  std::vector<uint8_t> slot_of(std::max<uint32_t>(nlab, 1), (uint8_t)NO_SLOT);
+  for (uint32_t s = 0; s < fq.nslots; ++s) {
+    auto it = G.label_id.find(slot_label[s]);
+    if (it != G.label_id.end()) slot_of[it->second] = (uint8_t)s;
+  }
+  // candidate storage: upper bound = frequency of u's label
+  Qd.can_off.assign(nv + 1, 0);
+  for (uint32_t u = 0; u < nv; ++u) {
+    auto it = G.label_id.find(Q.lab[u]);
+    uint64_t ub = it == G.label_id.end() ? 0 : G.label_freq[it->second];
+    Qd.can_off[u + 1] = Qd.can_off[u] + ub;
+  }
+
+  hipStream_t s = c->stream;
+  GEM_HIP_CHECK(hipEventRecord(c->ev[0], s));
+  Qd.fq.alloc(sizeof(FilterQuery), s);
+  Qd.slot_of.alloc(slot_of.size(), s);
+  GEM_HIP_CHECK(hipMemcpyAsync(Qd.fq.p, &fq, sizeof(fq), hipMemcpyHostToDevice, s));
+  GEM_HIP_CHECK(hipMemcpyAsync(Qd.slot_of.p, slot_of.data(), slot_of.size(), hipMemcpyHostToDevice, s));
+  Qd.bitmap.alloc((uint64_t)nv * W * 8, s);
+  DevBuf tile_count;
+  tile_count.alloc((uint64_t)nv * W * 4, s);
+  Qd.rank.alloc((uint64_t)nv * (W + 1) * 4, s);
+  Qd.can.alloc(std::max<uint64_t>(Qd.can_off[nv], 1) * 4, s);
+  DevBuf can_off;
+  can_off.alloc((nv + 1) * 8, s);
+  GEM_HIP_CHECK(hipMemcpyAsync(can_off.p, Qd.can_off.data(), (nv + 1) * 8, hipMemcpyHostToDevice, s));
+
+  const uint32_t blocks = (W + FILTER_WAVES - 1) / FILTER_WAVES;
+  if (G.lab8)
+    k_filter<uint8_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(
+        G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint8_t>(), G.n, Qd.fq.as<FilterQuery>(),
+        Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<uint64_t>(), tile_count.as<uint32_t>());
+  else
+    k_filter<uint32_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(
+        G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint32_t>(), G.n, Qd.fq.as<FilterQuery>(),
+        Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<uint64_t>(), tile_count.as<uint32_t>());
+  GEM_LAUNCH_CHECK("k_filter");
+  scan_segments_u32(c, tile_count.as<uint32_t>(), Qd.rank.as<uint32_t>(), nv, W);
+  k_compact<<<(W + 3) / 4, 256, 0, s>>>(Qd.bitmap.as<uint64_t>(), Qd.rank.as<uint32_t>(), nv, W,
+                                        can_off.as<uint64_t>(), Qd.can.as<uint32_t>());
+  GEM_LAUNCH_CHECK("k_compact");
+  // |Can(u)| = rank[u][W]
+  uint32_t *h = pinned<uint32_t>(c, nv);
+  for (uint32_t u = 0; u < nv; ++u)
+    GEM_HIP_CHECK(hipMemcpyAsync(h + u, Qd.rank.as<uint32_t>() + (uint64_t)u * (W + 1) + W, 4, hipMemcpyDeviceToHost, s));
+  GEM_HIP_CHECK(hipEventRecord(c->ev[1], s));
+  GEM_HIP_CHECK(hipStreamSynchronize(s));
+  Qd.ncan.assign(h, h + nv);
+  float ms = 0;
+  GEM_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->times.filter_ms = ms;
+  c->times.index_ms = 0;  // the index (bitmap + rank) is a by-product of the filter
+  Qd.filtered = true;
+}
+
+void run_position_lookup(gem_ctx *c, uint32_t u, const uint32_t *h_vertices, uint64_t n, uint32_t *h_pos) {
+  QueryDev &Qd = c->q;
+  if (!Qd.filtered) throw Error{GEM_E_STATE, "filter has not run"};
+  if (u >= Qd.Q.nv) throw Error{GEM_E_INVALID, "query vertex out of range"};
+  if (n == 0) return;
+  hipStream_t s = c->stream;
+  DevBuf dv, dp;
+  dv.alloc(n * 4, s);
+  dp.alloc(n * 4, s);
+  GEM_HIP_CHECK(hipMemcpyAsync(dv.p, h_vertices, n * 4, hipMemcpyHostToDevice, s));
+  k_position_lookup<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(Qd.bitmap.as<uint64_t>(), Qd.rank.as<uint32_t>(), Qd.W,
+                                                                c->g.n, u, dv.as<uint32_t>(), n, dp.as<uint32_t>());
+  GEM_LAUNCH_CHECK("k_position_lookup");
+  GEM_HIP_CHECK(hipMemcpyAsync(h_pos, dp.p, n * 4, hipMemcpyDeviceToHost, s));
+  GEM_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace gemd

@@ -1,0 +1,204 @@
+// Edge-candidate tables (the reference's construct_edge_candidates,
+// in-core/lib/structure/src/unifiedTrie.cu:1020-1391 with kernels
+// construct_kernel.cu:1183-1340; CPU form out-of-core/.../unifiedTrie.cu:1224-1427).
+//
+// For every query vertex u that is not a leaf and every query neighbour u_j of
+// u, table T_{u->u_j} has one row per v in Can(u) (row i <-> Can(u)[i]) holding
+// the neighbours w of v with w in Can(u_j), in adjacency order.  Values are
+// stored as positions of w in Can(u_j) (pos = rank in u_j's candidate bitmap),
+// so join rows carry positions and never need a hash probe.
+//
+// One pass over adjacency serves all of u's tables: a wave takes 64
+// consecutive candidates of one u, flattens their adjacency lists into one
+// item range (wave scan of degrees) and walks it 64 items per step; item ->
+// row by binary lifting over the 64 row starts in LDS; membership of w in each
+// Can(u_j) is one bit of u_j's bitmap (L2-resident); per-row counts and
+// in-row ranks come from ballots restricted to the row's lane segment, with
+// one carried count for the row that spans two steps.  Count pass -> per-table
+// exclusive scan -> write pass.
+#include "ctx.h"
+#include "wave.h"
+
+namespace gemd {
+
+constexpr uint32_t TAB_WAVES = 4;
+
+struct TableJob {
+  uint32_t nu;
+  uint32_t W;
+  uint32_t wave_start[GEM_MAX_QV + 1];
+  uint32_t ncan[GEM_MAX_QV];
+  uint64_t can_off[GEM_MAX_QV];
+  uint32_t nd[GEM_MAX_QV];    // tables keyed on this u
+  uint32_t tab0[GEM_MAX_QV];  // first entry in the TableDesc list
+};
+
+struct TableDesc {
+  uint32_t val_u;
+  uint64_t *off;
+  uint32_t *val;
+};
+
+__device__ __forceinline__ uint64_t lane_range(uint32_t a, uint32_t b) {  // bits [a, b], a <= b < 64
+  const uint64_t hi = (b >= 63) ? ~0ull : ((1ull << (b + 1)) - 1);
+  return hi & ~((1ull << a) - 1);
+}
+
+template <bool WRITE>
+__global__ void __launch_bounds__(TAB_WAVES * 64)
+k_tables(const uint64_t *__restrict__ goff, const uint32_t *__restrict__ gnbr, const uint32_t *__restrict__ can,
+         const uint64_t *__restrict__ bitmap, const uint32_t *__restrict__ rank, const TableJob job,
+         const TableDesc *__restrict__ desc) {
+  __shared__ uint32_t st[TAB_WAVES][65];
+  __shared__ uint64_t ob[TAB_WAVES][64];
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const uint32_t gw = blockIdx.x * TAB_WAVES + w;
+  if (gw >= job.wave_start[job.nu]) return;
+  uint32_t ju = 0;
+  while (job.wave_start[ju + 1] <= gw) ++ju;
+  const uint32_t row_base = (gw - job.wave_start[ju]) * 64;
+  const uint32_t ncan = job.ncan[ju];
+  const uint32_t i = row_base + l;
+  const bool valid = i < ncan;
+  uint64_t o0 = 0;
+  uint32_t len = 0;
+  if (valid) {
+    const uint32_t v = can[job.can_off[ju] + i];
+    o0 = goff[v];
+    len = (uint32_t)(goff[v + 1] - o0);
+  }
+  const uint32_t incl = wave_incl_scan_u32(len);
+  const uint32_t total = readlane_u32(incl, 63);
+  st[w][l] = incl - len;
+  ob[w][l] = o0;
+  if (l == 63) st[w][64] = total;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const uint32_t nd = job.nd[ju], t0 = job.tab0[ju];
+  if (!WRITE && valid && len == 0)
+    for (uint32_t j = 0; j < nd; ++j) desc[t0 + j].off[i] = 0;
+  const uint32_t W = job.W;
+  uint32_t carry_reg = 0;  // lane j: hits of the row carried into the next step, table j
+  for (uint32_t t = 0; t < total; t += 64) {
+    const uint32_t item = t + l;
+    const bool iv = item < total;
+    const uint32_t row = iv ? seg_of(st[w], item) : 63;
+    const uint32_t rs = st[w][row], re = st[w][row + 1];
+    const uint32_t wv = iv ? gnbr[ob[w][row] + (item - rs)] : 0;
+    const uint32_t last = min(63u, total - 1 - t);
+    const uint32_t row0 = readlane_u32(row, 0);
+    const uint32_t seg_lo = rs > t ? rs - t : 0;
+    const uint32_t seg_hi = (re < t + 64 ? re - t : 64) - 1;
+    const bool row_done = re <= t + 64;
+    const uint64_t seg_mask = iv ? lane_range(seg_lo, seg_hi) : 0;
+    const uint64_t below_mask = iv ? (lane_range(seg_lo, l) & ~(1ull << l)) : 0;
+    for (uint32_t j = 0; j < nd; ++j) {
+      const TableDesc d = desc[t0 + j];
+      const uint32_t vu = d.val_u;
+      bool hit = false;
+      uint64_t word = 0;
+      if (iv) {
+        word = bitmap[(uint64_t)vu * W + (wv >> 6)];
+        hit = (word >> (wv & 63)) & 1ull;
+      }
+      const uint64_t m = __ballot(hit);
+      const uint32_t carry = readlane_u32(carry_reg, j);
+      const uint32_t before = (row == row0) ? carry : 0;
+      const uint32_t tot_row = before + (uint32_t)__popcll(m & seg_mask);
+      if (WRITE) {
+        if (hit) {
+          const uint32_t r = before + (uint32_t)__popcll(m & below_mask);
+          const uint32_t p = rank[(uint64_t)vu * (W + 1) + (wv >> 6)] +
+                             (uint32_t)__popcll(word & ((1ull << (wv & 63)) - 1));
+          d.val[d.off[row_base + row] + r] = p;
+        }
+      } else {
+        if (iv && l == seg_hi && row_done) d.off[row_base + row] = tot_row;
+      }
+      const uint32_t carry_next = readlane_u32(row_done ? 0u : tot_row, last);
+      if (l == j) carry_reg = carry_next;
+    }
+  }
+}
+
+void run_tables(gem_ctx *c) {
+  QueryDev &Qd = c->q;
+  if (!Qd.filtered) throw Error{GEM_E_STATE, "filter has not run"};
+  const gemh::QueryInfo &Q = Qd.Q;
+  const GraphDev &G = c->g;
+  hipStream_t s = c->stream;
+  GEM_HIP_CHECK(hipEventRecord(c->ev[2], s));
+  Qd.tab.clear();
+  Qd.tab.resize(2 * Q.ne);
+  TableJob job{};
+  job.W = Qd.W;
+  std::vector<uint32_t> desc_tab;  // TableDesc index -> directed edge
+  uint32_t waves = 0;
+  for (uint32_t u = 0; u < Q.nv; ++u) {
+    if (Q.deg[u] == 1) continue;  // leaves key no table (unifiedTrie.cu:1045,1095)
+    const uint32_t ju = job.nu++;
+    job.wave_start[ju] = waves;
+    job.ncan[ju] = Qd.ncan[u];
+    job.can_off[ju] = Qd.can_off[u];
+    job.tab0[ju] = (uint32_t)desc_tab.size();
+    for (auto [un, e] : Q.adj[u]) {
+      const uint32_t d = 2 * e + (Q.edges[e].first == u ? 0 : 1);
+      TableDev &T = Qd.tab[d];
+      T.present = true;
+      T.key_u = u;
+      T.val_u = un;
+      T.off.alloc(((uint64_t)Qd.ncan[u] + 1) * 8, s);
+      desc_tab.push_back(d);
+      job.nd[ju]++;
+    }
+    waves += (Qd.ncan[u] + 63) / 64;
+  }
+  job.wave_start[job.nu] = waves;
+  std::vector<TableDesc> desc(desc_tab.size());
+  auto fill_desc = [&]() {
+    for (size_t k = 0; k < desc_tab.size(); ++k) {
+      TableDev &T = Qd.tab[desc_tab[k]];
+      desc[k] = TableDesc{T.val_u, T.off.as<uint64_t>(), T.val.as<uint32_t>()};
+    }
+  };
+  fill_desc();
+  DevBuf ddesc;
+  ddesc.alloc(std::max<size_t>(desc.size(), 1) * sizeof(TableDesc), s);
+  GEM_HIP_CHECK(hipMemcpyAsync(ddesc.p, desc.data(), desc.size() * sizeof(TableDesc), hipMemcpyHostToDevice, s));
+  const uint32_t blocks = (waves + TAB_WAVES - 1) / TAB_WAVES;
+  if (blocks) {
+    k_tables<false><<<blocks, TAB_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(),
+                                                      Qd.bitmap.as<uint64_t>(), Qd.rank.as<uint32_t>(), job,
+                                                      ddesc.as<TableDesc>());
+    GEM_LAUNCH_CHECK("k_tables<count>");
+  }
+  uint64_t *h = pinned<uint64_t>(c, std::max<size_t>(desc_tab.size(), 1));
+  for (size_t k = 0; k < desc_tab.size(); ++k) {
+    TableDev &T = Qd.tab[desc_tab[k]];
+    const uint64_t nrow = Qd.ncan[T.key_u];
+    scan_u64(c, T.off.as<uint64_t>(), T.off.as<uint64_t>(), nrow);
+    GEM_HIP_CHECK(hipMemcpyAsync(h + k, T.off.as<uint64_t>() + nrow, 8, hipMemcpyDeviceToHost, s));
+  }
+  GEM_HIP_CHECK(hipStreamSynchronize(s));
+  for (size_t k = 0; k < desc_tab.size(); ++k) {
+    TableDev &T = Qd.tab[desc_tab[k]];
+    T.size = h[k];
+    T.val.alloc(std::max<uint64_t>(T.size, 1) * 4, s);
+  }
+  fill_desc();
+  GEM_HIP_CHECK(hipMemcpyAsync(ddesc.p, desc.data(), desc.size() * sizeof(TableDesc), hipMemcpyHostToDevice, s));
+  if (blocks) {
+    k_tables<true><<<blocks, TAB_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(),
+                                                     Qd.bitmap.as<uint64_t>(), Qd.rank.as<uint32_t>(), job,
+                                                     ddesc.as<TableDesc>());
+    GEM_LAUNCH_CHECK("k_tables<write>");
+  }
+  GEM_HIP_CHECK(hipEventRecord(c->ev[3], s));
+  GEM_HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0;
+  GEM_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+  c->times.tables_ms = ms;
+  Qd.tabled = true;
+}
+
+}  // namespace gemd

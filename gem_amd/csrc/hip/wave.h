@@ -1,0 +1,86 @@
+// Wave64 building blocks for gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gemd {
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// popcount of the bits of `m` below this lane (v_mbcnt_lo/hi)
+__device__ __forceinline__ uint32_t rank_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t x, uint32_t lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)lane);
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t x, uint32_t lane) {
+  uint32_t lo = readlane_u32((uint32_t)x, lane), hi = readlane_u32((uint32_t)(x >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Inclusive wave scan (Hillis-Steele over 64 lanes).
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x) {
+  const uint32_t l = lane_id();
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    uint32_t y = __shfl_up(x, d, 64);
+    if (l >= d) x += y;
+  }
+  return x;
+}
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t x) {
+  const uint32_t l = lane_id();
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    uint64_t y = __shfl_up(x, d, 64);
+    if (l >= d) x += y;
+  }
+  return x;
+}
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+#pragma unroll
+  for (uint32_t d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
+// Largest r in [0, 64) with starts[r] <= item (starts ascending, starts[0] == 0).
+__device__ __forceinline__ uint32_t seg_of(const uint32_t *starts, uint32_t item) {
+  uint32_t lo = 0;
+#pragma unroll
+  for (uint32_t step = 32; step >= 1; step >>= 1)
+    if (starts[lo + step] <= item && lo + step < 64) lo += step;
+  return lo;
+}
+
+// Binary search for v in sorted a[0..n); true if present.
+template <class T>
+__device__ __forceinline__ bool sorted_contains(const T *a, uint64_t n, T v) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    T x = a[mid];
+    if (x < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < n && a[lo] == v;
+}
+
+// Map a hardware workgroup id to a logical one so that consecutive logical
+// blocks run on the same XCD (workgroups are dispatched round-robin over the
+// 8 XCDs); neighbouring row blocks then share one L2.
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t hw, uint32_t nblocks) {
+  constexpr uint32_t X = 8;
+  const uint32_t per = nblocks / X;
+  if (per == 0) return hw;
+  const uint32_t full = per * X;
+  if (hw >= full) return hw;
+  return (hw % X) * per + hw / X;
+}
+
+}  // namespace gemd

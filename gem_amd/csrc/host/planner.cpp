@@ -1,0 +1,237 @@
+// GEM ordering at threshold 0 without the edge-subset lattice.
+//
+// The reference scores candidates in find_best_edge_candidate
+// (in-core/lib/structure/src/unifiedTrie.cu:939-1018) over the lattice of
+// connected edge subsets built by Lattice::construct_linked_list_extend
+// (in-core/lib/structure/src/lattice.cpp:41-185).  At threshold 0 the only
+// relaxed query is the full query, so for every single edge the contribution
+// value (lattice.cpp:443-467) is c = 1 / (1 + 0.1 (|E|-1)), and a single-edge
+// node's lattice parents are exactly its connected 2-edge supersets.
+//
+// What remains of the lattice is the order in which the reference scans the
+// candidates of a "next edge" step: ascending lattice id.  Ids are handed out
+// breadth-first from the full edge set, a node getting its id from the first
+// (smallest-id) parent that reaches it, parents expanding their removable
+// edges in ascending index.  By induction on the level, a node's id order
+// within its level is the lexicographic order of its lexicographically
+// smallest removal sequence (remove edges of E \ N one at a time, keeping the
+// remainder connected), and that sequence is produced greedily: always remove
+// the smallest-index edge whose removal keeps the remainder connected (some
+// such edge exists while the remainder strictly contains a connected N).  So
+// ties are broken by comparing greedy sequences; no lattice is built, and the
+// order is defined for queries far beyond the reference's 32-edge bitset.
+#include "planner.h"
+
+#include <algorithm>
+#include <numeric>
+
+namespace gemh {
+
+int analyse_query(const gem_query_view *q, QueryInfo *out, std::string *why) {
+  auto fail = [&](int code, const std::string &m) {
+    if (why) *why = m;
+    return code;
+  };
+  if (!q || !out || !q->labels || (q->num_e && !q->edges)) return fail(GEM_E_INVALID, "null query");
+  if (q->num_v > GEM_MAX_QV) return fail(GEM_E_LIMIT, "query has more than GEM_MAX_QV vertices");
+  if (q->num_e > GEM_MAX_QE) return fail(GEM_E_LIMIT, "query has more than GEM_MAX_QE edges");
+  if (q->num_e < 2) return fail(GEM_E_QUERY, "query needs at least 2 edges");
+  QueryInfo &Q = *out;
+  Q.nv = q->num_v;
+  Q.ne = q->num_e;
+  Q.lab.assign(q->labels, q->labels + q->num_v);
+  Q.edges.resize(Q.ne);
+  Q.deg.assign(Q.nv, 0);
+  Q.adj.assign(Q.nv, {});
+  std::vector<uint64_t> keys;
+  for (uint32_t i = 0; i < Q.ne; ++i) {
+    uint32_t s = q->edges[2 * i], d = q->edges[2 * i + 1];
+    if (s >= Q.nv || d >= Q.nv) return fail(GEM_E_INVALID, "edge endpoint out of range");
+    if (s == d) return fail(GEM_E_QUERY, "self loop in query");
+    if (s > d) std::swap(s, d);
+    Q.edges[i] = {s, d};
+    Q.deg[s]++;
+    Q.deg[d]++;
+    Q.adj[s].push_back({d, i});
+    Q.adj[d].push_back({s, i});
+    keys.push_back(((uint64_t)s << 32) | d);
+  }
+  std::sort(keys.begin(), keys.end());
+  if (std::adjacent_find(keys.begin(), keys.end()) != keys.end()) return fail(GEM_E_QUERY, "repeated query edge");
+  EdgeSet all;
+  for (uint32_t i = 0; i < Q.ne; ++i) all.set(i);
+  for (uint32_t v = 0; v < Q.nv; ++v)
+    if (Q.deg[v] == 0) return fail(GEM_E_QUERY, "isolated query vertex");
+  if (!edges_connected(Q, all)) return fail(GEM_E_QUERY, "query is disconnected");
+  return GEM_OK;
+}
+
+bool edges_connected(const QueryInfo &q, const EdgeSet &s) {
+  uint32_t parent[GEM_MAX_QV];
+  for (uint32_t v = 0; v < q.nv; ++v) parent[v] = v;
+  auto find = [&](uint32_t x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+  };
+  int first = -1;
+  for (uint32_t i = 0; i < q.ne; ++i) {
+    if (!s[i]) continue;
+    if (first < 0) first = (int)i;
+    uint32_t a = find(q.edges[i].first), b = find(q.edges[i].second);
+    if (a != b) parent[a] = b;
+  }
+  if (first < 0) return false;
+  uint32_t r = find(q.edges[first].first);
+  for (uint32_t i = 0; i < q.ne; ++i)
+    if (s[i] && find(q.edges[i].first) != r) return false;
+  return true;
+}
+
+// Greedy (lexicographically smallest) removal sequence from the full edge set
+// down to `target`.
+static std::vector<uint32_t> removal_sequence(const QueryInfo &q, const EdgeSet &target) {
+  EdgeSet cur;
+  for (uint32_t i = 0; i < q.ne; ++i) cur.set(i);
+  std::vector<uint32_t> seq;
+  while (cur != target) {
+    bool removed = false;
+    for (uint32_t f = 0; f < q.ne; ++f) {
+      if (!cur[f] || target[f]) continue;
+      EdgeSet t = cur;
+      t.reset(f);
+      if (edges_connected(q, t)) {
+        seq.push_back(f);
+        cur = t;
+        removed = true;
+        break;
+      }
+    }
+    if (!removed) break;  // unreachable for a connected target
+  }
+  return seq;
+}
+
+static bool touches(const QueryInfo &q, const EdgeSet &cur, uint32_t e) {
+  // edge e shares a vertex with the edge-induced subgraph of cur
+  uint32_t a = q.edges[e].first, b = q.edges[e].second;
+  for (auto [n, f] : q.adj[a])
+    if (cur[f]) return true;
+  for (auto [n, f] : q.adj[b])
+    if (cur[f]) return true;
+  (void)a;
+  return false;
+}
+
+int gem_order(const QueryInfo &q, const uint64_t *sz, uint32_t *order) {
+  // lattice.cpp:457-463: decay of the single full-query target at distance |E|-1
+  const double contribution = 0.0 + 1.0 / (1.0 + 0.1 * (int)(q.ne - 1));
+  // First edge (unifiedTrie.cu:946-973): strict '>' over edge index order.
+  double best = -1.0;
+  uint32_t first = UINT32_MAX;
+  for (uint32_t i = 0; i < q.ne; ++i) {
+    const double size_factor = 1.0 / (1.0 + (double)sz[i] + 1);
+    const uint64_t supersets = (uint64_t)(q.deg[q.edges[i].first] - 1) + (q.deg[q.edges[i].second] - 1);
+    const double connectivity_factor = (double)supersets * 0.1;
+    const double score = 0.6 * size_factor + 0.3 * contribution + 0.1 * connectivity_factor;
+    if (score > best) {
+      best = score;
+      first = i;
+    }
+  }
+  EdgeSet cur;
+  cur.set(first);
+  order[0] = first;
+  for (uint32_t step = 1; step < q.ne; ++step) {
+    // Next edge (unifiedTrie.cu:978-1016): candidates are the lattice parents of
+    // the current node, scanned in ascending lattice id with strict '>'.
+    std::vector<uint32_t> cands;
+    std::vector<double> scores;
+    double mx = -1.0;
+    for (uint32_t e = 0; e < q.ne; ++e) {
+      if (cur[e] || !touches(q, cur, e)) continue;
+      const double size_factor = 1.0 / (1.0 + (double)sz[e] + 1);
+      const double score = 0.4 * size_factor + 0.4 * contribution;
+      cands.push_back(e);
+      scores.push_back(score);
+      mx = std::max(mx, score);
+    }
+    if (cands.empty()) return GEM_E_QUERY;
+    uint32_t pick = UINT32_MAX;
+    std::vector<uint32_t> pick_seq;
+    for (size_t i = 0; i < cands.size(); ++i) {
+      if (scores[i] != mx) continue;
+      EdgeSet next = cur;
+      next.set(cands[i]);
+      std::vector<uint32_t> seq = removal_sequence(q, next);
+      if (pick == UINT32_MAX || seq < pick_seq) {
+        pick = cands[i];
+        pick_seq.swap(seq);
+      }
+    }
+    order[step] = pick;
+    cur.set(pick);
+  }
+  return GEM_OK;
+}
+
+bool order_is_connected(const QueryInfo &q, const uint32_t *order) {
+  EdgeSet cur;
+  for (uint32_t i = 0; i < q.ne; ++i) {
+    uint32_t e = order[i];
+    if (e >= q.ne || cur[e]) return false;
+    if (i > 0 && !touches(q, cur, e)) return false;
+    cur.set(e);
+  }
+  return true;
+}
+
+int make_join_plan(const QueryInfo &q, const uint32_t *order, JoinPlan *plan) {
+  if (!order_is_connected(q, order)) return GEM_E_INVALID;
+  JoinPlan &P = *plan;
+  std::vector<int> col_of(q.nv, -1);
+  const uint32_t e0 = order[0];
+  const uint32_t s = q.edges[e0].first, d = q.edges[e0].second;
+  const bool key_small = q.deg[s] != 1;
+  const uint32_t a = key_small ? s : d, b = key_small ? d : s;
+  P.first_edge = e0;
+  P.first_directed = 2 * e0 + (key_small ? 0 : 1);
+  P.col_u = {a, b};
+  col_of[a] = 0;
+  col_of[b] = 1;
+  P.steps.clear();
+  for (uint32_t i = 1; i < q.ne; ++i) {
+    const uint32_t e = order[i];
+    const uint32_t x = q.edges[e].first, y = q.edges[e].second;  // x < y
+    JoinStep st;
+    st.edge = e;
+    if (col_of[x] >= 0 && col_of[y] >= 0) {
+      st.select = true;
+      st.key_col = (uint32_t)std::min(col_of[x], col_of[y]);
+      st.val_col = (uint32_t)std::max(col_of[x], col_of[y]);
+      st.directed = 2 * e + (P.col_u[st.key_col] == x ? 0 : 1);
+    } else {
+      const bool fwd = col_of[x] >= 0;  // key on the present endpoint
+      const uint32_t present = fwd ? x : y, fresh = fwd ? y : x;
+      st.select = false;
+      st.key_col = (uint32_t)col_of[present];
+      st.new_u = fresh;
+      st.directed = 2 * e + (fwd ? 0 : 1);
+      for (uint32_t c = 0; c < P.col_u.size(); ++c)
+        if (q.lab[P.col_u[c]] == q.lab[fresh]) st.dup_cols[st.num_dup++] = c;
+      col_of[fresh] = (int)P.col_u.size();
+      P.col_u.push_back(fresh);
+    }
+    P.steps.push_back(st);
+  }
+  return GEM_OK;
+}
+
+}  // namespace gemh
+
+extern "C" int gem_plan_gem_order(const gem_query_view *q, const uint64_t *table_sizes, uint32_t *order_out) {
+  if (!table_sizes || !order_out) return GEM_E_INVALID;
+  gemh::QueryInfo Q;
+  int st = gemh::analyse_query(q, &Q, nullptr);
+  if (st) return st;
+  return gemh::gem_order(Q, table_sizes, order_out);
+}

@@ -1,0 +1,66 @@
+// Query analysis, GEM ordering and the edge-at-a-time join plan (host C++).
+#pragma once
+
+#include <bitset>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gem_hip.h"
+
+namespace gemh {
+
+using EdgeSet = std::bitset<GEM_MAX_QE>;
+
+struct QueryInfo {
+  uint32_t nv = 0, ne = 0;
+  std::vector<uint32_t> lab;                          // original labels
+  std::vector<std::pair<uint32_t, uint32_t>> edges;   // (src, dst), src < dst
+  std::vector<uint32_t> deg;
+  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> adj;  // (nbr, edge)
+};
+
+// Validate and normalise a query view.  Rejects (GEM_E_QUERY) queries that are
+// disconnected, have fewer than 2 edges (the reference computes nothing for
+// them, in-core/lib/plan/src/relax.cpp:89), self loops, repeated edges, or
+// exceed GEM_MAX_QV / GEM_MAX_QE.
+int analyse_query(const gem_query_view *q, QueryInfo *out, std::string *why);
+
+// Is the edge-induced subgraph of `s` connected?  (cpuGraph::check_connectivity,
+// in-core/lib/graph/src/cpuGraph.cpp:74-194.)
+bool edges_connected(const QueryInfo &q, const EdgeSet &s);
+
+// GEM order; see gem_plan_gem_order in include/gem_hip.h.
+int gem_order(const QueryInfo &q, const uint64_t *table_sizes, uint32_t *order);
+
+// Is `order` a permutation of the edges whose every prefix is connected?
+bool order_is_connected(const QueryInfo &q, const uint32_t *order);
+
+struct JoinStep {
+  uint32_t edge = 0;
+  bool select = false;
+  uint32_t directed = 0;   // directed edge id of the table used (2e or 2e+1)
+  uint32_t key_col = 0;    // column whose position keys the table
+  uint32_t val_col = 0;    // select: column tested for membership
+  uint32_t new_u = 0;      // expand: query vertex of the new column
+  uint32_t num_dup = 0;    // expand: columns with the new vertex's label
+  uint32_t dup_cols[GEM_MAX_QV] = {};
+};
+
+struct JoinPlan {
+  uint32_t first_edge = 0;
+  uint32_t first_directed = 0;  // table that seeds the rows: a -> b
+  std::vector<uint32_t> col_u;  // query vertex of each column, in join order
+  std::vector<JoinStep> steps;  // steps after the first edge
+};
+
+// Column/step bookkeeping of the reference's process()
+// (in-core/lib/join/src/join_trie.cu:1025-1085): the first edge's rows come
+// from the table keyed on its smaller endpoint unless that endpoint is a leaf
+// (in-core/lib/structure/src/unifiedTrie.cu:1095-1191); an edge whose both
+// endpoints are columns is a select keyed on the earlier column; otherwise an
+// expansion keyed on the present endpoint, with an injectivity check against
+// the columns of equal label (join_trie.cu:450-462).
+int make_join_plan(const QueryInfo &q, const uint32_t *order, JoinPlan *plan);
+
+}  // namespace gemh

@@ -1,0 +1,165 @@
+"""Host graphs and queries (numpy views over the C library's host CSR).
+
+Mirrors the reference's loader layer (in-core/lib/io/src/io.cu:45-220,
+in-core/lib/graph/include/cpuGraph.h:12-67): data graphs are undirected,
+labelled, CSR with sorted adjacency; queries are small labelled graphs whose
+edge i is the reference's edge-tag bit i.
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import GEM_MAX_QE, GEM_MAX_QV, GemError, QueryView, CsrView, load
+
+
+def _check(st, what):
+    if st != 0:
+        raise GemError(st, what)
+
+
+class Graph:
+    """A host CSR graph owned by the C library."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        lib = load()
+        self.num_vertices = int(lib.gem_graph_num_vertices(self._h))
+        self.num_slots = int(lib.gem_graph_num_slots(self._h))
+        n, m = self.num_vertices, self.num_slots
+        self.offsets = np.ctypeslib.as_array(lib.gem_graph_offsets(self._h), shape=(n + 1,))
+        self.neighbors = (np.ctypeslib.as_array(lib.gem_graph_neighbors(self._h), shape=(m,))
+                          if m else np.zeros(0, np.uint32))
+        self.labels = np.ctypeslib.as_array(lib.gem_graph_labels(self._h), shape=(n,))
+
+    @property
+    def num_edges(self):
+        return self.num_slots // 2
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            load().gem_graph_free(h)
+            self._h = None
+
+    # ---- constructors
+    @classmethod
+    def erdos_renyi(cls, n, m, num_labels, seed):
+        h = C.c_void_p()
+        _check(load().gem_graph_generate_er(n, m, num_labels, seed, C.byref(h)), "gem_graph_generate_er")
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale, avg_degree, num_labels, seed, a=0.57, b=0.19, c=0.19, zipf=True):
+        h = C.c_void_p()
+        _check(load().gem_graph_generate_rmat(scale, avg_degree, a, b, c, num_labels, 1 if zipf else 0, seed,
+                                              C.byref(h)), "gem_graph_generate_rmat")
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, offsets, neighbors, labels):
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        nbr = np.ascontiguousarray(neighbors, dtype=np.uint32)
+        lab = np.ascontiguousarray(labels, dtype=np.uint32)
+        h = C.c_void_p()
+        _check(load().gem_graph_from_csr(len(lab), off.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         nbr.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         lab.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(h)),
+               "gem_graph_from_csr")
+        return cls(h)
+
+    @classmethod
+    def from_edges(cls, n, edges, labels):
+        """Build from an undirected edge list (pairs); duplicates are an error."""
+        e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+        src = np.concatenate([e[:, 0], e[:, 1]])
+        dst = np.concatenate([e[:, 1], e[:, 0]])
+        order = np.lexsort((dst, src))
+        src, dst = src[order], dst[order]
+        off = np.zeros(n + 1, np.uint64)
+        np.add.at(off, src + 1, 1)
+        off = np.cumsum(off).astype(np.uint64)
+        return cls.from_csr(off, dst.astype(np.uint32), labels)
+
+    @classmethod
+    def load(cls, path):
+        h = C.c_void_p()
+        _check(load().gem_graph_load(str(path).encode(), C.byref(h)), f"gem_graph_load({path})")
+        return cls(h)
+
+    def save(self, path):
+        _check(load().gem_graph_save(self._h, str(path).encode()), f"gem_graph_save({path})")
+
+    def csr_view(self):
+        return CsrView(self.num_vertices, self.num_slots,
+                       self.offsets.ctypes.data_as(C.POINTER(C.c_uint64)),
+                       self.neighbors.ctypes.data_as(C.POINTER(C.c_uint32)),
+                       self.labels.ctypes.data_as(C.POINTER(C.c_uint32)))
+
+    def degrees(self):
+        return np.diff(self.offsets).astype(np.int64)
+
+
+@dataclass
+class Query:
+    """num_v vertices with original labels; edges (src < dst) in edge-tag order."""
+    labels: np.ndarray
+    edges: np.ndarray  # shape (num_e, 2)
+
+    def __post_init__(self):
+        self.labels = np.ascontiguousarray(self.labels, dtype=np.uint32)
+        self.edges = np.ascontiguousarray(np.asarray(self.edges, dtype=np.uint32).reshape(-1, 2))
+
+    @property
+    def num_v(self):
+        return len(self.labels)
+
+    @property
+    def num_e(self):
+        return len(self.edges)
+
+    @property
+    def avg_degree(self):
+        return 2.0 * self.num_e / self.num_v
+
+    @property
+    def dense(self):
+        return self.avg_degree >= 3.0
+
+    def view(self):
+        return QueryView(self.num_v, self.num_e, self.labels.ctypes.data_as(C.POINTER(C.c_uint32)),
+                         self.edges.ctypes.data_as(C.POINTER(C.c_uint32)))
+
+    @classmethod
+    def random_walk(cls, graph, k, seed, max_e=GEM_MAX_QE):
+        lab = np.zeros(k, np.uint32)
+        edg = np.zeros(2 * max_e, np.uint32)
+        ne = C.c_uint32()
+        _check(load().gem_query_random_walk(graph._h, k, seed, lab.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                            edg.ctypes.data_as(C.POINTER(C.c_uint32)), max_e, C.byref(ne)),
+               "gem_query_random_walk")
+        return cls(lab, edg[:2 * ne.value].reshape(-1, 2))
+
+    @classmethod
+    def load(cls, path):
+        lab = np.zeros(GEM_MAX_QV, np.uint32)
+        edg = np.zeros(2 * GEM_MAX_QE, np.uint32)
+        nv, ne = C.c_uint32(), C.c_uint32()
+        _check(load().gem_query_load(str(path).encode(), GEM_MAX_QV, GEM_MAX_QE, C.byref(nv),
+                                     lab.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(ne),
+                                     edg.ctypes.data_as(C.POINTER(C.c_uint32))), f"gem_query_load({path})")
+        return cls(lab[:nv.value], edg[:2 * ne.value].reshape(-1, 2))
+
+    def save(self, path):
+        v = self.view()
+        _check(load().gem_query_save(C.byref(v), str(path).encode()), f"gem_query_save({path})")
+
+
+def gem_order(query, edge_sizes):
+    """GEM join order (edge indices) given |T_e| per undirected edge."""
+    sz = np.ascontiguousarray(edge_sizes, dtype=np.uint64)
+    out = np.zeros(query.num_e, np.uint32)
+    v = query.view()
+    _check(load().gem_plan_gem_order(C.byref(v), sz.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                     out.ctypes.data_as(C.POINTER(C.c_uint32))), "gem_plan_gem_order")
+    return out

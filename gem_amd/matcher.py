@@ -1,0 +1,138 @@
+"""GPU matcher: the reference's filter -> index -> tables -> join sequence
+(in-core/src/main.cu:148-236) driven through the C ABI of libgem_hip.so.
+
+Every call runs HIP kernels; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import GemError, JoinResult, PhaseTimes, StepLog, load
+
+NOT_FOUND = 0xFFFFFFFF
+
+
+def _p32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _p64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+class GemContext:
+    """One device context (one per host thread), holding the uploaded data
+    graph and the current query's candidates and edge tables."""
+
+    def __init__(self, device=0):
+        self._lib = load()
+        h = C.c_void_p()
+        st = self._lib.gem_hip_create(device, C.byref(h))
+        if st != 0:
+            raise GemError(st, f"gem_hip_create(device={device})")
+        self._h = h
+        self.graph = None
+        self.query = None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.gem_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st, what):
+        if st != 0:
+            msg = self._lib.gem_hip_last_error(self._h)
+            raise GemError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def set_level_budget(self, nbytes):
+        self._check(self._lib.gem_hip_set_level_budget(self._h, int(nbytes)), "gem_hip_set_level_budget")
+
+    # ---- reference phases
+    def load_data(self, graph):
+        v = graph.csr_view()
+        self._check(self._lib.gem_hip_load_data_csr(self._h, C.byref(v)), "gem_hip_load_data_csr")
+        self.graph = graph
+
+    def filter(self, query, threshold=0, want_lists=True):
+        """filterSSM: returns |Can(u)| and (optionally) the sorted lists."""
+        self.query = query
+        num = np.zeros(query.num_v, np.uint32)
+        v = query.view()
+        if not want_lists:
+            self._check(self._lib.gem_hip_filter(self._h, C.byref(v), threshold, _p32(num), None, 0),
+                        "gem_hip_filter")
+            return num, None
+        # upper bound on list length: the data graph's label frequency
+        freq = {}
+        labs, cnts = np.unique(self.graph.labels, return_counts=True)
+        for lab, cnt in zip(labs, cnts):
+            freq[int(lab)] = int(cnt)
+        pitch = max([freq.get(int(l), 0) for l in query.labels] + [1])
+        buf = np.zeros((query.num_v, pitch), np.uint32)
+        self._check(self._lib.gem_hip_filter(self._h, C.byref(v), threshold, _p32(num), _p32(buf), pitch),
+                    "gem_hip_filter")
+        return num, [buf[u, :num[u]].copy() for u in range(query.num_v)]
+
+    def position_lookup(self, u, vertices):
+        vs = np.ascontiguousarray(vertices, dtype=np.uint32)
+        out = np.zeros(len(vs), np.uint32)
+        self._check(self._lib.gem_hip_position_lookup(self._h, u, _p32(vs), len(vs), _p32(out)),
+                    "gem_hip_position_lookup")
+        return out
+
+    def build_edge_tables(self):
+        sizes = np.zeros(2 * self.query.num_e, np.uint64)
+        self._check(self._lib.gem_hip_build_edge_tables(self._h, _p64(sizes)), "gem_hip_build_edge_tables")
+        return sizes
+
+    def read_edge_table(self, d, num_rows, size):
+        off = np.zeros(num_rows + 1, np.uint64)
+        val = np.zeros(max(int(size), 1), np.uint32)
+        self._check(self._lib.gem_hip_read_edge_table(self._h, d, _p64(off), _p32(val)), "gem_hip_read_edge_table")
+        return off, val[:int(size)]
+
+    def join(self, order, cap=0, log=False, embeddings_capacity=0):
+        order = np.ascontiguousarray(order, dtype=np.uint32)
+        res = JoinResult()
+        lg = StepLog() if log else None
+        emb = (np.zeros(int(embeddings_capacity) * self.query.num_v, np.uint32)
+               if embeddings_capacity else None)
+        self._check(self._lib.gem_hip_join(self._h, _p32(order), cap, C.byref(res),
+                                           C.byref(lg) if lg is not None else None,
+                                           _p32(emb) if emb is not None else None,
+                                           int(embeddings_capacity)), "gem_hip_join")
+        out = {"count": int(res.count), "saturated": bool(res.saturated), "num_steps": int(res.num_steps)}
+        if lg is not None:
+            n = self.query.num_e
+            out["step_rows"] = np.array(lg.rows[:n], dtype=np.uint64)
+            out["step_ms"] = np.array(lg.ms[:n], dtype=np.float32)
+            out["step_select"] = np.array(lg.is_select[:n], dtype=bool)
+            out["step_edge"] = np.array(lg.edge[:n], dtype=np.uint32)
+        if emb is not None:
+            w = int(res.num_embeddings_written)
+            out["embeddings"] = emb[:w * self.query.num_v].reshape(w, self.query.num_v) if w else None
+        return out
+
+    def phase_times(self):
+        pt = PhaseTimes()
+        self._check(self._lib.gem_hip_phase_times(self._h, C.byref(pt)), "gem_hip_phase_times")
+        return {"filter_ms": pt.filter_ms, "index_ms": pt.index_ms, "tables_ms": pt.tables_ms,
+                "join_ms": pt.join_ms}
+
+    def match(self, query, cap=0):
+        """Whole plain pipeline with GEM order; returns (count, saturated, order)."""
+        self.query = query
+        res = JoinResult()
+        order = np.zeros(query.num_e, np.uint32)
+        v = query.view()
+        self._check(self._lib.gem_hip_match(self._h, C.byref(v), cap, C.byref(res), _p32(order)), "gem_hip_match")
+        return int(res.count), bool(res.saturated), order

@@ -1,0 +1,223 @@
+/*
+ * gem_hip.h — C ABI of the MI355X (gfx950) subgraph-matching library.
+ *
+ * The library enumerates/counts the embeddings of a labelled query graph in a
+ * labelled data graph (injective, label- and edge-preserving maps; threshold 0)
+ * with GEM's edge-centric pipeline:
+ *
+ *   data CSR -> candidate filter -> position index -> edge-candidate tables
+ *            -> GEM-ordered join (expand / select) -> uint64 count
+ *
+ * Every entry point returns an int status (GEM_OK == 0) and never exits the
+ * process; the message of the last failure on a context is available from
+ * gem_hip_last_error().  Host arrays are caller-owned and are copied by the
+ * library; device memory is owned by the opaque gem_ctx.  One context per host
+ * thread.  No torch or HIP types appear in these signatures.
+ *
+ * Reference interfaces replaced (paths relative to the Seungsu-J/GEM checkout):
+ *   in-core/lib/io/include/io.h:7-27            readGraphToCPUStandard / copyGraphToGPU
+ *   in-core/lib/filter/include/filter.h:10-14   filterSSM
+ *   in-core/lib/structure/include/unifiedTrie.cuh:58-61,244-251
+ *                                               HashTable::build, initVertexTries,
+ *                                               construct_edge_candidates(_cpu)
+ *   in-core/lib/structure/src/unifiedTrie.cu:939-1018
+ *                                               find_best_edge_candidate (GEM order)
+ *   in-core/lib/join/include/join_trie.cuh:31-32 join
+ *   in-core/src/main.cu:229-236                 per-query result readout
+ */
+#ifndef GEM_HIP_H
+#define GEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum gem_status {
+  GEM_OK = 0,
+  GEM_E_INVALID = 1, /* bad argument / malformed input                      */
+  GEM_E_HIP = 2,     /* HIP runtime error (message names the call)          */
+  GEM_E_NOMEM = 3,   /* device or host allocation failed                    */
+  GEM_E_QUERY = 4,   /* query unsupported: disconnected, < 2 edges, too big */
+  GEM_E_STATE = 5,   /* call out of order (e.g. join before filter)         */
+  GEM_E_IO = 6,      /* file could not be read or written                   */
+  GEM_E_LIMIT = 7    /* a documented size limit was exceeded                */
+};
+
+/* Query limits.  The reference caps queries at 16 vertices / 16-32 edges
+ * (in-core/lib/utils/include/defs.h:20-23); these limits are wider. */
+#define GEM_MAX_QV 32u
+#define GEM_MAX_QE 256u
+
+/* ------------------------------------------------------- host graph (CSR) */
+/* A simple undirected labelled graph held in host memory as CSR with sorted,
+ * duplicate-free adjacency and u64 offsets.  Replaces cpuGraph
+ * (in-core/lib/graph/include/cpuGraph.h:12-67) on the data side. */
+typedef struct gem_graph gem_graph;
+
+/* Erdos-Renyi G(n, m): m distinct unordered pairs, labels uniform in
+ * [0, num_labels).  Deterministic in seed. */
+int gem_graph_generate_er(uint32_t n, uint64_t m, uint32_t num_labels,
+                          uint64_t seed, gem_graph **out);
+/* R-MAT with 2^scale vertices and round(2^scale * avg_degree / 2) samples,
+ * quadrant probabilities (a, b, c, 1-a-b-c); symmetrised, self-loops and
+ * duplicates dropped.  label_dist: 0 = uniform, 1 = Zipf (s = 1). */
+int gem_graph_generate_rmat(uint32_t scale, double avg_degree, double a,
+                            double b, double c, uint32_t num_labels,
+                            uint32_t label_dist, uint64_t seed, gem_graph **out);
+/* Copy an existing CSR (offsets[n+1], neighbors[offsets[n]], labels[n]).
+ * Adjacency must be sorted ascending, without self-loops or duplicates, and
+ * symmetric; GEM_E_INVALID otherwise. */
+int gem_graph_from_csr(uint32_t n, const uint64_t *offsets,
+                       const uint32_t *neighbors, const uint32_t *labels,
+                       gem_graph **out);
+/* Reference text format: "t N M" / "v id label degree" / "e src dst"
+ * (in-core/lib/io/src/io.cu:45-220).  Declared degrees must match. */
+int gem_graph_load(const char *path, gem_graph **out);
+int gem_graph_save(const gem_graph *g, const char *path);
+uint32_t gem_graph_num_vertices(const gem_graph *g);
+uint64_t gem_graph_num_slots(const gem_graph *g); /* = 2 |E| */
+const uint64_t *gem_graph_offsets(const gem_graph *g);
+const uint32_t *gem_graph_neighbors(const gem_graph *g);
+const uint32_t *gem_graph_labels(const gem_graph *g);
+void gem_graph_free(gem_graph *g);
+
+/* ------------------------------------------------------------------- query */
+/* A query: num_v vertices with ORIGINAL labels, num_e undirected edges given as
+ * (src, dst) pairs in file order with src < dst.  Edge i is the reference's
+ * edge-tag bit i; directed edge 2i is src->dst and 2i+1 is dst->src
+ * (in-core/lib/io/src/io.cu:176-199). */
+typedef struct gem_query_view {
+  uint32_t num_v;
+  uint32_t num_e;
+  const uint32_t *labels; /* [num_v] */
+  const uint32_t *edges;  /* [2 * num_e] */
+} gem_query_view;
+
+/* Random-walk query: walk from a seeded start until k distinct vertices are
+ * visited; the query is the induced subgraph on them (vertex ids in visit
+ * order, labels copied, edges sorted).  Returns GEM_E_LIMIT when the induced
+ * subgraph has more than max_e edges. */
+int gem_query_random_walk(const gem_graph *g, uint32_t k, uint64_t seed,
+                          uint32_t *labels_out, uint32_t *edges_out,
+                          uint32_t max_e, uint32_t *num_e_out);
+int gem_query_load(const char *path, uint32_t max_v, uint32_t max_e,
+                   uint32_t *num_v, uint32_t *labels, uint32_t *num_e,
+                   uint32_t *edges);
+int gem_query_save(const gem_query_view *q, const char *path);
+
+/* ------------------------------------------------------------------- plan */
+/* GEM ordering at threshold 0 (in-core/lib/structure/src/unifiedTrie.cu:939-1018
+ * with the lattice of in-core/lib/structure/src/lattice.cpp:41-185, 322-467):
+ * first edge = argmax 0.6/(2+|T_e|) + 0.3c + 0.01 P_e (strict >, edge index
+ * order); then repeatedly the adjacent edge maximising 0.4/(2+|T_e|) + 0.4c,
+ * ties broken by the reference's lattice-id order.  table_sizes[i] = |T_e| of
+ * undirected edge i.  order_out[num_e] receives edge indices. */
+int gem_plan_gem_order(const gem_query_view *q, const uint64_t *table_sizes,
+                       uint32_t *order_out);
+
+/* -------------------------------------------------------- device context */
+typedef struct gem_ctx gem_ctx;
+
+int gem_hip_create(int device, gem_ctx **out);
+void gem_hip_destroy(gem_ctx *ctx);
+const char *gem_hip_last_error(const gem_ctx *ctx);
+/* Largest join level materialised at once (bytes, default 2 GiB, minimum
+ * 1 MiB); a larger level is split into row ranges finished depth-first.  No
+ * reference counterpart: the reference materialises every level whole and
+ * relies on its memory manager (in-core/lib/join/src/join_trie.cu:1087-1277). */
+int gem_hip_set_level_budget(gem_ctx *ctx, uint64_t bytes);
+
+/* Data graph view (plain host pointers) uploaded once per context.
+ * Replaces allocateMemGPU/copyGraphToGPU (in-core/lib/io/src/io.cu:348-452). */
+typedef struct gem_csr_view {
+  uint32_t num_v;
+  uint64_t num_slots;
+  const uint64_t *offsets;   /* [num_v + 1] */
+  const uint32_t *neighbors; /* [num_slots], sorted per vertex */
+  const uint32_t *labels;    /* [num_v], original labels */
+} gem_csr_view;
+int gem_hip_load_data_csr(gem_ctx *ctx, const gem_csr_view *g);
+
+/* Candidate filter for a new query (replaces filterSSM,
+ * in-core/lib/filter/src/filter.cu:34-167 and the CPU form
+ * out-of-core/lib/filter/src/filter.cu:61-295):
+ *   Can(u) = { v asc : lab(v) = lab(u), deg(v) + T >= deg(u),
+ *              sum_l max(0, nlc_u[l] - nlc_v[l]) <= T }.
+ * Makes `q` the context's current query.  h_num_can[num_v] receives |Can(u)|;
+ * if h_can is non-null, row u of h_can (pitch can_pitch) receives Can(u). */
+int gem_hip_filter(gem_ctx *ctx, const gem_query_view *q, uint32_t threshold,
+                   uint32_t *h_num_can, uint32_t *h_can, uint64_t can_pitch);
+
+/* Position index probe (the result the reference's cuckoo hash returns,
+ * in-core/lib/structure/include/unifiedTrie.cuh:84-125): pos_out[i] =
+ * index of vertices[i] in Can(u), or 0xFFFFFFFF when it is not a candidate. */
+int gem_hip_position_lookup(gem_ctx *ctx, uint32_t u, const uint32_t *vertices,
+                            uint64_t n, uint32_t *pos_out);
+
+/* Edge-candidate tables of the current query (replaces
+ * construct_edge_candidates, in-core/lib/structure/src/unifiedTrie.cu:1020-1391,
+ * and construct_edge_candidates_cpu, out-of-core/.../unifiedTrie.cu:1224-1427):
+ * for every directed query edge d = u->u' with deg_q(u) != 1, row i holds the
+ * neighbours w of Can(u)[i] that lie in Can(u'), in adjacency order, stored as
+ * their positions in Can(u').  h_table_sizes[2*num_e] receives |T_d| (0 for
+ * directions keyed on a leaf, which have no table). */
+int gem_hip_build_edge_tables(gem_ctx *ctx, uint64_t *h_table_sizes);
+
+/* Copy table d back: offsets[|Can(u)|+1] (u64), values[|T_d|] (positions in
+ * Can(u')).  For parity checks. */
+int gem_hip_read_edge_table(gem_ctx *ctx, uint32_t directed_edge,
+                            uint64_t *offsets, uint32_t *values);
+
+/* Result of one query. */
+typedef struct gem_join_result {
+  uint64_t count;      /* embeddings (exact when saturated == 0)         */
+  uint32_t saturated;  /* 1 when the count exceeded `cap` (count == cap) */
+  uint32_t num_steps;  /* join steps run (= num_e - 1)                   */
+  uint64_t num_embeddings_written; /* rows stored to emb_out            */
+} gem_join_result;
+
+/* Optional per-step log: rows after step s (s = 0 is the first edge table),
+ * time of the step in milliseconds, and whether it was a select. */
+typedef struct gem_step_log {
+  uint32_t edge[GEM_MAX_QE];
+  uint32_t is_select[GEM_MAX_QE];
+  uint64_t rows[GEM_MAX_QE];
+  float ms[GEM_MAX_QE];
+} gem_step_log;
+
+/* Join (replaces join(), in-core/lib/join/src/join_trie.cu:1087-1277) of the
+ * current query along edge_order[num_e] (any connected order; GEM order from
+ * gem_plan_gem_order reproduces the reference's plan).  The count stops at
+ * `cap` (0 = no cap): saturated = 1 iff the true count exceeds cap.  If
+ * emb_out is non-null and the count is <= emb_capacity, the embeddings are
+ * written as rows of num_v vertex ids (column u = image of query vertex u),
+ * in unspecified order. */
+int gem_hip_join(gem_ctx *ctx, const uint32_t *edge_order, uint64_t cap,
+                 gem_join_result *out, gem_step_log *log, uint32_t *emb_out,
+                 uint64_t emb_capacity);
+
+/* Phase timings (milliseconds, HIP events on the context stream) of the
+ * last filter / table build / join, mirroring the reference's printed phases
+ * (in-core/src/main.cu:147-227). */
+typedef struct gem_phase_times {
+  float filter_ms;
+  float index_ms;
+  float tables_ms;
+  float join_ms;
+} gem_phase_times;
+int gem_hip_phase_times(const gem_ctx *ctx, gem_phase_times *out);
+
+/* Whole plain pipeline for one query: filter -> index -> tables -> GEM order
+ * -> join.  Equivalent to the reference's main() for one query at -t 0. */
+int gem_hip_match(gem_ctx *ctx, const gem_query_view *q, uint64_t cap,
+                  gem_join_result *out, uint32_t *order_out);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* GEM_HIP_H */

@@ -1,0 +1,777 @@
+// ORACLE — test infrastructure only.  Nothing in the product (gem_amd/) links,
+// loads or calls this file; tests/, the smoke check and bench.py's CPU-baseline
+// leg use it as the checker.
+//
+// A C++17/OpenMP restatement, on the CPU, of the reference's (Seungsu-J/GEM)
+// plain subgraph-matching path at threshold 0.  Paths are relative to the
+// reference checkout.  Each function cites the code it restates:
+//
+//   orc_filter          out-of-core/lib/filter/src/filter.cu:61-295 (predicate
+//                       identical to in-core/lib/filter/src/filter_kernel.cu:198-265),
+//                       label remap of in-core/lib/io/src/io.cu:135-161
+//   orc_edge_table      out-of-core/lib/structure/src/unifiedTrie.cu:1224-1374
+//                       (= in-core construct_kernel.cu:1183-1300); position of a key
+//                       = closed form of the cuckoo lookup
+//                       (in-core/lib/structure/include/unifiedTrie.cuh:84-125,
+//                       unifiedTrie.cu:54-101): lower_bound(Can(u), v)
+//   orc_gem_order       the lattice of in-core/lib/structure/src/lattice.cpp:41-185,
+//                       322-467 built as the reference builds it (BFS ids), and
+//                       find_best_edge_candidate (unifiedTrie.cu:939-1018)
+//   orc_count           process()/host_select/direct_join/refer_join semantics
+//                       (in-core/lib/join/src/join_trie.cu:116-1085, driver 1087-1277):
+//                       first-edge rows, select keyed on the earlier column, expand
+//                       with injectivity against equal-label columns, count-only
+//                       last step; depth-first, OpenMP over first-table rows
+//   orc_enumerate       the same walk, materialising final rows
+//   orc_bruteforce      an independent backtracker on the data graph (no filter,
+//                       no tables): the definition of the count
+//
+// Pins (tests/test_oracle_*.py): closed-form counts (K_n, cycles, grids, stars,
+// bipartite, triangles), brute force on random small graphs, filter by
+// definition, table symmetry |T_{u->u'}| = |T_{u'->u}|, BFS/DFS step counts,
+// order invariance, lattice order vs the product's closed-form planner.
+#include <omp.h>
+
+#include <algorithm>
+#include <atomic>
+#include <bitset>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <queue>
+#include <set>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+constexpr uint32_t MAXV = 32;
+constexpr uint32_t MAXE = 256;
+using ESet = std::bitset<MAXE>;
+
+struct G {
+  uint32_t n;
+  const uint64_t *off;
+  const uint32_t *nbr;
+  const uint32_t *lab;
+  uint64_t deg(uint32_t v) const { return off[v + 1] - off[v]; }
+};
+
+struct Q {
+  uint32_t nv, ne;
+  std::vector<uint32_t> lab;  // original labels
+  std::vector<uint32_t> rlab; // remapped: first-seen order over vertices (io.cu:135-161)
+  uint32_t L = 0;             // number of distinct query labels
+  std::vector<std::pair<uint32_t, uint32_t>> e;  // (min, max)
+  std::vector<uint32_t> deg;
+  // query CSR in file edge order (io.cu:176-199)
+  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> adj;  // (nbr, directed eid)
+};
+
+Q make_q(uint32_t nv, uint32_t ne, const uint32_t *labels, const uint32_t *edges) {
+  Q q;
+  q.nv = nv;
+  q.ne = ne;
+  q.lab.assign(labels, labels + nv);
+  q.rlab.resize(nv);
+  std::map<uint32_t, uint32_t> remap;
+  for (uint32_t v = 0; v < nv; ++v) {
+    auto it = remap.find(labels[v]);
+    if (it == remap.end()) {
+      uint32_t id = (uint32_t)remap.size();
+      remap[labels[v]] = id;
+      q.rlab[v] = id;
+    } else {
+      q.rlab[v] = it->second;
+    }
+  }
+  q.L = (uint32_t)remap.size();
+  q.deg.assign(nv, 0);
+  q.adj.assign(nv, {});
+  for (uint32_t i = 0; i < ne; ++i) {
+    uint32_t s = edges[2 * i], d = edges[2 * i + 1];
+    if (s > d) std::swap(s, d);
+    q.e.push_back({s, d});
+    q.deg[s]++;
+    q.deg[d]++;
+    q.adj[s].push_back({d, 2 * i});
+    q.adj[d].push_back({s, 2 * i + 1});
+  }
+  return q;
+}
+
+// ------------------------------------------------------------------ filter
+// Returns Can(u) ascending for every u.
+std::vector<std::vector<uint32_t>> filter(const G &g, const Q &q, uint32_t T) {
+  // data label -> remapped query label, or L ("not in query")
+  std::unordered_map<uint32_t, uint32_t> remap;
+  for (uint32_t v = 0; v < q.nv; ++v) remap[q.lab[v]] = q.rlab[v];
+  std::vector<std::vector<uint32_t>> nlc_q(q.nv, std::vector<uint32_t>(q.L, 0));
+  for (uint32_t u = 0; u < q.nv; ++u)
+    for (auto [w, eid] : q.adj[u]) nlc_q[u][q.rlab[w]]++;
+  const int nchunks = std::max(1, omp_get_max_threads() * 4);
+  const uint64_t csz = ((uint64_t)g.n + nchunks - 1) / nchunks;
+  std::vector<std::vector<std::vector<uint32_t>>> part(nchunks, std::vector<std::vector<uint32_t>>(q.nv));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int c = 0; c < nchunks; ++c) {
+    std::vector<uint32_t> cnt(q.L + 1, 0);
+    const uint64_t vb = c * csz, ve = std::min<uint64_t>(g.n, vb + csz);
+    for (uint64_t v = vb; v < ve; ++v) {
+      auto it = remap.find(g.lab[v]);
+      if (it == remap.end()) continue;
+      const uint32_t dl = it->second;
+      std::fill(cnt.begin(), cnt.end(), 0);
+      for (uint64_t k = g.off[v]; k < g.off[v + 1]; ++k) {
+        auto jt = remap.find(g.lab[g.nbr[k]]);
+        if (jt != remap.end()) cnt[jt->second]++;
+      }
+      for (uint32_t u = 0; u < q.nv; ++u) {
+        if (q.rlab[u] != dl) continue;
+        if ((uint64_t)q.deg[u] > g.deg((uint32_t)v) + T) continue;
+        uint64_t diff = 0;
+        for (uint32_t l = 0; l < q.L; ++l)
+          if (nlc_q[u][l] > cnt[l]) diff += nlc_q[u][l] - cnt[l];
+        if (diff <= T) part[c][u].push_back((uint32_t)v);
+      }
+    }
+  }
+  std::vector<std::vector<uint32_t>> can(q.nv);
+  for (uint32_t u = 0; u < q.nv; ++u)
+    for (int c = 0; c < nchunks; ++c) can[u].insert(can[u].end(), part[c][u].begin(), part[c][u].end());
+  return can;
+}
+
+// ------------------------------------------------------------- edge tables
+struct Table {
+  bool present = false;
+  std::vector<uint64_t> off;   // |Can(u)| + 1
+  std::vector<uint32_t> val;   // vertex ids, adjacency order
+};
+
+// directed eid d: 2i = src->dst, 2i+1 = dst->src (src < dst)
+std::vector<Table> tables(const G &g, const Q &q, const std::vector<std::vector<uint32_t>> &can) {
+  std::vector<Table> t(2 * q.ne);
+  std::vector<std::vector<uint64_t>> bm(q.nv, std::vector<uint64_t>((g.n + 63) / 64, 0));
+  for (uint32_t u = 0; u < q.nv; ++u)
+    for (uint32_t v : can[u]) bm[u][v >> 6] |= 1ull << (v & 63);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int d = 0; d < (int)(2 * q.ne); ++d) {
+    const uint32_t i = d >> 1;
+    const uint32_t u = (d & 1) ? q.e[i].second : q.e[i].first;
+    const uint32_t un = (d & 1) ? q.e[i].first : q.e[i].second;
+    if (q.deg[u] == 1) continue;  // unifiedTrie.cu:1045,1095
+    Table &tb = t[d];
+    tb.present = true;
+    tb.off.assign(can[u].size() + 1, 0);
+    for (size_t r = 0; r < can[u].size(); ++r) {
+      const uint32_t v = can[u][r];
+      for (uint64_t k = g.off[v]; k < g.off[v + 1]; ++k) {
+        const uint32_t w = g.nbr[k];
+        if (bm[un][w >> 6] >> (w & 63) & 1) tb.val.push_back(w);
+      }
+      tb.off[r + 1] = tb.val.size();
+    }
+  }
+  return t;
+}
+
+uint32_t endpoint_key(const Q &q, uint32_t d) { return (d & 1) ? q.e[d >> 1].second : q.e[d >> 1].first; }
+uint32_t endpoint_val(const Q &q, uint32_t d) { return (d & 1) ? q.e[d >> 1].first : q.e[d >> 1].second; }
+
+// ---------------------------------------------------------------- lattice
+bool connected(const Q &q, const ESet &s) {
+  // BFS over the edge-induced subgraph (cpuGraph.cpp:74-194)
+  std::vector<int> vs;
+  std::vector<char> in(q.nv, 0), vis(q.nv, 0);
+  for (uint32_t i = 0; i < q.ne; ++i)
+    if (s[i]) in[q.e[i].first] = in[q.e[i].second] = 1;
+  int root = -1;
+  for (uint32_t v = 0; v < q.nv; ++v)
+    if (in[v]) {
+      root = (int)v;
+      break;
+    }
+  if (root < 0) return false;
+  std::queue<int> bq;
+  bq.push(root);
+  vis[root] = 1;
+  while (!bq.empty()) {
+    int x = bq.front();
+    bq.pop();
+    for (auto [w, eid] : q.adj[x])
+      if (s[eid >> 1] && !vis[w]) {
+        vis[w] = 1;
+        bq.push((int)w);
+      }
+  }
+  for (uint32_t v = 0; v < q.nv; ++v)
+    if (in[v] && !vis[v]) return false;
+  return true;
+}
+
+struct ESetHash {
+  size_t operator()(const ESet &s) const { return std::hash<ESet>()(s); }
+};
+
+// Build the reference's lattice ids and parent lists (lattice.cpp:41-185).
+struct Lattice {
+  std::unordered_map<ESet, int, ESetHash> id;
+  std::vector<ESet> et;
+  std::vector<std::vector<int>> parents;  // reversed_linked_list, sorted
+};
+
+int build_lattice(const Q &q, Lattice &L, uint64_t max_nodes) {
+  ESet full;
+  for (uint32_t i = 0; i < q.ne; ++i) full.set(i);
+  L.id[full] = 0;
+  L.et.push_back(full);
+  std::vector<int> level{0};
+  uint32_t lv = 0;
+  std::vector<std::vector<int>> children(1);
+  while (!level.empty()) {
+    std::vector<int> next;
+    std::unordered_map<ESet, int, ESetHash> next_seen;
+    if (lv + 1 == q.ne) break;  // single-edge level is final
+    for (int pid : level) {
+      const ESet cur = L.et[pid];
+      for (uint32_t i = 0; i < q.ne; ++i) {
+        if (!cur[i]) continue;
+        ESet t = cur;
+        t.flip(i);
+        auto it = next_seen.find(t);
+        if (it != next_seen.end()) {
+          children[pid].push_back(it->second);
+        } else if (connected(q, t)) {
+          int nid = (int)L.et.size();
+          L.id[t] = nid;
+          L.et.push_back(t);
+          children.push_back({});
+          next_seen[t] = nid;
+          next.push_back(nid);
+          children[pid].push_back(nid);
+          if (L.et.size() > max_nodes) return 1;
+        }
+      }
+    }
+    level.swap(next);
+    ++lv;
+  }
+  L.parents.assign(L.et.size(), {});
+  for (size_t p = 0; p < children.size(); ++p)
+    for (int c : children[p]) L.parents[c].push_back((int)p);
+  for (auto &v : L.parents) std::sort(v.begin(), v.end());
+  return 0;
+}
+
+int gem_order_lattice(const Q &q, const std::vector<uint64_t> &sz, uint32_t *order, uint64_t max_nodes) {
+  Lattice L;
+  if (build_lattice(q, L, max_nodes)) return 2;
+  ESet full;
+  for (uint32_t i = 0; i < q.ne; ++i) full.set(i);
+  // contribution (lattice.cpp:443-467): sum over uncomputed reachable relaxed
+  // queries (only the full query at T = 0) of 1 / (1 + 0.1 * distance)
+  auto contribution = [&](const ESet &s) {
+    double total = 0.0;
+    int distance = (int)(full ^ s).count();
+    total += 1.0 / (1.0 + 0.1 * distance);
+    return total;
+  };
+  double best = -1.0;
+  int best_e = -1;
+  for (uint32_t i = 0; i < q.ne; ++i) {
+    ESet s;
+    s.set(i);
+    int nid = L.id.at(s);
+    double size_factor = 1.0 / (1.0 + (double)sz[i] + 1);
+    double contribution_factor = contribution(s);
+    double connectivity_factor = L.parents[nid].size() * 0.1;
+    double score = 0.6 * size_factor + 0.3 * contribution_factor + 0.1 * connectivity_factor;
+    if (score > best) {
+      best = score;
+      best_e = (int)i;
+    }
+  }
+  ESet cur;
+  cur.set(best_e);
+  order[0] = (uint32_t)best_e;
+  for (uint32_t step = 1; step < q.ne; ++step) {
+    int cid = L.id.at(cur);
+    double bs = -1.0;
+    int pick = -1;
+    for (int nid : L.parents[cid]) {
+      ESet diff = L.et[nid] ^ cur;
+      if (diff.count() != 1) continue;
+      uint32_t e = (uint32_t)diff._Find_first();
+      double size_factor = 1.0 / (1.0 + (double)sz[e] + 1);
+      double score = 0.4 * size_factor + 0.4 * contribution(diff);
+      if (score > bs) {
+        bs = score;
+        pick = (int)e;
+      }
+    }
+    if (pick < 0) return 3;
+    order[step] = (uint32_t)pick;
+    cur.set(pick);
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------- join
+struct Step {
+  bool select;
+  uint32_t key_col, val_col, table;  // table = directed eid
+  uint32_t num_dup;
+  uint32_t dup[MAXV];
+};
+
+struct Plan {
+  uint32_t first_table;
+  std::vector<uint32_t> col_u;
+  std::vector<Step> steps;
+};
+
+// process() column bookkeeping (join_trie.cu:1025-1085)
+bool make_plan(const Q &q, const uint32_t *order, Plan &p) {
+  std::vector<int> col(q.nv, -1);
+  uint32_t e0 = order[0];
+  uint32_t s = q.e[e0].first, d = q.e[e0].second;
+  // the single-edge trie is created from the first non-leaf endpoint in vertex
+  // order (unifiedTrie.cu:1095-1191): the smaller one unless it is a leaf
+  bool small = q.deg[s] != 1;
+  p.first_table = 2 * e0 + (small ? 0 : 1);
+  p.col_u = {small ? s : d, small ? d : s};
+  col[p.col_u[0]] = 0;
+  col[p.col_u[1]] = 1;
+  ESet seen;
+  seen.set(e0);
+  for (uint32_t i = 1; i < q.ne; ++i) {
+    uint32_t e = order[i];
+    if (e >= q.ne || seen[e]) return false;
+    seen.set(e);
+    uint32_t x = q.e[e].first, y = q.e[e].second;
+    Step st{};
+    if (col[x] >= 0 && col[y] >= 0) {
+      st.select = true;
+      st.key_col = (uint32_t)std::min(col[x], col[y]);
+      st.val_col = (uint32_t)std::max(col[x], col[y]);
+      st.table = 2 * e + (p.col_u[st.key_col] == x ? 0 : 1);
+    } else if (col[x] >= 0 || col[y] >= 0) {
+      bool fwd = col[x] >= 0;
+      uint32_t pres = fwd ? x : y, fresh = fwd ? y : x;
+      st.select = false;
+      st.key_col = (uint32_t)col[pres];
+      st.table = 2 * e + (fwd ? 0 : 1);
+      st.num_dup = 0;
+      for (uint32_t c = 0; c < p.col_u.size(); ++c)
+        if (q.rlab[p.col_u[c]] == q.rlab[fresh]) st.dup[st.num_dup++] = c;
+      col[fresh] = (int)p.col_u.size();
+      p.col_u.push_back(fresh);
+    } else {
+      return false;  // disconnected prefix
+    }
+    p.steps.push_back(st);
+  }
+  return true;
+}
+
+struct Ctx {
+  const G *g;
+  const Q *q;
+  const std::vector<std::vector<uint32_t>> *can;
+  const std::vector<Table> *tab;
+  const Plan *plan;
+  uint64_t cap;
+  uint64_t work_limit;
+  std::atomic<uint64_t> *total;
+  std::atomic<uint64_t> *work;
+  std::atomic<int> *stop;
+};
+
+inline uint32_t pos_of(const std::vector<uint32_t> &c, uint32_t v) {
+  return (uint32_t)(std::lower_bound(c.begin(), c.end(), v) - c.begin());
+}
+
+inline bool in_list(const uint32_t *b, const uint32_t *e, uint32_t v) { return std::binary_search(b, e, v); }
+
+struct Walker {
+  const Ctx &C;
+  std::vector<uint64_t> steps;  // rows after step s (s = 0: first table)
+  uint32_t row[MAXV];
+  uint64_t local_final = 0;
+  uint64_t local_work = 0;
+  // enumeration sink (optional)
+  std::vector<uint32_t> *emb = nullptr;
+  uint64_t emb_cap = 0;
+  bool overflow = false;
+
+  explicit Walker(const Ctx &c) : C(c), steps(c.q->ne, 0) {}
+
+  const uint32_t *list(uint32_t table, uint32_t key_vertex, const uint32_t **end) {
+    const Table &t = (*C.tab)[table];
+    uint32_t ku = endpoint_key(*C.q, table);
+    uint32_t p = pos_of((*C.can)[ku], key_vertex);  // closed form of the hash probe
+    const uint32_t *b = t.val.data() + t.off[p];
+    *end = t.val.data() + t.off[p + 1];
+    return b;
+  }
+
+  void emit_final() {
+    if (!emb) return;
+    if (emb->size() / C.q->nv >= emb_cap) {
+      overflow = true;
+      return;
+    }
+    std::vector<uint32_t> r(C.q->nv);
+    for (uint32_t c = 0; c < C.plan->col_u.size(); ++c) r[C.plan->col_u[c]] = row[c];
+    emb->insert(emb->end(), r.begin(), r.end());
+  }
+
+  // rows at depth = s+1 columns known; apply step s
+  void go(uint32_t s, uint32_t ncols) {
+    const Plan &P = *C.plan;
+    if (s == P.steps.size()) {
+      emit_final();
+      return;
+    }
+    const Step &st = P.steps[s];
+    const bool last = (s + 1 == P.steps.size());
+    const uint32_t *e;
+    const uint32_t *b = list(st.table, row[st.key_col], &e);
+    if (st.select) {
+      if (in_list(b, e, row[st.val_col])) {
+        steps[s + 1]++;
+        if (last) {
+          local_final++;
+          if (local_final > (1u << 16)) flush_final();
+          emit_final();
+        } else {
+          go(s + 1, ncols);
+        }
+      }
+      return;
+    }
+    if (last && !emb) {
+      // count-only final expansion (join_trie.cu:1204 skip_final_materialization)
+      uint64_t n = (uint64_t)(e - b);
+      for (uint32_t k = 0; k < st.num_dup; ++k)
+        if (in_list(b, e, row[st.dup[k]])) n--;
+      steps[s + 1] += n;
+      local_final += n;
+      if (local_final > (1u << 16)) flush_final();
+      return;
+    }
+    for (const uint32_t *it = b; it != e; ++it) {
+      const uint32_t w = *it;
+      bool dup = false;
+      for (uint32_t k = 0; k < st.num_dup; ++k)
+        if (row[st.dup[k]] == w) {
+          dup = true;
+          break;
+        }
+      if (dup) continue;
+      steps[s + 1]++;
+      if (last) {
+        local_final++;
+        row[ncols] = w;
+        emit_final();
+        continue;
+      }
+      if (++local_work > 4096) flush_work();
+      row[ncols] = w;
+      go(s + 1, ncols + 1);
+      if (C.stop->load(std::memory_order_relaxed)) return;
+    }
+  }
+
+  void flush_work() {
+    if (C.work_limit) {
+      uint64_t w = C.work->fetch_add(local_work) + local_work;
+      if (w > C.work_limit) C.stop->store(2);
+    }
+    local_work = 0;
+  }
+
+  void flush_final() {
+    uint64_t t = C.total->fetch_add(local_final) + local_final;
+    local_final = 0;
+    if (C.cap && t > C.cap) C.stop->store(1);
+  }
+};
+
+struct Prepared {
+  Q q;
+  std::vector<std::vector<uint32_t>> can;
+  std::vector<Table> tab;
+  Plan plan;
+  double ms_filter = 0, ms_tables = 0;
+};
+
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); }
+
+// run the join; returns 0, or 2 when the work limit stopped it
+int run_join(const G &g, Prepared &pp, uint64_t cap, uint64_t work_limit, uint64_t *count, int *saturated,
+             uint64_t *step_rows, std::vector<uint32_t> *emb, uint64_t emb_cap, bool *emb_overflow) {
+  const Q &q = pp.q;
+  const Table &t0 = pp.tab[pp.plan.first_table];
+  const uint32_t a = pp.plan.col_u[0];
+  std::atomic<uint64_t> total{0}, work{0};
+  std::atomic<int> stop{0};
+  Ctx C{&g, &q, &pp.can, &pp.tab, &pp.plan, cap, work_limit, &total, &work, &stop};
+  std::vector<uint64_t> steps(q.ne, 0);
+  const int64_t nrows = (int64_t)t0.val.size();
+  bool overflow = false;
+  std::vector<std::vector<uint32_t>> parts;
+  const int nt = omp_get_max_threads();
+  parts.resize(emb ? nt : 0);
+#pragma omp parallel
+  {
+    Walker W(C);
+    if (emb) {
+      W.emb = &parts[omp_get_thread_num()];
+      W.emb_cap = emb_cap;
+    }
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t j = 0; j < nrows; ++j) {
+      if (stop.load(std::memory_order_relaxed)) continue;
+      // row j of the first table: key = Can(a)[r] with off[r] <= j < off[r+1]
+      const uint32_t r = (uint32_t)(std::upper_bound(t0.off.begin(), t0.off.end(), (uint64_t)j) - t0.off.begin() - 1);
+      W.row[0] = pp.can[a][r];
+      W.row[1] = t0.val[j];
+      W.steps[0]++;
+      if (pp.plan.steps.empty()) {
+        W.local_final++;
+        W.emit_final();
+      } else {
+        W.go(0, 2);
+      }
+      W.flush_final();
+      W.flush_work();
+    }
+#pragma omp critical
+    {
+      for (uint32_t s = 0; s < q.ne; ++s) steps[s] += W.steps[s];
+      overflow = overflow || W.overflow;
+    }
+  }
+  uint64_t tot = total.load();
+  int st = stop.load();
+  if (st == 2) return 2;
+  *saturated = (cap && tot > cap) ? 1 : 0;
+  *count = *saturated ? cap : tot;
+  if (step_rows)
+    for (uint32_t s = 0; s < q.ne; ++s) step_rows[s] = steps[s];
+  if (emb) {
+    for (auto &p : parts) emb->insert(emb->end(), p.begin(), p.end());
+    if (emb_overflow) *emb_overflow = overflow;
+  }
+  return 0;
+}
+
+int prepare(const G &g, uint32_t nv, uint32_t ne, const uint32_t *labels, const uint32_t *edges,
+            const uint32_t *order, Prepared &pp, bool need_plan) {
+  if (nv > MAXV || ne > MAXE || ne < 1) return 1;
+  pp.q = make_q(nv, ne, labels, edges);
+  auto t = clk::now();
+  pp.can = filter(g, pp.q, 0);
+  pp.ms_filter = ms_since(t);
+  t = clk::now();
+  pp.tab = tables(g, pp.q, pp.can);
+  pp.ms_tables = ms_since(t);
+  if (need_plan && !make_plan(pp.q, order, pp.plan)) return 1;
+  return 0;
+}
+
+void set_threads(int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_max_threads() { return omp_get_max_threads(); }
+
+// Can(u): num_can[nv]; can (nullable) rows of length pitch
+int orc_filter(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *lab, uint32_t nv, uint32_t ne,
+               const uint32_t *qlab, const uint32_t *qedges, uint32_t T, uint32_t *num_can, uint32_t *can,
+               uint64_t pitch, int nthreads) {
+  set_threads(nthreads);
+  G g{n, off, nbr, lab};
+  Q q = make_q(nv, ne, qlab, qedges);
+  auto c = filter(g, q, T);
+  for (uint32_t u = 0; u < nv; ++u) {
+    num_can[u] = (uint32_t)c[u].size();
+    if (can) {
+      if (c[u].size() > pitch) return 1;
+      std::copy(c[u].begin(), c[u].end(), can + u * pitch);
+    }
+  }
+  return 0;
+}
+
+// sizes[2*ne] = |T_d| (0 when the key endpoint is a leaf)
+int orc_table_sizes(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *lab, uint32_t nv,
+                    uint32_t ne, const uint32_t *qlab, const uint32_t *qedges, uint64_t *sizes, int nthreads) {
+  set_threads(nthreads);
+  G g{n, off, nbr, lab};
+  Prepared pp;
+  if (prepare(g, nv, ne, qlab, qedges, nullptr, pp, false)) return 1;
+  for (uint32_t d = 0; d < 2 * ne; ++d) sizes[d] = pp.tab[d].present ? pp.tab[d].val.size() : 0;
+  return 0;
+}
+
+// One directed table: offsets[|Can(u)|+1], ids[|T|] (vertex ids) and
+// pos[|T|] (positions in Can(u')).  Returns 1 if the table is absent.
+int orc_edge_table(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *lab, uint32_t nv,
+                   uint32_t ne, const uint32_t *qlab, const uint32_t *qedges, uint32_t d, uint64_t *offsets,
+                   uint32_t *ids, uint32_t *pos, int nthreads) {
+  set_threads(nthreads);
+  G g{n, off, nbr, lab};
+  Prepared pp;
+  if (prepare(g, nv, ne, qlab, qedges, nullptr, pp, false)) return 2;
+  const Table &t = pp.tab[d];
+  if (!t.present) return 1;
+  std::copy(t.off.begin(), t.off.end(), offsets);
+  const auto &cv = pp.can[endpoint_val(pp.q, d)];
+  for (size_t i = 0; i < t.val.size(); ++i) {
+    if (ids) ids[i] = t.val[i];
+    if (pos) pos[i] = pos_of(cv, t.val[i]);
+  }
+  return 0;
+}
+
+// GEM order through the reference's lattice.  Returns 2 if the lattice has
+// more than max_nodes nodes.
+int orc_gem_order(uint32_t nv, uint32_t ne, const uint32_t *qedges, const uint64_t *edge_sizes, uint32_t *order,
+                  uint64_t max_nodes) {
+  std::vector<uint32_t> lab(nv, 0);
+  Q q = make_q(nv, ne, lab.data(), qedges);
+  std::vector<uint64_t> sz(edge_sizes, edge_sizes + ne);
+  return gem_order_lattice(q, sz, order, max_nodes ? max_nodes : 2000000);
+}
+
+// Count along `order` (nullable: GEM order via the lattice with table sizes
+// from the oracle's own tables).  step_rows[ne] (nullable): rows after each
+// step.  ms[3] (nullable): filter, tables, join milliseconds.  Returns 0, 1 on
+// bad input, 2 when work_limit intermediate rows were exceeded, 3 when the
+// lattice was too large to order.
+int orc_count(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *lab, uint32_t nv, uint32_t ne,
+              const uint32_t *qlab, const uint32_t *qedges, const uint32_t *order, uint64_t cap, uint64_t work_limit,
+              uint64_t *count, int *saturated, uint64_t *step_rows, uint32_t *order_used, double *ms, int nthreads) {
+  set_threads(nthreads);
+  G g{n, off, nbr, lab};
+  Prepared pp;
+  if (prepare(g, nv, ne, qlab, qedges, nullptr, pp, false)) return 1;
+  std::vector<uint32_t> ord(ne);
+  if (order) {
+    std::copy(order, order + ne, ord.begin());
+  } else {
+    std::vector<uint64_t> sz(ne);
+    for (uint32_t i = 0; i < ne; ++i)
+      sz[i] = pp.tab[2 * i].present ? pp.tab[2 * i].val.size() : pp.tab[2 * i + 1].val.size();
+    if (gem_order_lattice(pp.q, sz, ord.data(), 2000000)) return 3;
+  }
+  if (order_used) std::copy(ord.begin(), ord.end(), order_used);
+  if (!make_plan(pp.q, ord.data(), pp.plan)) return 1;
+  auto t = clk::now();
+  int r = run_join(g, pp, cap, work_limit, count, saturated, step_rows, nullptr, 0, nullptr);
+  if (ms) {
+    ms[0] = pp.ms_filter;
+    ms[1] = pp.ms_tables;
+    ms[2] = ms_since(t);
+  }
+  return r;
+}
+
+// Enumerate all embeddings along `order` into out (rows of nv vertex ids,
+// column u = image of query vertex u), sorted lexicographically.  Returns 4 if
+// more than capacity rows exist.
+int orc_enumerate(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *lab, uint32_t nv, uint32_t ne,
+                  const uint32_t *qlab, const uint32_t *qedges, const uint32_t *order, uint32_t *out,
+                  uint64_t capacity, uint64_t *num_rows, int nthreads) {
+  set_threads(nthreads);
+  G g{n, off, nbr, lab};
+  Prepared pp;
+  if (prepare(g, nv, ne, qlab, qedges, order, pp, true)) return 1;
+  std::vector<uint32_t> emb;
+  bool overflow = false;
+  uint64_t count = 0;
+  int sat = 0;
+  int r = run_join(g, pp, 0, 0, &count, &sat, nullptr, &emb, capacity, &overflow);
+  if (r) return r;
+  if (overflow) return 4;
+  const uint64_t rows = emb.size() / nv;
+  std::vector<uint64_t> idx(rows);
+  for (uint64_t i = 0; i < rows; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](uint64_t x, uint64_t y) {
+    return std::lexicographical_compare(emb.begin() + x * nv, emb.begin() + (x + 1) * nv, emb.begin() + y * nv,
+                                        emb.begin() + (y + 1) * nv);
+  });
+  for (uint64_t i = 0; i < rows; ++i) std::copy(emb.begin() + idx[i] * nv, emb.begin() + (idx[i] + 1) * nv, out + i * nv);
+  *num_rows = rows;
+  return 0;
+}
+
+// Independent backtracker: counts injective label- and edge-preserving maps
+// directly on the data graph.  Query vertices are matched in BFS order from
+// vertex 0; each one is drawn from the neighbours of an earlier matched
+// neighbour.  Single-threaded; for small graphs.
+int orc_bruteforce(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *lab, uint32_t nv, uint32_t ne,
+                   const uint32_t *qlab, const uint32_t *qedges, uint64_t *count) {
+  std::vector<std::vector<uint32_t>> qa(nv);
+  for (uint32_t i = 0; i < ne; ++i) {
+    qa[qedges[2 * i]].push_back(qedges[2 * i + 1]);
+    qa[qedges[2 * i + 1]].push_back(qedges[2 * i]);
+  }
+  std::vector<uint32_t> ord{0}, seen(nv, 0);
+  seen[0] = 1;
+  for (size_t i = 0; i < ord.size(); ++i)
+    for (uint32_t w : qa[ord[i]])
+      if (!seen[w]) {
+        seen[w] = 1;
+        ord.push_back(w);
+      }
+  if (ord.size() != nv) return 1;
+  std::vector<int> rank(nv);
+  for (uint32_t i = 0; i < nv; ++i) rank[ord[i]] = (int)i;
+  std::vector<uint32_t> img(nv, UINT32_MAX);
+  uint64_t total = 0;
+  auto adjacent = [&](uint32_t x, uint32_t y) { return std::binary_search(nbr + off[x], nbr + off[x + 1], y); };
+  std::function<void(uint32_t)> rec = [&](uint32_t i) {
+    if (i == nv) {
+      total++;
+      return;
+    }
+    const uint32_t u = ord[i];
+    uint32_t anchor = UINT32_MAX;
+    for (uint32_t w : qa[u])
+      if (rank[w] < (int)i) {
+        anchor = w;
+        break;
+      }
+    auto try_v = [&](uint32_t v) {
+      if (lab[v] != qlab[u]) return;
+      for (uint32_t j = 0; j < i; ++j)
+        if (img[ord[j]] == v) return;
+      for (uint32_t w : qa[u])
+        if (rank[w] < (int)i && !adjacent(v, img[w])) return;
+      img[u] = v;
+      rec(i + 1);
+      img[u] = UINT32_MAX;
+    };
+    if (anchor == UINT32_MAX) {
+      for (uint32_t v = 0; v < n; ++v) try_v(v);
+    } else {
+      const uint32_t a = img[anchor];
+      for (uint64_t k = off[a]; k < off[a + 1]; ++k) try_v(nbr[k]);
+    }
+  };
+  rec(0);
+  *count = total;
+  return 0;
+}
+
+}  // extern "C"

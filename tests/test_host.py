@@ -1,0 +1,118 @@
+"""Host-side library: generators, reference text format, query extraction,
+and the C ABI's exported symbols (no GPU calls)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from gem_amd import GemError, Graph, Query, load
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_abi_exports_every_declared_symbol():
+    lib = load()
+    hdr = open(os.path.join(ROOT, "include", "gem_hip.h")).read()
+    names = set(re.findall(r"\b(gem_\w+)\s*\(", hdr))
+    assert len(names) > 20
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(ROOT, "gem_amd", "_lib", "libgem_hip.so")],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (gem_\w+)", out))
+    missing = names - exported
+    assert not missing, missing
+    for n in names:
+        assert hasattr(lib, n)
+
+
+def _csr_ok(g):
+    off, nbr = g.offsets, g.neighbors
+    n = g.num_vertices
+    for v in range(n):
+        row = nbr[off[v]:off[v + 1]]
+        assert np.all(np.diff(row.astype(np.int64)) > 0)
+        assert v not in row
+    # symmetric
+    src = np.repeat(np.arange(n), np.diff(off).astype(np.int64))
+    a = set(zip(src.tolist(), nbr.tolist()))
+    assert all((d, s) in a for s, d in list(a)[:5000])
+
+
+def test_er_generator_deterministic_and_exact():
+    g1 = Graph.erdos_renyi(1000, 5000, 8, 42)
+    g2 = Graph.erdos_renyi(1000, 5000, 8, 42)
+    g3 = Graph.erdos_renyi(1000, 5000, 8, 43)
+    assert g1.num_edges == 5000
+    assert np.array_equal(g1.neighbors, g2.neighbors) and np.array_equal(g1.labels, g2.labels)
+    assert not np.array_equal(g1.neighbors, g3.neighbors)
+    assert g1.labels.max() < 8
+    _csr_ok(g1)
+
+
+def test_rmat_generator():
+    g = Graph.rmat(12, 16.0, 16, 7)
+    h = Graph.rmat(12, 16.0, 16, 7)
+    assert np.array_equal(g.offsets, h.offsets) and np.array_equal(g.neighbors, h.neighbors)
+    assert g.num_vertices == 4096
+    assert g.num_slots <= 4096 * 16
+    _csr_ok(g)
+    # Zipf(s=1): label 0 is the most frequent, frequency ~ 1/H_16 = 0.296
+    freq = np.bincount(g.labels, minlength=16) / g.num_vertices
+    assert abs(freq[0] - 0.296) < 0.03 and freq[0] > freq[15]
+    # skew: R-MAT has hubs
+    deg = g.degrees()
+    assert deg.max() > 10 * deg.mean()
+
+
+def test_text_roundtrip(tmp_path):
+    g = Graph.erdos_renyi(300, 1200, 5, 3)
+    p = tmp_path / "data.graph"
+    g.save(p)
+    h = Graph.load(p)
+    assert np.array_equal(g.offsets, h.offsets)
+    assert np.array_equal(g.neighbors, h.neighbors)
+    assert np.array_equal(g.labels, h.labels)
+    first = p.read_text().splitlines()[0]
+    assert first == "t 300 1200"
+
+
+def test_loader_rejects_bad_degree(tmp_path):
+    p = tmp_path / "bad.graph"
+    p.write_text("t 3 2\nv 0 0 1\nv 1 0 1\nv 2 0 1\ne 0 1\ne 1 2\n")
+    with pytest.raises(GemError):
+        Graph.load(p)
+
+
+def test_query_random_walk_and_io(tmp_path):
+    g = Graph.erdos_renyi(2000, 10000, 4, 5)
+    q = Query.random_walk(g, 8, 1)
+    assert q.num_v == 8
+    assert np.all(q.edges[:, 0] < q.edges[:, 1])
+    # induced subgraph of distinct data vertices -> connected
+    seen = {0}
+    changed = True
+    while changed:
+        changed = False
+        for s, d in q.edges:
+            if (s in seen) != (d in seen):
+                seen |= {int(s), int(d)}
+                changed = True
+    assert len(seen) == 8
+    p = tmp_path / "Q_0.in"
+    q.save(p)
+    r = Query.load(p)
+    assert np.array_equal(q.labels, r.labels) and np.array_equal(q.edges, r.edges)
+    assert Query.random_walk(g, 8, 1).edges.tolist() == q.edges.tolist()
+
+
+def test_library_has_no_cpu_fallback():
+    """The product's matcher entry points go straight to HIP: without a GPU,
+    creating a context fails instead of computing on the CPU."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from gem_amd import GemContext
+    with pytest.raises(GemError):
+        GemContext(0)
