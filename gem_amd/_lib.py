@@ -31,12 +31,17 @@ class CsrView(C.Structure):
 
 class JoinResult(C.Structure):
     _fields_ = [("count", C.c_uint64), ("saturated", C.c_uint32), ("num_steps", C.c_uint32),
-                ("num_embeddings_written", C.c_uint64)]
+                ("num_embeddings_written", C.c_uint64), ("rows_processed", C.c_uint64),
+                ("work_limited", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class StepLog(C.Structure):
     _fields_ = [("edge", C.c_uint32 * GEM_MAX_QE), ("is_select", C.c_uint32 * GEM_MAX_QE),
                 ("rows", C.c_uint64 * GEM_MAX_QE), ("ms", C.c_float * GEM_MAX_QE)]
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 40), ("launches", C.c_uint64), ("ms", C.c_double), ("bytes", C.c_double)]
 
 
 class PhaseTimes(C.Structure):
@@ -66,6 +71,7 @@ SIGNATURES = [
     ("gem_hip_destroy", None, [C.c_void_p]),
     ("gem_hip_last_error", C.c_char_p, [C.c_void_p]),
     ("gem_hip_set_level_budget", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("gem_hip_set_work_limit", C.c_int, [C.c_void_p, C.c_uint64]),
     ("gem_hip_load_data_csr", C.c_int, [C.c_void_p, C.POINTER(CsrView)]),
     ("gem_hip_filter", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, u32p, u32p, C.c_uint64]),
     ("gem_hip_position_lookup", C.c_int, [C.c_void_p, C.c_uint32, u32p, C.c_uint64, u32p]),
@@ -74,6 +80,8 @@ SIGNATURES = [
     ("gem_hip_join", C.c_int, [C.c_void_p, u32p, C.c_uint64, C.POINTER(JoinResult), C.POINTER(StepLog),
                                u32p, C.c_uint64]),
     ("gem_hip_phase_times", C.c_int, [C.c_void_p, C.POINTER(PhaseTimes)]),
+    ("gem_hip_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
+    ("gem_hip_kernel_stats", C.c_int, [C.c_void_p, C.POINTER(KernelStat), C.c_uint32, u32p]),
     ("gem_hip_match", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint64, C.POINTER(JoinResult), u32p]),
 ]
 

@@ -79,6 +79,12 @@ int gem_hip_set_level_budget(gem_ctx *c, uint64_t bytes) {
   return GEM_OK;
 }
 
+int gem_hip_set_work_limit(gem_ctx *c, uint64_t rows) {
+  if (!c) return GEM_E_INVALID;
+  c->work_limit_rows = rows;
+  return GEM_OK;
+}
+
 int gem_hip_load_data_csr(gem_ctx *c, const gem_csr_view *g) {
   return guard(c, [&]() {
     if (!g || !g->offsets || !g->labels || (g->num_slots && !g->neighbors))
@@ -103,10 +109,12 @@ int gem_hip_load_data_csr(gem_ctx *c, const gem_csr_view *g) {
         id = (uint32_t)G.label_id.size();
         G.label_id.emplace(g->labels[v], id);
         G.label_freq.push_back(0);
+        G.label_deg.push_back(0);
       } else {
         id = it->second;
       }
       G.label_freq[id]++;
+      G.label_deg[id] += g->offsets[v + 1] - g->offsets[v];
       dense[v] = id;
     }
     G.lab8 = G.label_freq.size() <= 255;
@@ -189,6 +197,36 @@ int gem_hip_join(gem_ctx *c, const uint32_t *edge_order, uint64_t cap, gem_join_
     out->saturated = jo.saturated ? 1 : 0;
     out->num_steps = c->q.Q.ne - 1;
     out->num_embeddings_written = jo.emb_written;
+    out->rows_processed = jo.rows_processed;
+    out->work_limited = jo.work_limited ? 1 : 0;
+    out->reserved = 0;
+  });
+}
+
+int gem_hip_set_profiling(gem_ctx *c, int on) {
+  return guard(c, [&]() {
+    GEM_HIP_CHECK(hipStreamSynchronize(c->stream));
+    prof_flush(c);
+    for (auto &s : c->prof.st) s = KStat();
+    c->prof.on = on != 0;
+  });
+}
+
+int gem_hip_kernel_stats(gem_ctx *c, gem_kernel_stat *out, uint32_t max, uint32_t *n) {
+  return guard(c, [&]() {
+    if (!out || !n) throw Error{GEM_E_INVALID, "null argument"};
+    GEM_HIP_CHECK(hipStreamSynchronize(c->stream));
+    prof_flush(c);
+    uint32_t k = 0;
+    for (int id = 0; id < K_NUM && k < max; ++id) {
+      gem_kernel_stat &o = out[k++];
+      std::memset(o.name, 0, sizeof(o.name));
+      std::strncpy(o.name, kernel_name(id), sizeof(o.name) - 1);
+      o.launches = c->prof.st[id].launches;
+      o.ms = c->prof.st[id].ms;
+      o.bytes = c->prof.st[id].bytes;
+    }
+    *n = k;
   });
 }
 
@@ -219,6 +257,9 @@ int gem_hip_match(gem_ctx *c, const gem_query_view *q, uint64_t cap, gem_join_re
     out->saturated = jo.saturated ? 1 : 0;
     out->num_steps = Q.ne - 1;
     out->num_embeddings_written = 0;
+    out->rows_processed = jo.rows_processed;
+    out->work_limited = jo.work_limited ? 1 : 0;
+    out->reserved = 0;
   });
 }
 

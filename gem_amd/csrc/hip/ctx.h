@@ -37,7 +37,7 @@ struct Error {
   do {                                                                                    \
     hipError_t e_ = hipGetLastError();                                                    \
     if (e_ != hipSuccess)                                                                 \
-      throw ::gemd::Error{GEM_E_HIP, std::string("launch ") + name + ": " + hipGetErrorString(e_)}; \
+      throw ::gemd::Error{GEM_E_HIP, std::string("launch ") + (name) + ": " + hipGetErrorString(e_)}; \
   } while (0)
 
 // Stream-ordered device allocation (hipMallocAsync on the context stream; the
@@ -97,6 +97,7 @@ struct GraphDev {
   bool lab8 = true;
   std::unordered_map<uint32_t, uint32_t> label_id;  // original -> dense
   std::vector<uint64_t> label_freq;                 // dense id -> #vertices
+  std::vector<uint64_t> label_deg;                  // dense id -> sum of degrees
 };
 
 // Query description consumed by the filter kernel (device copy).
@@ -131,6 +132,45 @@ struct QueryDev {
   std::vector<TableDev> tab;           // per directed edge
 };
 
+// Kernel-level profiling: HIP events around launches on the context stream
+// plus the algorithmic bytes each launch must move (compulsory traffic: every
+// input element read once, every output element written once).
+enum KernelId {
+  K_FILTER,
+  K_FILTER_HEAVY,
+  K_COMPACT,
+  K_TABLES_COUNT,
+  K_TABLES_WRITE,
+  K_EXPAND_COUNT,
+  K_SELECT_COUNT,
+  K_EXPAND_FINAL,
+  K_SELECT_FINAL,
+  K_EXPAND_WRITE,
+  K_SELECT_WRITE,
+  K_NUM
+};
+inline const char *kernel_name(int id) {
+  static const char *n[K_NUM] = {"k_filter", "k_filter_heavy", "k_compact", "k_tables<count>",
+                                 "k_tables<write>",    "k_step_count<expand>", "k_step_count<select>",
+                                 "k_step_count<expand,final>", "k_step_count<select,final>", "k_expand_write",
+                                 "k_select_write"};
+  return n[id];
+}
+struct KStat {
+  uint64_t launches = 0;
+  double ms = 0, bytes = 0;
+};
+struct Prof {
+  bool on = false;
+  KStat st[K_NUM];
+  struct Pending {
+    int id;
+    hipEvent_t b, e;
+  };
+  std::vector<Pending> pend;
+  std::vector<hipEvent_t> pool;
+};
+
 }  // namespace gemd
 
 struct gem_ctx {
@@ -142,11 +182,65 @@ struct gem_ctx {
   gemd::QueryDev q;
   gem_phase_times times{};
   uint64_t level_budget_bytes = 1ull << 31;  // per join level buffer
+  uint64_t work_limit_rows = 0;             // 0 = none
   void *pinned = nullptr;                   // small pinned staging area
   size_t pinned_bytes = 0;
+  gemd::Prof prof;
 };
 
 namespace gemd {
+
+// Event pair around one launch; bytes are added separately (they may only be
+// known after a later count).
+inline hipEvent_t prof_event(gem_ctx *c) {
+  if (!c->prof.pool.empty()) {
+    hipEvent_t e = c->prof.pool.back();
+    c->prof.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  GEM_HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+struct ProfLaunch {
+  gem_ctx *c;
+  int id;
+  hipEvent_t b = nullptr;
+  ProfLaunch(gem_ctx *ctx, int kid) : c(ctx), id(kid) {
+    if (c->prof.on) {
+      b = prof_event(c);
+      GEM_HIP_CHECK(hipEventRecord(b, c->stream));
+    }
+  }
+  ~ProfLaunch() {
+    if (b) {
+      hipEvent_t e = nullptr;
+      try {
+        e = prof_event(c);
+      } catch (...) {
+        return;
+      }
+      (void)hipEventRecord(e, c->stream);
+      c->prof.pend.push_back({id, b, e});
+    }
+  }
+};
+inline void prof_bytes(gem_ctx *c, int id, double bytes) {
+  if (c->prof.on) c->prof.st[id].bytes += bytes;
+}
+// Resolve finished event pairs (call after a stream synchronisation).
+inline void prof_flush(gem_ctx *c) {
+  for (auto &p : c->prof.pend) {
+    float ms = 0;
+    if (hipEventSynchronize(p.e) == hipSuccess && hipEventElapsedTime(&ms, p.b, p.e) == hipSuccess) {
+      c->prof.st[p.id].ms += ms;
+      c->prof.st[p.id].launches += 1;
+    }
+    c->prof.pool.push_back(p.b);
+    c->prof.pool.push_back(p.e);
+  }
+  c->prof.pend.clear();
+}
 
 // Pinned staging for small device->host reads.
 template <class T>
@@ -171,6 +265,8 @@ struct JoinOut {
   uint64_t count = 0;
   bool saturated = false;
   uint64_t emb_written = 0;
+  uint64_t rows_processed = 0;
+  bool work_limited = false;
 };
 void run_join(gem_ctx *c, const uint32_t *order, uint64_t cap, JoinOut *out, gem_step_log *log, uint32_t *emb_out,
               uint64_t emb_capacity);

@@ -32,10 +32,9 @@ template <class LabT>
 __global__ void __launch_bounds__(FILTER_WAVES * 64)
 k_filter(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr, const LabT *__restrict__ lab, uint32_t n,
          const FilterQuery *__restrict__ fqp, const uint8_t *__restrict__ slot_of, uint64_t *__restrict__ bitmap,
-         uint32_t *__restrict__ tile_count) {
+         uint32_t *__restrict__ tile_count, uint32_t *__restrict__ heavy_list, uint32_t *__restrict__ heavy_n) {
   __shared__ FilterQuery fq;
   __shared__ uint32_t hist[FILTER_WAVES][GEM_MAX_QV][64];
-  __shared__ uint32_t heavy_cnt[FILTER_WAVES][GEM_MAX_QV];
   {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(fqp);
     uint32_t *dst = reinterpret_cast<uint32_t *>(&fq);
@@ -85,38 +84,14 @@ k_filter(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr, con
       if (deficit <= T) pass |= 1u << u;
     }
   }
-  // heavy vertices: whole wave, ballot counting
-  uint64_t heavy = __ballot(cmask != 0 && deg > LIGHT_DEG);
-  while (heavy) {
-    const uint32_t hl = __ffsll((unsigned long long)heavy) - 1;
-    heavy &= heavy - 1;
-    const uint64_t ho0 = readlane_u64(o0, hl), hdeg = readlane_u64(deg, hl);
-    const uint32_t hmask = readlane_u32(cmask, hl);
-    uint32_t acc = 0;  // lane s holds the count of slot s
-    for (uint64_t j = 0; j < hdeg; j += 64) {
-      uint32_t s = NO_SLOT;
-      if (j + l < hdeg) s = slot_of[lab[nbr[ho0 + j + l]]];
-      for (uint32_t q = 0; q < nslots; ++q) {
-        uint32_t c = (uint32_t)__popcll(__ballot(s == q));
-        if (l == q) acc += c;
-      }
-    }
-    if (l < nslots) heavy_cnt[w][l] = acc;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // lane u decides query vertex u
-    bool ok = false;
-    if (l < fq.nv && (hmask >> l & 1u)) {
-      uint32_t deficit = 0;
-      for (uint32_t r = 0; r < fq.nreq[l]; ++r) {
-        uint32_t need = fq.req_cnt[l][r], have = heavy_cnt[w][fq.req_slot[l][r]];
-        deficit += need > have ? need - have : 0;
-      }
-      ok = deficit <= T;
-    }
-    uint32_t pm = (uint32_t)__ballot(ok);
-    if (l == hl) pass = pm;
-    __builtin_amdgcn_wave_barrier();
+  // heavy vertices are deferred to k_filter_heavy (one wave each), so a tile
+  // full of hubs does not hold its wave for thousands of steps
+  const uint64_t heavy = __ballot(cmask != 0 && deg > LIGHT_DEG);
+  if (heavy) {
+    uint32_t base = 0;
+    if (l == 0) base = atomicAdd(heavy_n, (uint32_t)__popcll(heavy));
+    base = readlane_u32(base, 0);
+    if ((heavy >> l) & 1ull) heavy_list[base + rank_below(heavy)] = (uint32_t)v;
   }
   const uint32_t W = fq.W;
   for (uint32_t u = 0; u < fq.nv; ++u) {
@@ -125,6 +100,59 @@ k_filter(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr, con
       bitmap[(uint64_t)u * W + tile] = m;
       tile_count[(uint64_t)u * W + tile] = (uint32_t)__popcll(m);
     }
+  }
+}
+
+// Vertices of degree > LIGHT_DEG: one wave per vertex (persistent grid over
+// the list k_filter built).  Neighbour labels are counted 64 at a time, one
+// ballot + popcount per query label (lane s keeps label s's count); lane u then
+// decides query vertex u and sets its bit with an atomic OR.
+template <class LabT>
+__global__ void __launch_bounds__(FILTER_WAVES * 64)
+k_filter_heavy(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr, const LabT *__restrict__ lab,
+               const FilterQuery *__restrict__ fqp, const uint8_t *__restrict__ slot_of,
+               unsigned long long *__restrict__ bitmap, uint32_t *__restrict__ tile_count,
+               const uint32_t *__restrict__ heavy_list, const uint32_t *__restrict__ heavy_n) {
+  __shared__ FilterQuery fq;
+  __shared__ uint32_t cnt[FILTER_WAVES][GEM_MAX_QV];
+  {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(fqp);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&fq);
+    for (uint32_t i = threadIdx.x; i < sizeof(FilterQuery) / 4; i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const uint32_t nh = *heavy_n;
+  const uint32_t nslots = fq.nslots, T = fq.T;
+  for (uint32_t i = blockIdx.x * FILTER_WAVES + w; i < nh; i += gridDim.x * FILTER_WAVES) {
+    const uint32_t v = heavy_list[i];
+    const uint64_t o0 = off[v], deg = off[v + 1] - o0;
+    const uint32_t g = fq.group[slot_of[lab[v]]];
+    uint32_t acc = 0;
+    for (uint64_t j = 0; j < deg; j += 64) {
+      uint32_t s = NO_SLOT;
+      if (j + l < deg) s = slot_of[lab[nbr[o0 + j + l]]];
+      for (uint32_t q = 0; q < nslots; ++q) {
+        const uint32_t c = (uint32_t)__popcll(__ballot(s == q));
+        if (l == q) acc += c;
+      }
+    }
+    if (l < nslots) cnt[w][l] = acc;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (l < fq.nv && ((g >> l) & 1u) && (uint64_t)fq.qdeg[l] <= deg + T) {
+      uint32_t deficit = 0;
+      for (uint32_t r = 0; r < fq.nreq[l]; ++r) {
+        const uint32_t need = fq.req_cnt[l][r], have = cnt[w][fq.req_slot[l][r]];
+        deficit += need > have ? need - have : 0;
+      }
+      if (deficit <= T) {
+        atomicOr(&bitmap[(uint64_t)l * fq.W + (v >> 6)], 1ull << (v & 63));
+        atomicAdd(&tile_count[(uint64_t)l * fq.W + (v >> 6)], 1u);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 }
 
@@ -229,19 +257,55 @@ void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T) {
   GEM_HIP_CHECK(hipMemcpyAsync(can_off.p, Qd.can_off.data(), (nv + 1) * 8, hipMemcpyHostToDevice, s));
 
   const uint32_t blocks = (W + FILTER_WAVES - 1) / FILTER_WAVES;
-  if (G.lab8)
-    k_filter<uint8_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(
-        G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint8_t>(), G.n, Qd.fq.as<FilterQuery>(),
-        Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<uint64_t>(), tile_count.as<uint32_t>());
-  else
-    k_filter<uint32_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(
-        G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint32_t>(), G.n, Qd.fq.as<FilterQuery>(),
-        Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<uint64_t>(), tile_count.as<uint32_t>());
-  GEM_LAUNCH_CHECK("k_filter");
+  DevBuf heavy_list, heavy_n;
+  heavy_list.alloc(std::max<uint64_t>(G.n, 1) * 4, s);
+  heavy_n.alloc(4, s);
+  GEM_HIP_CHECK(hipMemsetAsync(heavy_n.p, 0, 4, s));
+  {
+    ProfLaunch pl(c, K_FILTER);
+    if (G.lab8)
+      k_filter<uint8_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(
+          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint8_t>(), G.n, Qd.fq.as<FilterQuery>(),
+          Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<uint64_t>(), tile_count.as<uint32_t>(), heavy_list.as<uint32_t>(),
+          heavy_n.as<uint32_t>());
+    else
+      k_filter<uint32_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(
+          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint32_t>(), G.n, Qd.fq.as<FilterQuery>(),
+          Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<uint64_t>(), tile_count.as<uint32_t>(), heavy_list.as<uint32_t>(),
+          heavy_n.as<uint32_t>());
+    GEM_LAUNCH_CHECK("k_filter");
+  }
+  {
+    ProfLaunch pl(c, K_FILTER_HEAVY);
+    const uint32_t hb = 1024;  // persistent: 4096 waves walk the heavy list
+    if (G.lab8)
+      k_filter_heavy<uint8_t><<<hb, FILTER_WAVES * 64, 0, s>>>(
+          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint8_t>(), Qd.fq.as<FilterQuery>(),
+          Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<unsigned long long>(), tile_count.as<uint32_t>(),
+          heavy_list.as<uint32_t>(), heavy_n.as<uint32_t>());
+    else
+      k_filter_heavy<uint32_t><<<hb, FILTER_WAVES * 64, 0, s>>>(
+          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint32_t>(), Qd.fq.as<FilterQuery>(),
+          Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<unsigned long long>(), tile_count.as<uint32_t>(),
+          heavy_list.as<uint32_t>(), heavy_n.as<uint32_t>());
+    GEM_LAUNCH_CHECK("k_filter_heavy");
+  }
+  // algorithmic bytes: 16 B per vertex (label, degree, offset pair), 8 B per
+  // adjacency slot of a label-matching vertex (neighbour id + its label), the
+  // candidate bitmaps and per-tile counts written
+  double adj = 0;
+  for (uint32_t sl = 0; sl < fq.nslots; ++sl) {
+    auto it = G.label_id.find(slot_label[sl]);
+    if (it != G.label_id.end()) adj += (double)G.label_deg[it->second];
+  }
+  prof_bytes(c, K_FILTER, 16.0 * G.n + 8.0 * adj + nv * (double)W * 8 + nv * (double)W * 4);
   scan_segments_u32(c, tile_count.as<uint32_t>(), Qd.rank.as<uint32_t>(), nv, W);
-  k_compact<<<(W + 3) / 4, 256, 0, s>>>(Qd.bitmap.as<uint64_t>(), Qd.rank.as<uint32_t>(), nv, W,
-                                        can_off.as<uint64_t>(), Qd.can.as<uint32_t>());
-  GEM_LAUNCH_CHECK("k_compact");
+  {
+    ProfLaunch pl(c, K_COMPACT);
+    k_compact<<<(W + 3) / 4, 256, 0, s>>>(Qd.bitmap.as<uint64_t>(), Qd.rank.as<uint32_t>(), nv, W,
+                                          can_off.as<uint64_t>(), Qd.can.as<uint32_t>());
+    GEM_LAUNCH_CHECK("k_compact");
+  }
   // |Can(u)| = rank[u][W]
   uint32_t *h = pinned<uint32_t>(c, nv);
   for (uint32_t u = 0; u < nv; ++u)
@@ -249,6 +313,10 @@ void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T) {
   GEM_HIP_CHECK(hipEventRecord(c->ev[1], s));
   GEM_HIP_CHECK(hipStreamSynchronize(s));
   Qd.ncan.assign(h, h + nv);
+  double ncan_sum = 0;
+  for (uint32_t u = 0; u < nv; ++u) ncan_sum += Qd.ncan[u];
+  prof_bytes(c, K_COMPACT, nv * (double)W * 12 + 4.0 * ncan_sum);
+  prof_flush(c);
   float ms = 0;
   GEM_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   c->times.filter_ms = ms;

@@ -55,11 +55,19 @@ __device__ __forceinline__ uint32_t position_in(const uint64_t *bm, const uint32
   return rank[v >> 6] + (uint32_t)__popcll(word & ((1ull << b) - 1));
 }
 
+// Bytes charged for one binary search over a list of len entries:
+// 4 * ceil(log2(len) + 1).
+__device__ __forceinline__ uint64_t search_bytes(uint64_t len) {
+  if (len == 0) return 0;
+  const uint32_t cl = len == 1 ? 0u : 64u - (uint32_t)__clzll(len - 1);
+  return 4ull * (cl + 1);
+}
+
 // Count of row r's expansion: list length minus equal-label columns that occur
-// in the list.  Also returns the list start/length and (optionally) the dup
-// positions.
+// in the list.  Also returns the list start/length, (optionally) the dup
+// positions, and the search bytes spent.
 __device__ __forceinline__ uint64_t expand_count_row(const StepArgs &a, uint64_t r, uint64_t *o0, uint64_t *len,
-                                                     uint32_t *dup_pos) {
+                                                     uint32_t *dup_pos, uint64_t *sbytes) {
   const uint32_t *row = a.rows + r * a.k;
   const uint32_t p = row[a.key_col];
   const uint64_t b = a.toff[p], e = a.toff[p + 1];
@@ -70,15 +78,19 @@ __device__ __forceinline__ uint64_t expand_count_row(const StepArgs &a, uint64_t
     const uint32_t vid = a.dup_can[c][row[a.dup_col[c]]];
     const uint32_t pos = position_in(a.bm_new, a.rank_new, vid);
     if (dup_pos) dup_pos[c] = pos;
-    if (pos != NOT_FOUND && sorted_contains(a.tval + b, e - b, pos)) --cnt;
+    if (pos != NOT_FOUND) {
+      *sbytes += search_bytes(e - b);
+      if (sorted_contains(a.tval + b, e - b, pos)) --cnt;
+    }
   }
   return cnt;
 }
 
-__device__ __forceinline__ bool select_row(const StepArgs &a, uint64_t r) {
+__device__ __forceinline__ bool select_row(const StepArgs &a, uint64_t r, uint64_t *len) {
   const uint32_t *row = a.rows + r * a.k;
   const uint32_t p = row[a.key_col], v = row[a.val_col];
   const uint64_t b = a.toff[p], e = a.toff[p + 1];
+  *len = e - b;
   return sorted_contains(a.tval + b, e - b, v);
 }
 
@@ -93,28 +105,38 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t x, uint64_t *sm) {
   return t;
 }
 
-// per-block counts (or, with FINAL, one atomic add of the block sum)
+// Per-block counts (or, with FINAL, one atomic add of the block sum into
+// aux[0]).  With `prof` set, aux[1] += list entries addressed and aux[2] +=
+// binary-search bytes (algorithmic-byte bookkeeping).
 template <bool SELECT, bool FINAL>
 __global__ void __launch_bounds__(JB) k_step_count(const StepArgs a, uint64_t *__restrict__ block_counts,
-                                                  unsigned long long *__restrict__ total) {
+                                                  unsigned long long *__restrict__ aux, int prof) {
   __shared__ uint64_t sm[JW];
   const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
   const uint64_t r = (uint64_t)b * JB + threadIdx.x;
-  uint64_t cnt = 0;
+  uint64_t cnt = 0, len = 0, sb = 0;
   if (r < a.nrows) {
     if (SELECT) {
-      cnt = select_row(a, r) ? 1 : 0;
+      cnt = select_row(a, r, &len) ? 1 : 0;
+      sb = search_bytes(len);
     } else {
-      uint64_t o0, len;
-      cnt = expand_count_row(a, r, &o0, &len, nullptr);
+      uint64_t o0;
+      cnt = expand_count_row(a, r, &o0, &len, nullptr, &sb);
     }
   }
   const uint64_t s = block_sum_u64(cnt, sm);
   if (threadIdx.x == 0) {
     if (FINAL) {
-      if (s) atomicAdd(total, (unsigned long long)s);
+      if (s) atomicAdd(aux, (unsigned long long)s);
     } else {
       block_counts[b] = s;
+    }
+  }
+  if (prof) {
+    const uint64_t ls = block_sum_u64(len, sm), ss = block_sum_u64(sb, sm);
+    if (threadIdx.x == 0) {
+      atomicAdd(aux + 1, (unsigned long long)ls);
+      atomicAdd(aux + 2, (unsigned long long)ss);
     }
   }
 }
@@ -137,7 +159,8 @@ __global__ void __launch_bounds__(JB) k_select_write(const StepArgs a, const uin
   __shared__ uint64_t sm[JW];
   const uint32_t b = b_first + xcd_swizzle(blockIdx.x, gridDim.x);
   const uint64_t r = (uint64_t)b * JB + threadIdx.x;
-  const bool keep = r < a.nrows && select_row(a, r);
+  uint64_t len;
+  const bool keep = r < a.nrows && select_row(a, r, &len);
   const uint64_t ex = block_excl_u64(keep ? 1 : 0, sm);
   if (keep) {
     const uint64_t o = block_off[b] - out_base + ex;
@@ -154,7 +177,8 @@ __device__ __forceinline__ uint64_t lanes_between(uint32_t lo, uint32_t hi) {  /
 
 // Expand write.  Dynamic LDS: JW * 64 * ndup u32 dup positions.
 __global__ void __launch_bounds__(JB) k_expand_write(const StepArgs a, const uint64_t *__restrict__ block_off,
-                                                    uint32_t b_first, uint64_t out_base, uint32_t *__restrict__ out) {
+                                                    uint32_t b_first, uint64_t out_base, uint32_t *__restrict__ out,
+                                                    unsigned long long *__restrict__ aux, int prof) {
   __shared__ uint64_t sm[JW];
   __shared__ uint32_t st[JW][65];
   __shared__ uint64_t ob[JW][64];
@@ -164,12 +188,13 @@ __global__ void __launch_bounds__(JB) k_expand_write(const StepArgs a, const uin
   const uint32_t b = b_first + xcd_swizzle(blockIdx.x, gridDim.x);
   const uint64_t r = (uint64_t)b * JB + threadIdx.x;
   uint32_t *dp = dup_lds + (uint64_t)(w * 64 + l) * a.ndup;
-  uint64_t cnt = 0, o0 = 0, len = 0;
-  if (r < a.nrows) cnt = expand_count_row(a, r, &o0, &len, dp);
+  uint64_t cnt = 0, o0 = 0, len = 0, sb = 0;
+  if (r < a.nrows) cnt = expand_count_row(a, r, &o0, &len, dp, &sb);
   const uint64_t ex = block_excl_u64(cnt, sm);
   const uint32_t ilen = (uint32_t)len;
   const uint32_t incl = wave_incl_scan_u32(ilen);
   const uint32_t total = readlane_u32(incl, 63);
+  if (prof && l == 0 && total) atomicAdd(aux + 1, (unsigned long long)total);
   st[w][l] = incl - ilen;
   if (l == 63) st[w][64] = total;
   ob[w][l] = o0;
@@ -263,12 +288,13 @@ class JoinRunner {
     rows_.assign(steps_.size(), 0);
     ms_.assign(steps_.size(), 0.0f);
     budget_ = std::max<uint64_t>(c->level_budget_bytes, 1ull << 20);
+    work_limit_ = c->work_limit_rows;
   }
 
   void run(JoinOut *out) {
     hipStream_t s = c_->stream;
-    d_total_.alloc(8, s);
-    GEM_HIP_CHECK(hipMemsetAsync(d_total_.p, 0, 8, s));
+    d_aux_.alloc(24, s);
+    GEM_HIP_CHECK(hipMemsetAsync(d_aux_.p, 0, 24, s));
     GEM_HIP_CHECK(hipEventCreate(&e0_));
     GEM_HIP_CHECK(hipEventCreate(&e1_));
     const uint32_t a = col_u_[0];
@@ -285,6 +311,12 @@ class JoinRunner {
     out->count = saturated_ ? cap_ : final_total_;
     out->saturated = saturated_;
     out->emb_written = emb_rows_;
+    out->rows_processed = processed_;
+    out->work_limited = work_limited_;
+    if (work_limited_) {
+      out->count = 0;
+      out->saturated = false;
+    }
     if (log_) {
       for (size_t i = 0; i < steps_.size() && i < GEM_MAX_QE; ++i) {
         log_->edge[i] = i == 0 ? first_edge_ : steps_[i].table >> 1;
@@ -309,10 +341,13 @@ class JoinRunner {
   std::vector<uint64_t> rows_;
   std::vector<float> ms_;
   uint64_t budget_;
-  DevBuf d_total_;
+  DevBuf d_aux_;
   uint64_t final_total_ = 0;
   bool saturated_ = false;
   bool stop_ = false;
+  uint64_t processed_ = 0;
+  uint64_t work_limit_ = 0;
+  bool work_limited_ = false;
   uint64_t emb_rows_ = 0;
   hipEvent_t e0_{}, e1_{};
 
@@ -356,19 +391,37 @@ class JoinRunner {
     ms_[s] += ms;
   }
 
+  // aux[0] = final count, aux[1] = list entries addressed, aux[2] = search bytes
+  void read_aux(uint64_t *out3) {
+    uint64_t *h = pinned<uint64_t>(c_, 3);
+    GEM_HIP_CHECK(hipMemcpyAsync(h, d_aux_.p, 24, hipMemcpyDeviceToHost, c_->stream));
+    GEM_HIP_CHECK(hipStreamSynchronize(c_->stream));
+    out3[0] = h[0];
+    out3[1] = h[1];
+    out3[2] = h[2];
+  }
+
   void final_count(uint32_t si, const uint32_t *rows, uint64_t n, uint32_t k) {
     const JStep &st = steps_[si];
     StepArgs a = args(st, rows, n, k);
     const uint32_t nb = (uint32_t)((n + JB - 1) / JB);
+    const int prof = c_->prof.on ? 1 : 0;
     time_begin();
-    GEM_HIP_CHECK(hipMemsetAsync(d_total_.p, 0, 8, c_->stream));
-    if (st.select)
-      k_step_count<true, true><<<nb, JB, 0, c_->stream>>>(a, nullptr, d_total_.as<unsigned long long>());
-    else
-      k_step_count<false, true><<<nb, JB, 0, c_->stream>>>(a, nullptr, d_total_.as<unsigned long long>());
-    GEM_LAUNCH_CHECK("k_step_count<final>");
-    const uint64_t got = read_u64(d_total_.as<uint64_t>());
+    GEM_HIP_CHECK(hipMemsetAsync(d_aux_.p, 0, 24, c_->stream));
+    {
+      ProfLaunch pl(c_, st.select ? K_SELECT_FINAL : K_EXPAND_FINAL);
+      if (st.select)
+        k_step_count<true, true><<<nb, JB, 0, c_->stream>>>(a, nullptr, d_aux_.as<unsigned long long>(), prof);
+      else
+        k_step_count<false, true><<<nb, JB, 0, c_->stream>>>(a, nullptr, d_aux_.as<unsigned long long>(), prof);
+      GEM_LAUNCH_CHECK("k_step_count<final>");
+    }
+    uint64_t aux[3];
+    read_aux(aux);
     time_end(si);
+    // rows + offset pair per row + searches
+    prof_bytes(c_, st.select ? K_SELECT_FINAL : K_EXPAND_FINAL, (4.0 * k + 8.0) * n + (double)aux[2]);
+    const uint64_t got = aux[0];
     rows_[si] += got;
     final_total_ += got;
     if (cap_ && final_total_ > cap_) {
@@ -406,6 +459,12 @@ class JoinRunner {
   // Apply step si to `n` rows of width k.
   void level(uint32_t si, const uint32_t *rows, uint64_t n, uint32_t k) {
     if (stop_ || n == 0) return;
+    processed_ += n;
+    if (work_limit_ && processed_ > work_limit_) {
+      work_limited_ = true;
+      stop_ = true;
+      return;
+    }
     const JStep &st = steps_[si];
     const bool last = si + 1 == steps_.size();
     if (last && !emb_out_) {
@@ -415,21 +474,34 @@ class JoinRunner {
     hipStream_t s = c_->stream;
     StepArgs a = args(st, rows, n, k);
     const uint32_t nb = (uint32_t)((n + JB - 1) / JB);
+    const int prof = c_->prof.on ? 1 : 0;
     time_begin();
     DevBuf bc, bo;
     bc.alloc((uint64_t)nb * 8, s);
     bo.alloc(((uint64_t)nb + 1) * 8, s);
-    if (st.select)
-      k_step_count<true, false><<<nb, JB, 0, s>>>(a, bc.as<uint64_t>(), nullptr);
-    else
-      k_step_count<false, false><<<nb, JB, 0, s>>>(a, bc.as<uint64_t>(), nullptr);
-    GEM_LAUNCH_CHECK("k_step_count");
+    if (prof) GEM_HIP_CHECK(hipMemsetAsync(d_aux_.p, 0, 24, s));
+    {
+      ProfLaunch pl(c_, st.select ? K_SELECT_COUNT : K_EXPAND_COUNT);
+      if (st.select)
+        k_step_count<true, false><<<nb, JB, 0, s>>>(a, bc.as<uint64_t>(), d_aux_.as<unsigned long long>(), prof);
+      else
+        k_step_count<false, false><<<nb, JB, 0, s>>>(a, bc.as<uint64_t>(), d_aux_.as<unsigned long long>(), prof);
+      GEM_LAUNCH_CHECK("k_step_count");
+    }
     scan_u64(c_, bc.as<uint64_t>(), bo.as<uint64_t>(), nb);
     const uint64_t total = read_u64(bo.as<uint64_t>() + nb);
     bc.reset();
     rows_[si] += total;
     const uint32_t ko = st.select ? k : k + 1;
     const uint64_t row_bytes = (uint64_t)ko * 4;
+    double search_per_row = 0;
+    if (prof) {
+      uint64_t aux[3];
+      read_aux(aux);
+      // count pass: rows, offset pairs, searches
+      prof_bytes(c_, st.select ? K_SELECT_COUNT : K_EXPAND_COUNT, (4.0 * k + 8.0) * n + (double)aux[2]);
+      search_per_row = n ? (double)aux[2] / (double)n : 0.0;
+    }
     if (total == 0) {
       time_end(si);
       return;
@@ -462,13 +534,26 @@ class JoinRunner {
       DevBuf outb;
       outb.alloc(cnt * row_bytes, s);
       const uint32_t g = b1 - b0;
-      if (st.select) {
-        k_select_write<<<g, JB, 0, s>>>(a, bo.as<uint64_t>(), b0, base, outb.as<uint32_t>());
-        GEM_LAUNCH_CHECK("k_select_write");
-      } else {
-        const size_t lds = (size_t)JW * 64 * st.ndup * 4;
-        k_expand_write<<<g, JB, lds, s>>>(a, bo.as<uint64_t>(), b0, base, outb.as<uint32_t>());
-        GEM_LAUNCH_CHECK("k_expand_write");
+      if (prof) GEM_HIP_CHECK(hipMemsetAsync(d_aux_.p, 0, 24, s));
+      {
+        ProfLaunch pl(c_, st.select ? K_SELECT_WRITE : K_EXPAND_WRITE);
+        if (st.select) {
+          k_select_write<<<g, JB, 0, s>>>(a, bo.as<uint64_t>(), b0, base, outb.as<uint32_t>());
+          GEM_LAUNCH_CHECK("k_select_write");
+        } else {
+          const size_t lds = (size_t)JW * 64 * st.ndup * 4;
+          k_expand_write<<<g, JB, lds, s>>>(a, bo.as<uint64_t>(), b0, base, outb.as<uint32_t>(),
+                                            d_aux_.as<unsigned long long>(), prof);
+          GEM_LAUNCH_CHECK("k_expand_write");
+        }
+      }
+      if (prof) {
+        // rows of the range (+ their searches), list entries walked, rows written
+        uint64_t aux[3];
+        read_aux(aux);
+        const double nr = (double)std::min<uint64_t>((uint64_t)(b1 - b0) * JB, n - (uint64_t)b0 * JB);
+        prof_bytes(c_, st.select ? K_SELECT_WRITE : K_EXPAND_WRITE,
+                   (4.0 * k + 8.0 + search_per_row) * nr + 4.0 * (double)aux[1] + (double)row_bytes * cnt);
       }
       time_end(si);
       timed = true;
@@ -499,6 +584,7 @@ void run_join(gem_ctx *c, const uint32_t *order, uint64_t cap, JoinOut *out, gem
   float ms = 0;
   GEM_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
   c->times.join_ms = ms;
+  prof_flush(c);
   if (emb_out && out->emb_written > emb_capacity) out->emb_written = 0;
 }
 

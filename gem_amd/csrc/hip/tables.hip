@@ -44,11 +44,16 @@ __device__ __forceinline__ uint64_t lane_range(uint32_t a, uint32_t b) {  // bit
   return hi & ~((1ull << a) - 1);
 }
 
+// Rows longer than HEAVY_ROW are walked by k_tables_heavy, one wave per row,
+// so that a wave of 64 hub rows cannot become a straggler.
+constexpr uint32_t HEAVY_ROW = 256;
+
 template <bool WRITE>
 __global__ void __launch_bounds__(TAB_WAVES * 64)
 k_tables(const uint64_t *__restrict__ goff, const uint32_t *__restrict__ gnbr, const uint32_t *__restrict__ can,
          const uint64_t *__restrict__ bitmap, const uint32_t *__restrict__ rank, const TableJob job,
-         const TableDesc *__restrict__ desc) {
+         const TableDesc *__restrict__ desc, unsigned long long *__restrict__ adj_len,
+         uint64_t *__restrict__ heavy_list, uint32_t *__restrict__ heavy_n) {
   __shared__ uint32_t st[TAB_WAVES][65];
   __shared__ uint64_t ob[TAB_WAVES][64];
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
@@ -61,21 +66,32 @@ k_tables(const uint64_t *__restrict__ goff, const uint32_t *__restrict__ gnbr, c
   const uint32_t i = row_base + l;
   const bool valid = i < ncan;
   uint64_t o0 = 0;
-  uint32_t len = 0;
+  uint32_t len = 0, full_len = 0;
   if (valid) {
     const uint32_t v = can[job.can_off[ju] + i];
     o0 = goff[v];
-    len = (uint32_t)(goff[v + 1] - o0);
+    full_len = (uint32_t)(goff[v + 1] - o0);
+    len = full_len > HEAVY_ROW ? 0u : full_len;
+  }
+  if (!WRITE) {
+    const uint64_t hv = __ballot(full_len > HEAVY_ROW);
+    if (hv) {
+      uint32_t base = 0;
+      if (l == 0) base = atomicAdd(heavy_n, (uint32_t)__popcll(hv));
+      base = readlane_u32(base, 0);
+      if ((hv >> l) & 1ull) heavy_list[base + rank_below(hv)] = ((uint64_t)ju << 32) | i;
+    }
   }
   const uint32_t incl = wave_incl_scan_u32(len);
   const uint32_t total = readlane_u32(incl, 63);
+  if (!WRITE && l == 0 && total) atomicAdd(adj_len + ju, (unsigned long long)total);
   st[w][l] = incl - len;
   ob[w][l] = o0;
   if (l == 63) st[w][64] = total;
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   const uint32_t nd = job.nd[ju], t0 = job.tab0[ju];
-  if (!WRITE && valid && len == 0)
+  if (!WRITE && valid && full_len == 0)
     for (uint32_t j = 0; j < nd; ++j) desc[t0 + j].off[i] = 0;
   const uint32_t W = job.W;
   uint32_t carry_reg = 0;  // lane j: hits of the row carried into the next step, table j
@@ -118,6 +134,49 @@ k_tables(const uint64_t *__restrict__ goff, const uint32_t *__restrict__ gnbr, c
       const uint32_t carry_next = readlane_u32(row_done ? 0u : tot_row, last);
       if (l == j) carry_reg = carry_next;
     }
+  }
+}
+
+// One wave per heavy row (persistent grid over the list built by the light
+// pass); lane j carries table j's running hit count across 64-item steps.
+template <bool WRITE>
+__global__ void __launch_bounds__(TAB_WAVES * 64)
+k_tables_heavy(const uint64_t *__restrict__ goff, const uint32_t *__restrict__ gnbr, const uint32_t *__restrict__ can,
+               const uint64_t *__restrict__ bitmap, const uint32_t *__restrict__ rank, const TableJob job,
+               const TableDesc *__restrict__ desc, unsigned long long *__restrict__ adj_len,
+               const uint64_t *__restrict__ heavy_list, const uint32_t *__restrict__ heavy_n) {
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const uint32_t nh = *heavy_n, W = job.W;
+  for (uint32_t h = blockIdx.x * TAB_WAVES + w; h < nh; h += gridDim.x * TAB_WAVES) {
+    const uint64_t e = heavy_list[h];
+    const uint32_t ju = (uint32_t)(e >> 32), i = (uint32_t)e;
+    const uint32_t v = can[job.can_off[ju] + i];
+    const uint64_t o0 = goff[v], len = goff[v + 1] - o0;
+    if (!WRITE && l == 0) atomicAdd(adj_len + ju, (unsigned long long)len);
+    const uint32_t nd = job.nd[ju], t0 = job.tab0[ju];
+    uint32_t run_reg = 0;
+    for (uint64_t t = 0; t < len; t += 64) {
+      const bool iv = t + l < len;
+      const uint32_t wv = iv ? gnbr[o0 + t + l] : 0;
+      for (uint32_t j = 0; j < nd; ++j) {
+        const TableDesc d = desc[t0 + j];
+        bool hit = false;
+        uint64_t word = 0;
+        if (iv) {
+          word = bitmap[(uint64_t)d.val_u * W + (wv >> 6)];
+          hit = (word >> (wv & 63)) & 1ull;
+        }
+        const uint64_t m = __ballot(hit);
+        const uint32_t run = readlane_u32(run_reg, j);
+        if (WRITE && hit) {
+          const uint32_t p = rank[(uint64_t)d.val_u * (W + 1) + (wv >> 6)] +
+                             (uint32_t)__popcll(word & ((1ull << (wv & 63)) - 1));
+          d.val[d.off[i] + run + rank_below(m)] = p;
+        }
+        if (l == j) run_reg = run + (uint32_t)__popcll(m);
+      }
+    }
+    if (!WRITE && l < nd) desc[t0 + l].off[i] = run_reg;
   }
 }
 
@@ -166,35 +225,70 @@ void run_tables(gem_ctx *c) {
   ddesc.alloc(std::max<size_t>(desc.size(), 1) * sizeof(TableDesc), s);
   GEM_HIP_CHECK(hipMemcpyAsync(ddesc.p, desc.data(), desc.size() * sizeof(TableDesc), hipMemcpyHostToDevice, s));
   const uint32_t blocks = (waves + TAB_WAVES - 1) / TAB_WAVES;
-  if (blocks) {
-    k_tables<false><<<blocks, TAB_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(),
-                                                      Qd.bitmap.as<uint64_t>(), Qd.rank.as<uint32_t>(), job,
-                                                      ddesc.as<TableDesc>());
-    GEM_LAUNCH_CHECK("k_tables<count>");
-  }
-  uint64_t *h = pinned<uint64_t>(c, std::max<size_t>(desc_tab.size(), 1));
-  for (size_t k = 0; k < desc_tab.size(); ++k) {
+  DevBuf adj_len, heavy_list, heavy_n;
+  adj_len.alloc(GEM_MAX_QV * 8, s);
+  GEM_HIP_CHECK(hipMemsetAsync(adj_len.p, 0, GEM_MAX_QV * 8, s));
+  uint64_t rows_total = 0;
+  for (uint32_t ju = 0; ju < job.nu; ++ju) rows_total += job.ncan[ju];
+  heavy_list.alloc(std::max<uint64_t>(rows_total, 1) * 8, s);
+  heavy_n.alloc(4, s);
+  GEM_HIP_CHECK(hipMemsetAsync(heavy_n.p, 0, 4, s));
+  const uint32_t heavy_blocks = 512;  // persistent: 2048 waves walk the heavy rows
+  auto launch = [&](bool write) {
+    if (!blocks) return;
+    ProfLaunch pl(c, write ? K_TABLES_WRITE : K_TABLES_COUNT);
+    if (write) {
+      k_tables<true><<<blocks, TAB_WAVES * 64, 0, s>>>(
+          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
+          Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
+          heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>());
+      k_tables_heavy<true><<<heavy_blocks, TAB_WAVES * 64, 0, s>>>(
+          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
+          Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
+          heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>());
+    } else {
+      k_tables<false><<<blocks, TAB_WAVES * 64, 0, s>>>(
+          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
+          Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
+          heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>());
+      k_tables_heavy<false><<<heavy_blocks, TAB_WAVES * 64, 0, s>>>(
+          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
+          Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
+          heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>());
+    }
+    GEM_LAUNCH_CHECK(write ? "k_tables<write>" : "k_tables<count>");
+  };
+  launch(false);
+  const size_t nt = desc_tab.size();
+  uint64_t *h = pinned<uint64_t>(c, nt + GEM_MAX_QV);
+  for (size_t k = 0; k < nt; ++k) {
     TableDev &T = Qd.tab[desc_tab[k]];
     const uint64_t nrow = Qd.ncan[T.key_u];
     scan_u64(c, T.off.as<uint64_t>(), T.off.as<uint64_t>(), nrow);
     GEM_HIP_CHECK(hipMemcpyAsync(h + k, T.off.as<uint64_t>() + nrow, 8, hipMemcpyDeviceToHost, s));
   }
+  GEM_HIP_CHECK(hipMemcpyAsync(h + nt, adj_len.p, GEM_MAX_QV * 8, hipMemcpyDeviceToHost, s));
   GEM_HIP_CHECK(hipStreamSynchronize(s));
-  for (size_t k = 0; k < desc_tab.size(); ++k) {
+  // algorithmic bytes: per keyed u, its candidates (4 B), offset pairs (8 B)
+  // and adjacency (4 B per slot) once for all its tables; per table the value
+  // vertex's bitmap (|V|/8) and the row counts / offsets (8 B per row)
+  double shared_bytes = 0, count_bytes = 0, write_bytes = 0;
+  for (uint32_t ju = 0; ju < job.nu; ++ju) shared_bytes += 12.0 * job.ncan[ju] + 4.0 * (double)h[nt + ju];
+  for (size_t k = 0; k < nt; ++k) {
     TableDev &T = Qd.tab[desc_tab[k]];
     T.size = h[k];
+    count_bytes += 8.0 * Qd.W + 8.0 * Qd.ncan[T.key_u];
+    write_bytes += 12.0 * Qd.W + 8.0 * Qd.ncan[T.key_u] + 4.0 * (double)T.size;
     T.val.alloc(std::max<uint64_t>(T.size, 1) * 4, s);
   }
+  prof_bytes(c, K_TABLES_COUNT, shared_bytes + count_bytes);
+  prof_bytes(c, K_TABLES_WRITE, shared_bytes + write_bytes);
   fill_desc();
   GEM_HIP_CHECK(hipMemcpyAsync(ddesc.p, desc.data(), desc.size() * sizeof(TableDesc), hipMemcpyHostToDevice, s));
-  if (blocks) {
-    k_tables<true><<<blocks, TAB_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(),
-                                                     Qd.bitmap.as<uint64_t>(), Qd.rank.as<uint32_t>(), job,
-                                                     ddesc.as<TableDesc>());
-    GEM_LAUNCH_CHECK("k_tables<write>");
-  }
+  launch(true);
   GEM_HIP_CHECK(hipEventRecord(c->ev[3], s));
   GEM_HIP_CHECK(hipStreamSynchronize(s));
+  prof_flush(c);
   float ms = 0;
   GEM_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
   c->times.tables_ms = ms;

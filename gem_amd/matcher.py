@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import GemError, JoinResult, PhaseTimes, StepLog, load
+from ._lib import GemError, JoinResult, KernelStat, PhaseTimes, StepLog, load
 
 NOT_FOUND = 0xFFFFFFFF
 
@@ -122,17 +122,41 @@ class GemContext:
             out["embeddings"] = emb[:w * self.query.num_v].reshape(w, self.query.num_v) if w else None
         return out
 
+    def set_profiling(self, on=True):
+        self._check(self._lib.gem_hip_set_profiling(self._h, 1 if on else 0), "gem_hip_set_profiling")
+
+    def kernel_stats(self):
+        """{kernel name: dict(launches, ms, bytes)} accumulated while profiling."""
+        arr = (KernelStat * 32)()
+        n = C.c_uint32()
+        self._check(self._lib.gem_hip_kernel_stats(self._h, arr, 32, C.byref(n)), "gem_hip_kernel_stats")
+        return {arr[i].name.decode(): {"launches": int(arr[i].launches), "ms": float(arr[i].ms),
+                                       "bytes": float(arr[i].bytes)} for i in range(n.value)}
+
     def phase_times(self):
         pt = PhaseTimes()
         self._check(self._lib.gem_hip_phase_times(self._h, C.byref(pt)), "gem_hip_phase_times")
         return {"filter_ms": pt.filter_ms, "index_ms": pt.index_ms, "tables_ms": pt.tables_ms,
                 "join_ms": pt.join_ms}
 
-    def match(self, query, cap=0):
-        """Whole plain pipeline with GEM order; returns (count, saturated, order)."""
+    def set_work_limit(self, rows):
+        """Stop a join after `rows` processed rows (0 = no limit)."""
+        self._check(self._lib.gem_hip_set_work_limit(self._h, int(rows)), "gem_hip_set_work_limit")
+
+    def match_full(self, query, cap=0):
+        """Whole plain pipeline with GEM order: dict(count, saturated,
+        work_limited, rows_processed, order)."""
         self.query = query
         res = JoinResult()
         order = np.zeros(query.num_e, np.uint32)
         v = query.view()
         self._check(self._lib.gem_hip_match(self._h, C.byref(v), cap, C.byref(res), _p32(order)), "gem_hip_match")
-        return int(res.count), bool(res.saturated), order
+        return {"count": int(res.count), "saturated": bool(res.saturated), "work_limited": bool(res.work_limited),
+                "rows_processed": int(res.rows_processed), "order": order}
+
+    def match(self, query, cap=0):
+        """Whole plain pipeline with GEM order; returns (count, saturated, order)."""
+        r = self.match_full(query, cap)
+        if r["work_limited"]:
+            raise GemError(7, "join stopped at the work limit")
+        return r["count"], r["saturated"], r["order"]

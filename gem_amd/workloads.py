@@ -1,0 +1,72 @@
+"""Seeded synthetic workloads (data graph + query set) for the benchmark and
+the parity tests.
+
+  er10k  Erdos-Renyi, 10,000 vertices, 50,000 edges, 8 uniform labels;
+         20 random-walk 5-vertex queries.
+  rmat20 R-MAT 2^20 vertices, average degree 16 (a,b,c,d = .57,.19,.19,.05),
+         16 Zipf(s=1) labels; 100 random-walk 8-vertex queries, 50 dense
+         (average degree >= 3) and 50 sparse, by rejection sampling.
+  rmat22 R-MAT 2^22 vertices, average degree 16, 32 Zipf labels; 512
+         random-walk queries with sizes uniform in [4, 32], mixed density.
+  rmat20_learn  the rmat20 graph with 256 queries of 8-24 vertices.
+"""
+import numpy as np
+
+from .graph import Graph, Query
+
+WORKLOADS = {
+    "er10k": dict(kind="er", n=10000, m=50000, labels=8, seed=1001),
+    "rmat20": dict(kind="rmat", scale=20, degree=16.0, labels=16, seed=1002),
+    "rmat22": dict(kind="rmat", scale=22, degree=16.0, labels=32, seed=1003),
+    "rmat20_learn": dict(kind="rmat", scale=20, degree=16.0, labels=16, seed=1002),
+}
+
+CAP = 10 ** 9  # enumeration cap for the R-MAT workloads
+
+
+def make_graph(name):
+    w = WORKLOADS[name]
+    if w["kind"] == "er":
+        return Graph.erdos_renyi(w["n"], w["m"], w["labels"], w["seed"])
+    return Graph.rmat(w["scale"], w["degree"], w["labels"], w["seed"])
+
+
+def make_queries(name, graph, count=None):
+    seed = WORKLOADS[name]["seed"]
+    qs = []
+    if name == "er10k":
+        n = count or 20
+        i = 0
+        while len(qs) < n:
+            q = Query.random_walk(graph, 5, seed * 1000 + i)
+            i += 1
+            if q.num_e >= 2:
+                qs.append(q)
+        return qs
+    if name == "rmat20":
+        n = count or 100
+        dense, sparse = [], []
+        i = 0
+        while len(dense) < n // 2 or len(sparse) < n - n // 2:
+            q = Query.random_walk(graph, 8, seed * 1000 + i)
+            i += 1
+            if q.dense and len(dense) < n // 2:
+                dense.append(q)
+            elif not q.dense and len(sparse) < n - n // 2:
+                sparse.append(q)
+        # interleave so any prefix is mixed
+        for d, s in zip(dense, sparse):
+            qs += [d, s]
+        return qs
+    lo, hi, n = (4, 32, count or 512) if name == "rmat22" else (8, 24, count or 256)
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(lo, hi + 1, size=n)
+    i = 0
+    for k in sizes:
+        while True:
+            q = Query.random_walk(graph, int(k), seed * 100000 + i)
+            i += 1
+            if q.num_e >= 2:
+                qs.append(q)
+                break
+    return qs

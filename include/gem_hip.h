@@ -178,7 +178,15 @@ typedef struct gem_join_result {
   uint32_t saturated;  /* 1 when the count exceeded `cap` (count == cap) */
   uint32_t num_steps;  /* join steps run (= num_e - 1)                   */
   uint64_t num_embeddings_written; /* rows stored to emb_out            */
+  uint64_t rows_processed; /* rows read by the join's steps              */
+  uint32_t work_limited;   /* 1: stopped at the work limit, count unknown */
+  uint32_t reserved;
 } gem_join_result;
+
+/* Stop a join once it has processed more than `rows` rows (0 = no limit);
+ * the result then has work_limited = 1 and no valid count.  No reference
+ * counterpart (the reference has no time or work limit). */
+int gem_hip_set_work_limit(gem_ctx *ctx, uint64_t rows);
 
 /* Optional per-step log: rows after step s (s = 0 is the first edge table),
  * time of the step in milliseconds, and whether it was a select. */
@@ -210,6 +218,20 @@ typedef struct gem_phase_times {
   float join_ms;
 } gem_phase_times;
 int gem_hip_phase_times(const gem_ctx *ctx, gem_phase_times *out);
+
+/* Kernel-level profiling (no reference counterpart; the reference times
+ * phases on the host only, in-core/src/main.cu:147-227).  While on, every
+ * launch of the library's kernels is bracketed by HIP events on the context
+ * stream and its algorithmic bytes (compulsory traffic) are accumulated.
+ * gem_hip_set_profiling(ctx, 1) also clears the statistics. */
+typedef struct gem_kernel_stat {
+  char name[40];
+  uint64_t launches;
+  double ms;    /* summed launch durations */
+  double bytes; /* summed algorithmic bytes */
+} gem_kernel_stat;
+int gem_hip_set_profiling(gem_ctx *ctx, int on);
+int gem_hip_kernel_stats(gem_ctx *ctx, gem_kernel_stat *out, uint32_t max, uint32_t *n);
 
 /* Whole plain pipeline for one query: filter -> index -> tables -> GEM order
  * -> join.  Equivalent to the reference's main() for one query at -t 0. */
