@@ -40,6 +40,11 @@ class StepLog(C.Structure):
                 ("rows", C.c_uint64 * GEM_MAX_QE), ("ms", C.c_float * GEM_MAX_QE)]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("cap", C.c_uint64), ("work_limit", C.c_uint64), ("level_budget", C.c_uint64),
+                ("group", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 40), ("launches", C.c_uint64), ("ms", C.c_double), ("bytes", C.c_double)]
 
@@ -83,6 +88,9 @@ SIGNATURES = [
     ("gem_hip_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_kernel_stats", C.c_int, [C.c_void_p, C.POINTER(KernelStat), C.c_uint32, u32p]),
     ("gem_hip_match", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint64, C.POINTER(JoinResult), u32p]),
+    ("gem_hip_match_batch", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, C.POINTER(u32p),
+                                      C.POINTER(BatchOpts), C.POINTER(JoinResult), u64p]),
+    ("gem_hip_vjoin", C.c_int, [C.c_void_p, u32p, C.c_uint64, C.POINTER(JoinResult), u64p]),
 ]
 
 _lib = None

@@ -230,6 +230,21 @@ int gem_hip_kernel_stats(gem_ctx *c, gem_kernel_stat *out, uint32_t max, uint32_
   });
 }
 
+int gem_hip_match_batch(gem_ctx *c, const gem_query_view *queries, uint32_t nq, const uint32_t *const *vertex_orders,
+                        const gem_batch_opts *opts, gem_join_result *results, uint64_t *level_rows) {
+  return guard(c, [&]() {
+    if (nq && (!queries || !results)) throw Error{GEM_E_INVALID, "null argument"};
+    run_match_batch(c, queries, nq, vertex_orders, opts, results, level_rows);
+  });
+}
+
+int gem_hip_vjoin(gem_ctx *c, const uint32_t *vertex_order, uint64_t cap, gem_join_result *out, uint64_t *level_rows) {
+  return guard(c, [&]() {
+    if (!out) throw Error{GEM_E_INVALID, "null result"};
+    run_vjoin(c, vertex_order, cap, out, level_rows);
+  });
+}
+
 int gem_hip_phase_times(const gem_ctx *c, gem_phase_times *out) {
   if (!c || !out) return GEM_E_INVALID;
   *out = c->times;

@@ -1,0 +1,91 @@
+// Host orchestration of the batched path: per group of queries, filter and
+// edge tables for each query, a vertex plan per query (given, or derived from
+// GEM's edge order), then one VBatchRunner over the whole group.
+#include <algorithm>
+
+#include "ctx.h"
+#include "vjoin.h"
+
+namespace gemd {
+
+static void plan_query(gem_ctx *c, VQueryState &S, const QueryDev &Qd, const uint32_t *vorder_in) {
+  const gemh::QueryInfo &Q = Qd.Q;
+  std::vector<uint32_t> vorder;
+  if (vorder_in) {
+    vorder.assign(vorder_in, vorder_in + Q.nv);
+  } else {
+    std::vector<uint64_t> sz(Q.ne);
+    for (uint32_t e = 0; e < Q.ne; ++e) {
+      const TableDev &a = Qd.tab[2 * e], &b = Qd.tab[2 * e + 1];
+      sz[e] = a.present ? a.size : b.size;
+    }
+    std::vector<uint32_t> order(Q.ne);
+    int st = gemh::gem_order(Q, sz.data(), order.data());
+    if (st) throw Error{st, "GEM ordering failed"};
+    st = gemh::vertex_order_from_edges(Q, order.data(), &vorder);
+    if (st) throw Error{st, "vertex order from edge order failed"};
+  }
+  if (gemh::make_vertex_plan(Q, vorder, &S.plan))
+    throw Error{GEM_E_INVALID, "invalid vertex order (leaf first, or a vertex without an earlier neighbour)"};
+  S.qd = &Qd;
+  build_vlevels(c, S);
+}
+
+static void fill_result(const VQueryState &S, uint64_t cap, gem_join_result *r, uint64_t *level_rows) {
+  r->count = S.work_limited ? 0 : (S.saturated ? cap : S.count);
+  r->saturated = S.saturated ? 1 : 0;
+  r->num_steps = S.qd->Q.nv - 1;
+  r->num_embeddings_written = 0;
+  r->rows_processed = S.processed;
+  r->work_limited = S.work_limited ? 1 : 0;
+  r->reserved = 0;
+  if (level_rows)
+    for (uint32_t i = 0; i < GEM_MAX_QV; ++i) level_rows[i] = i < S.level_rows.size() ? S.level_rows[i] : 0;
+}
+
+void run_match_batch(gem_ctx *c, const gem_query_view *qs, uint32_t nq, const uint32_t *const *vorders,
+                     const gem_batch_opts *opts, gem_join_result *results, uint64_t *level_rows) {
+  if (!c->g.loaded) throw Error{GEM_E_STATE, "no data graph loaded"};
+  gem_batch_opts o{};
+  if (opts) o = *opts;
+  const uint32_t group = o.group ? o.group : std::max<uint32_t>(nq, 1);
+  hipStream_t s = c->stream;
+  GEM_HIP_CHECK(hipEventRecord(c->ev[6], s));
+  for (uint32_t g0 = 0; g0 < nq; g0 += group) {
+    const uint32_t gn = std::min(group, nq - g0);
+    const uint64_t budget = o.level_budget ? o.level_budget : std::max<uint64_t>(64ull << 20, (16ull << 30) / gn);
+    std::vector<QueryDev> qd(gn);
+    std::vector<VQueryState> st(gn);
+    std::vector<VQueryState *> ptrs;
+    for (uint32_t i = 0; i < gn; ++i) {
+      run_filter(c, qd[i], &qs[g0 + i], 0);
+      run_tables(c, qd[i]);
+      plan_query(c, st[i], qd[i], vorders ? vorders[g0 + i] : nullptr);
+      ptrs.push_back(&st[i]);
+    }
+    VBatchRunner runner(c, o.cap, o.work_limit, budget);
+    runner.run(ptrs);
+    for (uint32_t i = 0; i < gn; ++i)
+      fill_result(st[i], o.cap, &results[g0 + i], level_rows ? level_rows + (uint64_t)(g0 + i) * GEM_MAX_QV : nullptr);
+  }
+  GEM_HIP_CHECK(hipEventRecord(c->ev[7], s));
+  GEM_HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0;
+  GEM_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+  c->times.join_ms = ms;
+  prof_flush(c);
+}
+
+void run_vjoin(gem_ctx *c, const uint32_t *vorder, uint64_t cap, gem_join_result *out, uint64_t *level_rows) {
+  QueryDev &Qd = c->q;
+  if (!Qd.tabled) throw Error{GEM_E_STATE, "edge tables have not been built"};
+  VQueryState S;
+  plan_query(c, S, Qd, vorder);
+  std::vector<VQueryState *> ptrs{&S};
+  VBatchRunner runner(c, cap, c->work_limit_rows, c->level_budget_bytes);
+  runner.run(ptrs);
+  fill_result(S, cap, out, level_rows);
+  prof_flush(c);
+}
+
+}  // namespace gemd

@@ -147,13 +147,17 @@ enum KernelId {
   K_SELECT_FINAL,
   K_EXPAND_WRITE,
   K_SELECT_WRITE,
+  K_VLEN,
+  K_VEXT_WRITE,
+  K_VEXT_FINAL,
+  K_VCOUNT1,
   K_NUM
 };
 inline const char *kernel_name(int id) {
   static const char *n[K_NUM] = {"k_filter", "k_filter_heavy", "k_compact", "k_tables<count>",
                                  "k_tables<write>",    "k_step_count<expand>", "k_step_count<select>",
                                  "k_step_count<expand,final>", "k_step_count<select,final>", "k_expand_write",
-                                 "k_select_write"};
+                                 "k_select_write", "k_vlen", "k_vext<write>", "k_vext<final>", "k_vcount1"};
   return n[id];
 }
 struct KStat {
@@ -259,6 +263,9 @@ T *pinned(gem_ctx *c, size_t n) {
 // ---------------------------------------------------------------- phases
 void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T);
 void run_tables(gem_ctx *c);
+// the same phases on an explicit per-query state (several queries resident)
+void run_filter(gem_ctx *c, QueryDev &Qd, const gem_query_view *qv, uint32_t T);
+void run_tables(gem_ctx *c, QueryDev &Qd);
 void run_position_lookup(gem_ctx *c, uint32_t u, const uint32_t *h_vertices, uint64_t n, uint32_t *h_pos);
 
 struct JoinOut {
@@ -270,6 +277,10 @@ struct JoinOut {
 };
 void run_join(gem_ctx *c, const uint32_t *order, uint64_t cap, JoinOut *out, gem_step_log *log, uint32_t *emb_out,
               uint64_t emb_capacity);
+
+void run_match_batch(gem_ctx *c, const gem_query_view *qs, uint32_t nq, const uint32_t *const *vorders,
+                     const gem_batch_opts *opts, gem_join_result *results, uint64_t *level_rows);
+void run_vjoin(gem_ctx *c, const uint32_t *vorder, uint64_t cap, gem_join_result *out, uint64_t *level_rows);
 
 // ------------------------------------------------------------------ scans
 // Exclusive scan of n u64 values in place-free form: out[i] = sum(in[0..i)),

@@ -187,9 +187,10 @@ __global__ void k_position_lookup(const uint64_t *__restrict__ bitmap, const uin
   pos[i] = p;
 }
 
-void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T) {
+void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T) { run_filter(c, c->q, qv, T); }
+
+void run_filter(gem_ctx *c, QueryDev &Qd, const gem_query_view *qv, uint32_t T) {
   if (!c->g.loaded) throw Error{GEM_E_STATE, "no data graph loaded"};
-  QueryDev &Qd = c->q;
   Qd = QueryDev();
   std::string why;
   int st = gemh::analyse_query(qv, &Qd.Q, &why);

@@ -180,8 +180,9 @@ k_tables_heavy(const uint64_t *__restrict__ goff, const uint32_t *__restrict__ g
   }
 }
 
-void run_tables(gem_ctx *c) {
-  QueryDev &Qd = c->q;
+void run_tables(gem_ctx *c) { run_tables(c, c->q); }
+
+void run_tables(gem_ctx *c, QueryDev &Qd) {
   if (!Qd.filtered) throw Error{GEM_E_STATE, "filter has not run"};
   const gemh::QueryInfo &Q = Qd.Q;
   const GraphDev &G = c->g;

@@ -1,0 +1,71 @@
+// Vertex-at-a-time batched join: device descriptors and the host driver.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "ctx.h"
+
+namespace gemd {
+
+// One level of one query: extend rows (positions of vorder[0..i-1]) by
+// vorder[i].  Device copy, one per (query, level).
+struct VLevel {
+  uint32_t nb, ndup;
+  uint32_t key_col[GEM_MAX_QV];
+  const uint64_t *toff[GEM_MAX_QV];
+  const uint32_t *tval[GEM_MAX_QV];
+  uint32_t dup_col[GEM_MAX_QV];
+  const uint32_t *dup_can[GEM_MAX_QV];
+  const uint64_t *bm_new;
+  const uint32_t *rank_new;
+};
+
+// One unit of a launch: a row range of one pending level of one query.
+struct VItem {
+  const VLevel *lv;
+  const uint32_t *rows;
+  uint64_t row_begin, row_end;
+  uint32_t k, pad;
+  uint32_t *out;
+  uint64_t out_cap;
+  unsigned long long *counter;
+};
+
+struct VStackItem {
+  uint32_t level;                  // the level these rows extend to next
+  std::shared_ptr<DevBuf> rows;
+  uint64_t nrows;
+  uint32_t k;                      // columns per row
+  std::vector<uint64_t> block_u;   // prefix of per-block output bounds
+  uint64_t cursor;                 // next block to process
+  bool measured;
+};
+
+struct VQueryState {
+  const QueryDev *qd = nullptr;
+  gemh::VertexPlan plan;
+  DevBuf levels;  // VLevel[nv]
+  std::vector<uint32_t> max_nb, max_ndup;
+  std::vector<VStackItem> stack;
+  std::vector<uint64_t> level_rows;  // rows after each level
+  uint64_t count = 0, processed = 0;
+  bool saturated = false, done = false, work_limited = false;
+};
+
+void build_vlevels(gem_ctx *c, VQueryState &S);
+
+class VBatchRunner {
+ public:
+  VBatchRunner(gem_ctx *c, uint64_t cap, uint64_t work_limit, uint64_t budget_bytes)
+      : c_(c), cap_(cap), work_limit_(work_limit), budget_bytes_(budget_bytes) {}
+  void run(std::vector<VQueryState *> &qs);
+  uint64_t iterations() const { return iterations_; }
+
+ private:
+  gem_ctx *c_;
+  uint64_t cap_, work_limit_, budget_bytes_;
+  uint64_t iterations_ = 0;
+};
+
+}  // namespace gemd

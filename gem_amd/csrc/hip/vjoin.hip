@@ -1,0 +1,556 @@
+// Vertex-at-a-time join over many queries at once (the batched path).
+//
+// Same counts as the edge-at-a-time join (the reference's join(),
+// in-core/lib/join/src/join_trie.cu:1087-1277), which is order-invariant, but
+// each level adds one query vertex and applies ALL its edges to earlier
+// vertices at once: the candidates of the new vertex are the intersection of
+// the backward tables' rows (sorted position lists in its candidate set), so a
+// closing edge prunes rows before they are ever written instead of after a
+// whole level has been materialised.
+//
+//   k_vlen    per 256-row block: sum over rows of the shortest list length
+//             (an upper bound on the level's output, used to size and split)
+//   k_vext    waves walk the shortest ("driver") list of each of their 64
+//             rows, 64 items per step; each item is binary-searched in the
+//             row's other lists and compared with the equal-label columns;
+//             survivors are counted (last level) or appended to the output
+//             (one atomic reservation per wave step; row order is free)
+//   k_vcount1 last level with a single backward list: |list| minus the
+//             equal-label columns found in it (no walk)
+//
+// Work distribution: each launch covers many (query, level, row range) items;
+// blocks pull 256-row tasks from a device counter (persistent grid), so queries
+// of very different sizes share one launch.  The host keeps, per query, a
+// depth-first stack of pending levels: a level larger than the per-query budget
+// is processed in block ranges, the deepest pending range first, which bounds
+// memory; a query stops as soon as its count exceeds the cap.
+#include <algorithm>
+#include <memory>
+
+#include "ctx.h"
+#include "vjoin.h"
+#include "wave.h"
+
+namespace gemd {
+
+constexpr uint32_t VB = 256;  // rows per task block
+constexpr uint32_t VW = VB / 64;
+
+__device__ __forceinline__ uint32_t vposition_in(const uint64_t *bm, const uint32_t *rank, uint32_t v) {
+  const uint64_t word = bm[v >> 6];
+  const uint32_t b = v & 63;
+  if (!((word >> b) & 1ull)) return NOT_FOUND;
+  return rank[v >> 6] + (uint32_t)__popcll(word & ((1ull << b) - 1));
+}
+
+// task g (global block index of the launch) -> item t, row block within item
+__device__ __forceinline__ uint32_t find_item(const uint64_t *blk_prefix, uint32_t nitems, uint64_t g) {
+  uint32_t lo = 0, hi = nitems;  // largest t with blk_prefix[t] <= g
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (blk_prefix[mid] <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------------ length pass
+__global__ void __launch_bounds__(VB) k_vlen(const VItem *__restrict__ items, uint32_t nitems,
+                                            const uint64_t *__restrict__ blk_prefix,
+                                            unsigned int *__restrict__ queue, uint64_t *__restrict__ block_u) {
+  __shared__ uint64_t sm[VW];
+  __shared__ uint32_t task;
+  const uint64_t total = blk_prefix[nitems];
+  for (;;) {
+    if (threadIdx.x == 0) task = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint64_t g = task;
+    __syncthreads();
+    if (g >= total) break;
+    const uint32_t t = find_item(blk_prefix, nitems, g);
+    const VItem it = items[t];
+    const VLevel &L = *it.lv;
+    const uint64_t r = it.row_begin + (g - blk_prefix[t]) * VB + threadIdx.x;
+    uint64_t u = 0;
+    if (r < it.row_end) {
+      const uint32_t *row = it.rows + r * it.k;
+      u = ~0ull;
+      for (uint32_t j = 0; j < L.nb; ++j) {
+        const uint32_t p = row[L.key_col[j]];
+        const uint64_t len = L.toff[j][p + 1] - L.toff[j][p];
+        u = len < u ? len : u;
+      }
+    }
+    u = wave_sum_u64(u);
+    if (threadIdx.x % 64 == 0) sm[threadIdx.x / 64] = u;
+    __syncthreads();
+    if (threadIdx.x == 0) block_u[g] = sm[0] + sm[1] + sm[2] + sm[3];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------ extend (walk) pass
+// Dynamic LDS per wave: other lists (nb-1) x 64 x (u64 base + u32 len), dup
+// positions ndup x 64 u32.
+template <bool FINAL>
+__global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, uint32_t nitems,
+                                            const uint64_t *__restrict__ blk_prefix, unsigned int *__restrict__ queue,
+                                            uint32_t max_nb, uint32_t max_ndup,
+                                            unsigned long long *__restrict__ prof_items) {
+  __shared__ uint32_t task;
+  __shared__ uint32_t st[VW][65];
+  __shared__ uint64_t drv[VW][64];
+  __shared__ uint32_t dlist[VW][64];  // which backward list drives each row
+  extern __shared__ unsigned char vlds[];
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const uint32_t per_wave = 64 * ((max_nb > 0 ? max_nb - 1 : 0) * 12 + max_ndup * 4);
+  unsigned char *mine = vlds + (size_t)w * per_wave;
+  uint64_t *obase = reinterpret_cast<uint64_t *>(mine);                                  // [nb-1][64]
+  uint32_t *olen = reinterpret_cast<uint32_t *>(mine + (size_t)64 * 8 * (max_nb - 1));   // [nb-1][64]
+  uint32_t *dpos = reinterpret_cast<uint32_t *>(mine + (size_t)64 * 12 * (max_nb - 1));  // [ndup][64]
+  const uint64_t total = blk_prefix[nitems];
+  for (;;) {
+    if (threadIdx.x == 0) task = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint64_t g = task;
+    __syncthreads();
+    if (g >= total) break;
+    const uint32_t t = find_item(blk_prefix, nitems, g);
+    const VItem it = items[t];
+    const VLevel &L = *it.lv;
+    const uint32_t nb = L.nb, ndup = L.ndup, k = it.k;
+    const uint64_t rbase = it.row_begin + (g - blk_prefix[t]) * VB + w * 64;
+    const uint64_t r = rbase + l;
+    const bool valid = r < it.row_end;
+    const uint32_t *row = it.rows + r * k;
+    // per-row lists: the shortest one drives, the others are probed
+    uint32_t dlen = 0, dj = 0;
+    uint64_t dbase = 0;
+    if (valid) {
+      dlen = 0xFFFFFFFFu;
+      for (uint32_t j = 0; j < nb; ++j) {
+        const uint32_t p = row[L.key_col[j]];
+        const uint64_t b = L.toff[j][p];
+        const uint32_t len = (uint32_t)(L.toff[j][p + 1] - b);
+        if (len < dlen) {
+          dlen = len;
+          dj = j;
+          dbase = b;
+        }
+      }
+      uint32_t o = 0;
+      for (uint32_t j = 0; j < nb; ++j) {
+        if (j == dj) continue;
+        const uint32_t p = row[L.key_col[j]];
+        const uint64_t b = L.toff[j][p];
+        obase[o * 64 + l] = b;
+        olen[o * 64 + l] = (uint32_t)(L.toff[j][p + 1] - b);
+        ++o;
+      }
+      for (uint32_t c = 0; c < ndup; ++c) {
+        const uint32_t vid = L.dup_can[c][row[L.dup_col[c]]];
+        dpos[c * 64 + l] = vposition_in(L.bm_new, L.rank_new, vid);
+      }
+    }
+    const uint32_t incl = wave_incl_scan_u32(dlen);
+    const uint32_t wtotal = readlane_u32(incl, 63);
+    st[w][l] = incl - dlen;
+    if (l == 63) st[w][64] = wtotal;
+    drv[w][l] = dbase;
+    dlist[w][l] = dj;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint64_t survivors = 0;
+    for (uint32_t s = 0; s < wtotal; s += 64) {
+      const uint32_t item = s + l;
+      const bool iv = item < wtotal;
+      bool keep = false;
+      uint32_t val = 0, rw = 0;
+      if (iv) {
+        rw = seg_of(st[w], item);
+        const uint32_t j0 = dlist[w][rw];
+        val = L.tval[j0][drv[w][rw] + (item - st[w][rw])];
+        keep = true;
+        for (uint32_t c = 0; c < ndup && keep; ++c)
+          if (dpos[c * 64 + rw] == val) keep = false;
+        uint32_t o = 0;
+        for (uint32_t j = 0; j < nb && keep; ++j) {
+          if (j == j0) continue;
+          keep = sorted_contains(L.tval[j] + obase[o * 64 + rw], olen[o * 64 + rw], val);
+          ++o;
+        }
+      }
+      const uint64_t m = __ballot(keep);
+      const uint32_t nk = (uint32_t)__popcll(m);
+      if (FINAL) {
+        survivors += nk;
+      } else if (nk) {
+        uint64_t base = 0;
+        if (l == 0) base = atomicAdd(it.counter, (unsigned long long)nk);
+        base = readlane_u64(base, 0);
+        if (keep) {
+          const uint64_t o = base + rank_below(m);
+          if (o < it.out_cap) {
+            const uint32_t *src = it.rows + (rbase + rw) * k;
+            uint32_t *dst = it.out + o * (k + 1);
+            for (uint32_t c = 0; c < k; ++c) dst[c] = src[c];
+            dst[k] = val;
+          }
+        }
+      }
+    }
+    if (FINAL && l == 0 && survivors) atomicAdd(it.counter, (unsigned long long)survivors);
+    if (prof_items && l == 0 && wtotal) atomicAdd(prof_items, (unsigned long long)wtotal);
+    __syncthreads();
+  }
+}
+
+// Last level, one backward list: |list| - equal-label columns present.
+__global__ void __launch_bounds__(VB) k_vcount1(const VItem *__restrict__ items, uint32_t nitems,
+                                               const uint64_t *__restrict__ blk_prefix,
+                                               unsigned int *__restrict__ queue) {
+  __shared__ uint32_t task;
+  __shared__ uint64_t sm[VW];
+  const uint64_t total = blk_prefix[nitems];
+  for (;;) {
+    if (threadIdx.x == 0) task = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint64_t g = task;
+    __syncthreads();
+    if (g >= total) break;
+    const uint32_t t = find_item(blk_prefix, nitems, g);
+    const VItem it = items[t];
+    const VLevel &L = *it.lv;
+    const uint64_t r = it.row_begin + (g - blk_prefix[t]) * VB + threadIdx.x;
+    uint64_t cnt = 0;
+    if (r < it.row_end) {
+      const uint32_t *row = it.rows + r * it.k;
+      const uint32_t p = row[L.key_col[0]];
+      const uint64_t b = L.toff[0][p], e = L.toff[0][p + 1];
+      cnt = e - b;
+      for (uint32_t c = 0; c < L.ndup; ++c) {
+        const uint32_t pos = vposition_in(L.bm_new, L.rank_new, L.dup_can[c][row[L.dup_col[c]]]);
+        if (pos != NOT_FOUND && sorted_contains(L.tval[0] + b, e - b, pos)) --cnt;
+      }
+    }
+    cnt = wave_sum_u64(cnt);
+    if (threadIdx.x % 64 == 0) sm[threadIdx.x / 64] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint64_t s = sm[0] + sm[1] + sm[2] + sm[3];
+      if (s) atomicAdd(it.counter, (unsigned long long)s);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_viota(uint32_t *p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
+}
+
+// ================================================================== host
+
+void build_vlevels(gem_ctx *c, VQueryState &S) {
+  const QueryDev &Qd = *S.qd;
+  const gemh::VertexPlan &P = S.plan;
+  const uint32_t nv = Qd.Q.nv;
+  std::vector<VLevel> lv(nv);
+  for (uint32_t i = 1; i < nv; ++i) {
+    const gemh::VLevelPlan &lp = P.levels[i];
+    VLevel &L = lv[i];
+    L.nb = lp.nb;
+    L.ndup = lp.ndup;
+    for (uint32_t j = 0; j < lp.nb; ++j) {
+      const TableDev &T = Qd.tab[lp.directed[j]];
+      if (!T.present) throw Error{GEM_E_STATE, "vertex plan uses a table that was not built"};
+      L.key_col[j] = lp.key_col[j];
+      L.toff[j] = T.off.as<uint64_t>();
+      L.tval[j] = T.val.as<uint32_t>();
+    }
+    const uint32_t u = P.vorder[i];
+    for (uint32_t d = 0; d < lp.ndup; ++d) {
+      L.dup_col[d] = lp.dup_col[d];
+      L.dup_can[d] = Qd.can.as<uint32_t>() + Qd.can_off[P.vorder[lp.dup_col[d]]];
+    }
+    L.bm_new = Qd.bitmap.as<uint64_t>() + (uint64_t)u * Qd.W;
+    L.rank_new = Qd.rank.as<uint32_t>() + (uint64_t)u * (Qd.W + 1);
+  }
+  S.levels.alloc(sizeof(VLevel) * nv, c->stream);
+  GEM_HIP_CHECK(hipMemcpyAsync(S.levels.p, lv.data(), sizeof(VLevel) * nv, hipMemcpyHostToDevice, c->stream));
+  S.max_nb.assign(nv, 0);
+  S.max_ndup.assign(nv, 0);
+  for (uint32_t i = 1; i < nv; ++i) {
+    S.max_nb[i] = lv[i].nb;
+    S.max_ndup[i] = lv[i].ndup;
+  }
+}
+
+namespace {
+
+struct Launch {
+  std::vector<VItem> items;
+  std::vector<uint64_t> blk;  // prefix, items.size() + 1
+  uint32_t max_nb = 1, max_ndup = 0;
+  void add(const VItem &it, uint32_t nb, uint32_t ndup) {
+    if (blk.empty()) blk.push_back(0);
+    items.push_back(it);
+    blk.push_back(blk.back() + (it.row_end - it.row_begin + VB - 1) / VB);
+    max_nb = std::max(max_nb, nb);
+    max_ndup = std::max(max_ndup, ndup);
+  }
+  bool empty() const { return items.empty(); }
+};
+
+// Upload an item table + block prefix, return device pointers (kept alive by bufs).
+struct DevLaunch {
+  DevBuf items, blk, queue;
+};
+
+void upload(gem_ctx *c, const Launch &L, DevLaunch &D) {
+  hipStream_t s = c->stream;
+  D.items.alloc(sizeof(VItem) * L.items.size(), s);
+  D.blk.alloc(8 * L.blk.size(), s);
+  D.queue.alloc(4, s);
+  GEM_HIP_CHECK(hipMemcpyAsync(D.items.p, L.items.data(), sizeof(VItem) * L.items.size(), hipMemcpyHostToDevice, s));
+  GEM_HIP_CHECK(hipMemcpyAsync(D.blk.p, L.blk.data(), 8 * L.blk.size(), hipMemcpyHostToDevice, s));
+  GEM_HIP_CHECK(hipMemsetAsync(D.queue.p, 0, 4, s));
+}
+
+uint32_t grid_for(uint64_t blocks) {
+  // persistent: enough blocks to fill the chip, never more than the tasks
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 2048));
+}
+
+}  // namespace
+
+void VBatchRunner::run(std::vector<VQueryState *> &qs) {
+  hipStream_t s = c_->stream;
+  const uint64_t budget = std::max<uint64_t>(budget_bytes_, 1ull << 20);
+  // seed each query with the identity level of its first vertex
+  for (VQueryState *S : qs) {
+    const uint32_t a = S->plan.vorder[0];
+    const uint64_t n0 = S->qd->ncan[a];
+    S->level_rows.assign(S->qd->Q.nv, 0);
+    S->level_rows[0] = n0;
+    if (n0 == 0) {
+      S->done = true;
+      continue;
+    }
+    auto buf = std::make_shared<DevBuf>();
+    buf->alloc(n0 * 4, s);
+    k_viota<<<(uint32_t)((n0 + 255) / 256), 256, 0, s>>>(buf->as<uint32_t>(), n0);
+    GEM_LAUNCH_CHECK("k_viota");
+    S->stack.push_back(VStackItem{1, buf, n0, 1, {}, 0, false});
+  }
+  DevBuf counters;
+  for (;;) {
+    std::vector<VQueryState *> act;
+    for (VQueryState *S : qs)
+      if (!S->done && !S->stack.empty()) act.push_back(S);
+    if (act.empty()) break;
+    ++iterations_;
+    // ---- length pass for unmeasured, non-final top items
+    Launch lenL;
+    std::vector<VQueryState *> len_owner;
+    for (VQueryState *S : act) {
+      VStackItem &top = S->stack.back();
+      const bool final_level = top.level + 1 == S->qd->Q.nv;
+      if (final_level || top.measured) continue;
+      VItem it{};
+      it.lv = S->levels.as<VLevel>() + top.level;
+      it.rows = top.rows->as<uint32_t>();
+      it.row_begin = 0;
+      it.row_end = top.nrows;
+      it.k = top.k;
+      lenL.add(it, S->max_nb[top.level], S->max_ndup[top.level]);
+      len_owner.push_back(S);
+    }
+    if (!lenL.empty()) {
+      DevLaunch D;
+      upload(c_, lenL, D);
+      const uint64_t nblk = lenL.blk.back();
+      DevBuf bu, bp;
+      bu.alloc(nblk * 8, s);
+      bp.alloc((nblk + 1) * 8, s);
+      {
+        ProfLaunch pl(c_, K_VLEN);
+        k_vlen<<<grid_for(nblk), VB, 0, s>>>(D.items.as<VItem>(), (uint32_t)lenL.items.size(), D.blk.as<uint64_t>(),
+                                              D.queue.as<unsigned int>(), bu.as<uint64_t>());
+        GEM_LAUNCH_CHECK("k_vlen");
+      }
+      scan_u64(c_, bu.as<uint64_t>(), bp.as<uint64_t>(), nblk);
+      std::vector<uint64_t> pref(nblk + 1);
+      GEM_HIP_CHECK(hipMemcpyAsync(pref.data(), bp.p, (nblk + 1) * 8, hipMemcpyDeviceToHost, s));
+      GEM_HIP_CHECK(hipStreamSynchronize(s));
+      for (size_t t = 0; t < len_owner.size(); ++t) {
+        VStackItem &top = len_owner[t]->stack.back();
+        const uint64_t b0 = lenL.blk[t], b1 = lenL.blk[t + 1];
+        top.block_u.assign(pref.begin() + b0, pref.begin() + b1 + 1);
+        const uint64_t base = top.block_u[0];
+        for (auto &x : top.block_u) x -= base;
+        top.measured = true;
+        // rows + one offset pair per backward list
+        prof_bytes(c_, K_VLEN, (double)top.nrows * (4.0 * top.k + 16.0 * len_owner[t]->max_nb[top.level]));
+      }
+    }
+    // ---- extend pass: write chunks and final counts
+    Launch wr, fin, fin1;
+    struct WTask {
+      VQueryState *S;
+      std::shared_ptr<DevBuf> out;
+      uint32_t level, k;
+      uint64_t cap_rows;
+      size_t ctr;
+    };
+    struct FTask {
+      VQueryState *S;
+      size_t ctr;
+    };
+    std::vector<WTask> wt;
+    std::vector<FTask> ft;
+    std::vector<std::shared_ptr<DevBuf>> keep_alive;  // input rows stay allocated until the launches finish
+    size_t nctr = 0;  // one u64 counter per task
+    for (VQueryState *S : act) {
+      VStackItem &top = S->stack.back();
+      const uint32_t nv = S->qd->Q.nv;
+      const bool final_level = top.level + 1 == nv;
+      const uint64_t nblocks = (top.nrows + VB - 1) / VB;
+      VItem it{};
+      it.lv = S->levels.as<VLevel>() + top.level;
+      it.rows = top.rows->as<uint32_t>();
+      it.k = top.k;
+      keep_alive.push_back(top.rows);
+      if (final_level) {
+        it.row_begin = (uint64_t)top.cursor * VB;
+        it.row_end = top.nrows;
+        it.counter = reinterpret_cast<unsigned long long *>((uintptr_t)nctr);  // patched below
+        ft.push_back(FTask{S, nctr++});
+        S->processed += it.row_end - it.row_begin;
+        if (S->max_nb[top.level] == 1)
+          fin1.add(it, 1, S->max_ndup[top.level]);
+        else
+          fin.add(it, S->max_nb[top.level], S->max_ndup[top.level]);
+        S->stack.pop_back();
+        continue;
+      }
+      // one chunk of blocks whose output bound fits the per-query budget
+      const uint64_t row_bytes = 4ull * (top.k + 1);
+      const uint64_t lim = std::max<uint64_t>(budget / row_bytes, 1);
+      uint64_t b0 = top.cursor, b1 = b0 + 1;
+      while (b1 < nblocks && top.block_u[b1 + 1] - top.block_u[b0] <= lim) ++b1;
+      const uint64_t ub = top.block_u[b1] - top.block_u[b0];
+      it.row_begin = b0 * VB;
+      it.row_end = std::min<uint64_t>(b1 * VB, top.nrows);
+      S->processed += it.row_end - it.row_begin;
+      top.cursor = b1;
+      if (ub == 0) {
+        if (top.cursor >= nblocks) S->stack.pop_back();
+        continue;
+      }
+      auto out = std::make_shared<DevBuf>();
+      out->alloc(ub * row_bytes, s);
+      it.out = out->as<uint32_t>();
+      it.out_cap = ub;
+      it.counter = reinterpret_cast<unsigned long long *>((uintptr_t)nctr);
+      wt.push_back(WTask{S, out, top.level + 1, top.k + 1, ub, nctr++});
+      wr.add(it, S->max_nb[top.level], S->max_ndup[top.level]);
+      if (top.cursor >= nblocks) S->stack.pop_back();
+    }
+    if (nctr == 0) continue;
+    counters.alloc(nctr * 8, s);
+    GEM_HIP_CHECK(hipMemsetAsync(counters.p, 0, nctr * 8, s));
+    auto patch = [&](Launch &L) {
+      for (auto &it : L.items)
+        it.counter = counters.as<unsigned long long>() + (size_t)(uintptr_t)it.counter;
+    };
+    patch(wr);
+    patch(fin);
+    patch(fin1);
+    DevLaunch Dw, Df, Df1;
+    auto ext_lds = [](const Launch &L) {
+      const size_t b = (size_t)VW * 64 * ((L.max_nb > 0 ? L.max_nb - 1 : 0) * 12 + L.max_ndup * 4);
+      if (b > 60 * 1024) throw Error{GEM_E_LIMIT, "a level has too many backward edges for the vjoin LDS layout"};
+      return b;
+    };
+    // profiling: list entries walked per launch (written to counters[nctr])
+    const bool prof = c_->prof.on;
+    DevBuf pitems;
+    if (prof) {
+      pitems.alloc(16, s);
+      GEM_HIP_CHECK(hipMemsetAsync(pitems.p, 0, 16, s));
+    }
+    unsigned long long *pw = prof ? pitems.as<unsigned long long>() : nullptr;
+    if (!wr.empty()) {
+      upload(c_, wr, Dw);
+      ProfLaunch pl(c_, K_VEXT_WRITE);
+      k_vext<false><<<grid_for(wr.blk.back()), VB, ext_lds(wr), s>>>(
+          Dw.items.as<VItem>(), (uint32_t)wr.items.size(), Dw.blk.as<uint64_t>(), Dw.queue.as<unsigned int>(),
+          wr.max_nb, wr.max_ndup, pw);
+      GEM_LAUNCH_CHECK("k_vext<write>");
+    }
+    if (!fin.empty()) {
+      upload(c_, fin, Df);
+      ProfLaunch pl(c_, K_VEXT_FINAL);
+      k_vext<true><<<grid_for(fin.blk.back()), VB, ext_lds(fin), s>>>(
+          Df.items.as<VItem>(), (uint32_t)fin.items.size(), Df.blk.as<uint64_t>(), Df.queue.as<unsigned int>(),
+          fin.max_nb, fin.max_ndup, prof ? pw + 1 : nullptr);
+      GEM_LAUNCH_CHECK("k_vext<final>");
+    }
+    if (!fin1.empty()) {
+      upload(c_, fin1, Df1);
+      ProfLaunch pl(c_, K_VCOUNT1);
+      k_vcount1<<<grid_for(fin1.blk.back()), VB, 0, s>>>(Df1.items.as<VItem>(), (uint32_t)fin1.items.size(),
+                                                        Df1.blk.as<uint64_t>(), Df1.queue.as<unsigned int>());
+      GEM_LAUNCH_CHECK("k_vcount1");
+    }
+    std::vector<uint64_t> hc(nctr + 2, 0);
+    GEM_HIP_CHECK(hipMemcpyAsync(hc.data(), counters.p, nctr * 8, hipMemcpyDeviceToHost, s));
+    if (prof) GEM_HIP_CHECK(hipMemcpyAsync(hc.data() + nctr, pitems.p, 16, hipMemcpyDeviceToHost, s));
+    GEM_HIP_CHECK(hipStreamSynchronize(s));
+    if (prof) {
+      // algorithmic bytes: rows read (4 B per column + an offset pair per
+      // backward list), 4 B per list entry walked, output rows written
+      auto rows_bytes = [&](const Launch &L) {
+        double b = 0;
+        for (const auto &it : L.items)
+          b += (double)(it.row_end - it.row_begin) * (4.0 * it.k + 16.0 * (double)L.max_nb);
+        return b;
+      };
+      double outb = 0;
+      for (auto &w : wt) outb += 4.0 * w.k * (double)hc[w.ctr];
+      prof_bytes(c_, K_VEXT_WRITE, rows_bytes(wr) + 4.0 * (double)hc[nctr] + outb);
+      prof_bytes(c_, K_VEXT_FINAL, rows_bytes(fin) + 4.0 * (double)hc[nctr + 1]);
+      prof_bytes(c_, K_VCOUNT1, rows_bytes(fin1));
+    }
+    for (auto &w : wt) {
+      const uint64_t got = hc[w.ctr];
+      if (got > w.cap_rows) throw Error{GEM_E_STATE, "vjoin output exceeded its bound"};
+      w.S->level_rows[w.level - 1] += got;
+      if (got) w.S->stack.push_back(VStackItem{w.level, w.out, got, w.k, {}, 0, false});
+    }
+    for (auto &f : ft) {
+      const uint64_t got = hc[f.ctr];
+      f.S->level_rows[f.S->qd->Q.nv - 1] += got;
+      f.S->count += got;
+    }
+    for (VQueryState *S : act) {
+      if (cap_ && S->count > cap_) {
+        S->saturated = true;
+        S->done = true;
+        S->stack.clear();
+      } else if (work_limit_ && S->processed > work_limit_) {
+        S->work_limited = true;
+        S->done = true;
+        S->stack.clear();
+      } else if (S->stack.empty()) {
+        S->done = true;
+      }
+    }
+    prof_flush(c_);
+  }
+}
+
+}  // namespace gemd

@@ -226,6 +226,54 @@ int make_join_plan(const QueryInfo &q, const uint32_t *order, JoinPlan *plan) {
   return GEM_OK;
 }
 
+int vertex_order_from_edges(const QueryInfo &q, const uint32_t *edge_order, std::vector<uint32_t> *vorder) {
+  if (!order_is_connected(q, edge_order)) return GEM_E_INVALID;
+  std::vector<char> seen(q.nv, 0);
+  vorder->clear();
+  const uint32_t s = q.edges[edge_order[0]].first, d = q.edges[edge_order[0]].second;
+  const uint32_t a = q.deg[s] != 1 ? s : d, b = a == s ? d : s;
+  for (uint32_t v : {a, b}) {
+    vorder->push_back(v);
+    seen[v] = 1;
+  }
+  for (uint32_t i = 1; i < q.ne; ++i) {
+    const uint32_t x = q.edges[edge_order[i]].first, y = q.edges[edge_order[i]].second;
+    for (uint32_t v : {x, y})
+      if (!seen[v]) {
+        seen[v] = 1;
+        vorder->push_back(v);
+      }
+  }
+  return vorder->size() == q.nv ? GEM_OK : GEM_E_INVALID;
+}
+
+int make_vertex_plan(const QueryInfo &q, const std::vector<uint32_t> &vorder, VertexPlan *plan) {
+  if (vorder.size() != q.nv) return GEM_E_INVALID;
+  std::vector<int> col(q.nv, -1);
+  for (uint32_t i = 0; i < q.nv; ++i) {
+    if (vorder[i] >= q.nv || col[vorder[i]] >= 0) return GEM_E_INVALID;
+    col[vorder[i]] = (int)i;
+  }
+  if (q.deg[vorder[0]] == 1) return GEM_E_INVALID;  // no table is keyed on a leaf
+  plan->vorder = vorder;
+  plan->levels.assign(q.nv, VLevelPlan());
+  for (uint32_t i = 1; i < q.nv; ++i) {
+    const uint32_t u = vorder[i];
+    VLevelPlan &L = plan->levels[i];
+    for (auto [w, e] : q.adj[u]) {
+      if (col[w] >= (int)i) continue;
+      // table keyed on the earlier vertex w towards u
+      L.key_col[L.nb] = (uint32_t)col[w];
+      L.directed[L.nb] = 2 * e + (q.edges[e].first == w ? 0 : 1);
+      L.nb++;
+    }
+    if (L.nb == 0) return GEM_E_INVALID;
+    for (uint32_t c = 0; c < i; ++c)
+      if (q.lab[vorder[c]] == q.lab[u]) L.dup_col[L.ndup++] = c;
+  }
+  return GEM_OK;
+}
+
 }  // namespace gemh
 
 extern "C" int gem_plan_gem_order(const gem_query_view *q, const uint64_t *table_sizes, uint32_t *order_out) {

@@ -63,4 +63,27 @@ struct JoinPlan {
 // the columns of equal label (join_trie.cu:450-462).
 int make_join_plan(const QueryInfo &q, const uint32_t *order, JoinPlan *plan);
 
+// Vertex-at-a-time plan: level i adds query vertex vorder[i] and applies every
+// query edge between it and an earlier vertex (its backward neighbours) at
+// once, as an intersection of their tables; equal-label earlier vertices are
+// excluded (injectivity).  The final count does not depend on the order (the
+// count of injective, edge-preserving maps into the candidate sets).
+struct VLevelPlan {
+  uint32_t nb = 0;                   // backward neighbours
+  uint32_t key_col[GEM_MAX_QV] = {};
+  uint32_t directed[GEM_MAX_QV] = {};  // table key -> new vertex
+  uint32_t ndup = 0;
+  uint32_t dup_col[GEM_MAX_QV] = {};
+};
+struct VertexPlan {
+  std::vector<uint32_t> vorder;       // query vertex of each column
+  std::vector<VLevelPlan> levels;     // levels[i] extends to vorder[i] (i >= 1)
+};
+// Vertex order of first appearance along a connected edge order, starting at
+// the non-leaf endpoint of the first edge (as the reference's first table).
+int vertex_order_from_edges(const QueryInfo &q, const uint32_t *edge_order, std::vector<uint32_t> *vorder);
+// Requires vorder[0] to be a non-leaf and every later vertex to have an
+// earlier neighbour.
+int make_vertex_plan(const QueryInfo &q, const std::vector<uint32_t> &vorder, VertexPlan *plan);
+
 }  // namespace gemh

@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import GemError, JoinResult, KernelStat, PhaseTimes, StepLog, load
+from ._lib import GEM_MAX_QV, BatchOpts, GemError, JoinResult, KernelStat, PhaseTimes, QueryView, StepLog, load
 
 NOT_FOUND = 0xFFFFFFFF
 
@@ -153,6 +153,46 @@ class GemContext:
         self._check(self._lib.gem_hip_match(self._h, C.byref(v), cap, C.byref(res), _p32(order)), "gem_hip_match")
         return {"count": int(res.count), "saturated": bool(res.saturated), "work_limited": bool(res.work_limited),
                 "rows_processed": int(res.rows_processed), "order": order}
+
+    def vjoin(self, vertex_order=None, cap=0):
+        """Vertex-at-a-time join of the current query (after filter and
+        build_edge_tables); vertex_order None = from GEM's edge order."""
+        res = JoinResult()
+        lr = np.zeros(GEM_MAX_QV, np.uint64)
+        vo = None if vertex_order is None else np.ascontiguousarray(vertex_order, dtype=np.uint32)
+        self._check(self._lib.gem_hip_vjoin(self._h, _p32(vo) if vo is not None else None, cap, C.byref(res),
+                                            _p64(lr)), "gem_hip_vjoin")
+        return {"count": int(res.count), "saturated": bool(res.saturated), "work_limited": bool(res.work_limited),
+                "rows_processed": int(res.rows_processed), "level_rows": lr[:self.query.num_v].copy()}
+
+    def match_batch(self, queries, vertex_orders=None, cap=0, work_limit=0, level_budget=0, group=0):
+        """Batched vertex-at-a-time path over a list of queries.  Returns a
+        list of dict(count, saturated, work_limited, rows_processed, level_rows)."""
+        nq = len(queries)
+        views = (QueryView * max(nq, 1))(*[q.view() for q in queries])
+        keep = []
+        vo_arr = None
+        if vertex_orders is not None:
+            vo_arr = (C.POINTER(C.c_uint32) * max(nq, 1))()
+            for i, vo in enumerate(vertex_orders):
+                if vo is None:
+                    vo_arr[i] = C.POINTER(C.c_uint32)()
+                else:
+                    a = np.ascontiguousarray(vo, dtype=np.uint32)
+                    keep.append(a)
+                    vo_arr[i] = _p32(a)
+        opts = BatchOpts(int(cap), int(work_limit), int(level_budget), int(group), 0)
+        res = (JoinResult * max(nq, 1))()
+        lr = np.zeros(max(nq, 1) * GEM_MAX_QV, np.uint64)
+        self._check(self._lib.gem_hip_match_batch(self._h, views, nq, vo_arr, C.byref(opts), res, _p64(lr)),
+                    "gem_hip_match_batch")
+        out = []
+        for i, q in enumerate(queries):
+            r = res[i]
+            out.append({"count": int(r.count), "saturated": bool(r.saturated), "work_limited": bool(r.work_limited),
+                        "rows_processed": int(r.rows_processed),
+                        "level_rows": lr[i * GEM_MAX_QV:i * GEM_MAX_QV + q.num_v].copy()})
+        return out
 
     def match(self, query, cap=0):
         """Whole plain pipeline with GEM order; returns (count, saturated, order)."""

@@ -233,6 +233,35 @@ typedef struct gem_kernel_stat {
 int gem_hip_set_profiling(gem_ctx *ctx, int on);
 int gem_hip_kernel_stats(gem_ctx *ctx, gem_kernel_stat *out, uint32_t max, uint32_t *n);
 
+/* ------------------------------------------------------- batched path */
+/* The batched path has no single reference counterpart: it is the
+ * reference's per-query pipeline (in-core/src/main.cu:148-236, join() at
+ * in-core/lib/join/src/join_trie.cu:1087-1277) run for many queries at once,
+ * with the join done vertex-at-a-time: every edge between a new query vertex
+ * and earlier ones is applied in the same step (intersection of table rows).
+ * Counts equal the plain path's (the count does not depend on the order). */
+typedef struct gem_batch_opts {
+  uint64_t cap;          /* per-query count cap (0 = none)                  */
+  uint64_t work_limit;   /* rows a query may process (0 = none)             */
+  uint64_t level_budget; /* bytes of one pending level chunk per query (0 = auto) */
+  uint32_t group;        /* queries resident on the GPU at once (0 = all)   */
+  uint32_t reserved;
+} gem_batch_opts;
+
+/* Match nq queries.  vertex_orders (nullable, and each entry nullable) gives a
+ * query's vertex order (vertex_orders[i][0] must not be a leaf, every later
+ * vertex needs an earlier neighbour); absent orders come from GEM's edge order
+ * (first appearance).  results[nq]; level_rows (nullable) receives, per query,
+ * GEM_MAX_QV entries: rows after each vertex level. */
+int gem_hip_match_batch(gem_ctx *ctx, const gem_query_view *queries, uint32_t nq,
+                        const uint32_t *const *vertex_orders, const gem_batch_opts *opts,
+                        gem_join_result *results, uint64_t *level_rows);
+
+/* Vertex-at-a-time join of the current query (after gem_hip_filter and
+ * gem_hip_build_edge_tables) along vertex_order[num_v]. */
+int gem_hip_vjoin(gem_ctx *ctx, const uint32_t *vertex_order, uint64_t cap, gem_join_result *out,
+                  uint64_t *level_rows);
+
 /* Whole plain pipeline for one query: filter -> index -> tables -> GEM order
  * -> join.  Equivalent to the reference's main() for one query at -t 0. */
 int gem_hip_match(gem_ctx *ctx, const gem_query_view *q, uint64_t cap,

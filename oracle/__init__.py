@@ -40,6 +40,8 @@ def _load():
     lib.orc_count.argtypes = G + Qv + [u32p, C.c_uint64, C.c_uint64, u64p, C.POINTER(C.c_int), u64p, u32p,
                                        C.POINTER(C.c_double), C.c_int]
     lib.orc_enumerate.argtypes = G + Qv + [u32p, u32p, C.c_uint64, u64p, C.c_int]
+    lib.orc_vcount.argtypes = G + Qv + [u32p, C.c_uint64, C.c_uint64, u64p, C.POINTER(C.c_int), u64p, C.c_int]
+    lib.orc_vcount.restype = C.c_int
     lib.orc_bruteforce.argtypes = G + Qv + [u64p]
     for f in ("orc_filter", "orc_table_sizes", "orc_edge_table", "orc_gem_order", "orc_count", "orc_enumerate",
               "orc_bruteforce"):
@@ -139,6 +141,36 @@ def count(graph, query, order=None, cap=0, work_limit=0, nthreads=0):
     assert st == 0, st
     return {"count": int(cnt.value), "saturated": bool(sat.value), "step_rows": steps, "order": used,
             "ms": [ms[0], ms[1], ms[2]]}
+
+
+def vcount(graph, query, vorder, cap=0, work_limit=0, nthreads=0):
+    """Vertex-level counts along vorder: dict(count, saturated, level_rows) or
+    None at the work limit."""
+    keep_g, ga = _g(graph)
+    keep_q, qa = _q(query)
+    vo = np.ascontiguousarray(vorder, dtype=np.uint32)
+    cnt = C.c_uint64()
+    sat = C.c_int()
+    lr = np.zeros(query.num_v, np.uint64)
+    st = _load().orc_vcount(*ga, *qa, _p32(vo), cap, work_limit, C.byref(cnt), C.byref(sat), _p64(lr), nthreads)
+    if st == 2:
+        return None
+    assert st == 0, st
+    return {"count": int(cnt.value), "saturated": bool(sat.value), "level_rows": lr}
+
+
+def vertex_order_from_edges(query, order):
+    """First-appearance vertex order of an edge order, starting at the first
+    edge's non-leaf endpoint (restates the product's rule for tests)."""
+    deg = np.bincount(np.asarray(query.edges).reshape(-1), minlength=query.num_v)
+    s, d = (int(x) for x in query.edges[order[0]])
+    a = s if deg[s] != 1 else d
+    vo = [a, d if a == s else s]
+    for e in order[1:]:
+        for v in (int(x) for x in query.edges[e]):
+            if v not in vo:
+                vo.append(v)
+    return np.array(vo, np.uint32)
 
 
 def enumerate_embeddings(graph, query, order, capacity=1 << 22, nthreads=0):

@@ -686,6 +686,98 @@ int orc_count(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32
   return r;
 }
 
+// Vertex-level counts: level_rows[i] = number of injective maps of
+// {vorder[0..i]} into the candidate sets that preserve every query edge among
+// them.  Depth-first over vorder; candidates of vorder[i] are the data
+// neighbours of one earlier matched neighbour, checked against every other
+// earlier neighbour by binary search in the data adjacency (not through the
+// edge tables), so this pins the batched path's intermediate sizes
+// independently of the tables.  Returns 2 at the work limit.
+int orc_vcount(uint32_t n, const uint64_t *off, const uint32_t *nbr, const uint32_t *lab, uint32_t nv, uint32_t ne,
+               const uint32_t *qlab, const uint32_t *qedges, const uint32_t *vorder, uint64_t cap, uint64_t work_limit,
+               uint64_t *count, int *saturated, uint64_t *level_rows, int nthreads) {
+  set_threads(nthreads);
+  (void)n;
+  G g{n, off, nbr, lab};
+  Prepared pp;
+  if (prepare(g, nv, ne, qlab, qedges, nullptr, pp, false)) return 1;
+  const Q &q = pp.q;
+  std::vector<int> col(nv, -1);
+  for (uint32_t i = 0; i < nv; ++i) col[vorder[i]] = (int)i;
+  std::vector<std::vector<uint32_t>> back(nv), same(nv);
+  for (uint32_t i = 1; i < nv; ++i) {
+    for (auto [w, eid] : q.adj[vorder[i]])
+      if (col[w] < (int)i) back[i].push_back((uint32_t)col[w]);
+    for (uint32_t c = 0; c < i; ++c)
+      if (q.rlab[vorder[c]] == q.rlab[vorder[i]]) same[i].push_back(c);
+    if (back[i].empty()) return 1;
+  }
+  std::vector<std::vector<uint8_t>> incan(nv, std::vector<uint8_t>(n, 0));
+  for (uint32_t u = 0; u < nv; ++u)
+    for (uint32_t v : pp.can[u]) incan[u][v] = 1;
+  std::atomic<uint64_t> total{0}, work{0};
+  std::atomic<int> stop{0};
+  std::vector<uint64_t> lr(nv, 0);
+  const auto &c0 = pp.can[vorder[0]];
+#pragma omp parallel
+  {
+    std::vector<uint64_t> mylr(nv, 0);
+    uint64_t myfinal = 0, mywork = 0;
+    uint32_t img[MAXV];
+    std::function<void(uint32_t)> rec = [&](uint32_t i) {
+      if (stop.load(std::memory_order_relaxed)) return;
+      const uint32_t u = vorder[i];
+      const uint32_t a = img[back[i][0]];
+      for (uint64_t k = off[a]; k < off[a + 1]; ++k) {
+        const uint32_t v = nbr[k];
+        if (!incan[u][v]) continue;
+        bool ok = true;
+        for (size_t b = 1; b < back[i].size() && ok; ++b) {
+          const uint32_t x = img[back[i][b]];
+          ok = std::binary_search(nbr + off[x], nbr + off[x + 1], v);
+        }
+        for (uint32_t c : same[i])
+          if (img[c] == v) ok = false;
+        if (!ok) continue;
+        mylr[i]++;
+        if (i + 1 == nv) {
+          if (++myfinal > (1u << 16)) {
+            uint64_t t = total.fetch_add(myfinal) + myfinal;
+            myfinal = 0;
+            if (cap && t > cap) stop.store(1);
+          }
+          continue;
+        }
+        if (++mywork > 4096) {
+          uint64_t w = work.fetch_add(mywork) + mywork;
+          mywork = 0;
+          if (work_limit && w > work_limit) stop.store(2);
+        }
+        img[i] = v;
+        rec(i + 1);
+      }
+    };
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t r = 0; r < (int64_t)c0.size(); ++r) {
+      if (stop.load(std::memory_order_relaxed)) continue;
+      img[0] = c0[r];
+      mylr[0]++;
+      rec(1);
+      uint64_t t = total.fetch_add(myfinal) + myfinal;
+      myfinal = 0;
+      if (cap && t > cap) stop.store(1);
+    }
+#pragma omp critical
+    for (uint32_t i = 0; i < nv; ++i) lr[i] += mylr[i];
+  }
+  if (stop.load() == 2) return 2;
+  const uint64_t t = total.load();
+  *saturated = (cap && t > cap) ? 1 : 0;
+  *count = *saturated ? cap : t;
+  if (level_rows) std::copy(lr.begin(), lr.end(), level_rows);
+  return 0;
+}
+
 // Enumerate all embeddings along `order` into out (rows of nv vertex ids,
 // column u = image of query vertex u), sorted lexicographically.  Returns 4 if
 // more than capacity rows exist.
