@@ -1,11 +1,15 @@
 #!/usr/bin/env python
 """Benchmark: embeddings/s and queries/s of GEM subgraph matching on one MI355X.
 
-A "step" is one pass of the matcher over the workload's whole query set
-(filter -> position index -> edge tables -> GEM order -> join, per query) with
-the data graph already resident in HBM.  Counts are exact (capped at 1e9 per
-query on the R-MAT workloads: a query whose count exceeds the cap reports
-saturated and count = cap).
+A "step" is one pass of the batched matcher over the workload's whole query
+set (filter -> position index -> edge tables -> plan -> vertex-at-a-time join
+for every query, many queries per kernel launch), with the data graph already
+resident in HBM.  `value` is that path's queries/s.  The plain path (the
+reference's per-query pipeline with GEM's edge-at-a-time join) is timed over
+one pass in the same run, and the counts of the two are compared query by
+query.  Counts are exact; on the R-MAT workloads each query's count is capped
+at 1e9 (saturated queries report count = cap), and a query that processes more
+than --work-limit rows is reported unfinished.
 
 Prints ONE JSON line on stdout (progress goes to stderr).
 """
@@ -45,6 +49,12 @@ def cpu_model():
     return platform.processor()
 
 
+def kernel_table(stats):
+    return {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
+                "GB/s": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else 0.0}
+            for k, v in stats.items() if v["launches"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,16 +63,18 @@ def main():
     ap.add_argument("--workload", default="rmat20", choices=sorted(DESCR))
     ap.add_argument("--queries", type=int, default=0, help="use only the first N queries")
     ap.add_argument("--cap", type=float, default=1e9)
+    ap.add_argument("--work-limit", type=float, default=2e10,
+                    help="rows a query's join may process before it is reported unfinished")
+    ap.add_argument("--group", type=int, default=0, help="queries resident at once in the batched path (0 = all)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path pass")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-oracle work (0 = skip)")
     ap.add_argument("--cpu-work-limit", type=float, default=2e9,
                     help="intermediate rows after which one oracle query is abandoned")
-    ap.add_argument("--work-limit", type=float, default=2e10,
-                    help="rows a GPU join may process before the query is reported unfinished")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        log("bench.py measures one GPU: every box in this setup has a single MI355X; "
-            "multi-GPU sharding is described in DESIGN.md but not run.")
+        log("bench.py measures one GPU only: the multi-GPU form (root-partitioned join with one allreduce of "
+            "the per-query counts) is not implemented, and this setup has a single MI355X per box.")
         sys.exit(2)
 
     from gem_amd import GemContext
@@ -74,58 +86,59 @@ def main():
     if args.queries:
         qs = qs[:args.queries]
     cap = int(args.cap) if args.workload != "er10k" else 0
+    wl = int(args.work_limit)
     log(f"[bench] {args.workload}: |V|={g.num_vertices} |E|={g.num_edges} queries={len(qs)} "
         f"dense={sum(q.dense for q in qs)} built in {time.perf_counter() - t0:.1f}s")
 
     ctx = GemContext(0)
     ctx.load_data(g)
-    ctx.set_work_limit(int(args.work_limit))
+    ctx.set_work_limit(wl)
 
-    def one_pass(record=None):
-        tot = 0
-        for i, q in enumerate(qs):
-            r = ctx.match_full(q, cap=cap)
-            tot += r["count"]
-            if record is not None:
-                record.append(r)
-            if (i + 1) % 10 == 0:
-                log(f"[bench]   {i + 1}/{len(qs)} queries")
-        return tot
-
+    # ---- batched path (the measured value)
     for w in range(args.warmup):
         t = time.perf_counter()
-        one_pass()
-        log(f"[bench] warmup pass {w} {time.perf_counter() - t:.2f}s")
-
+        ctx.match_batch(qs, cap=cap, work_limit=wl, group=args.group)
+        log(f"[bench] batched warmup pass {w}: {time.perf_counter() - t:.2f}s")
     ctx.set_profiling(True)
-    results = []
-    emb = 0
     t = time.perf_counter()
+    batched = None
+    emb = 0
     for s in range(args.steps):
-        rec = [] if s == 0 else None
-        emb += one_pass(rec)
-        if rec is not None:
-            results = rec
-        log(f"[bench] timed pass {s} done at {time.perf_counter() - t:.2f}s")
+        res = ctx.match_batch(qs, cap=cap, work_limit=wl, group=args.group)
+        emb += sum(r["count"] for r in res)
+        if batched is None:
+            batched = res
+        log(f"[bench] batched timed pass {s} done at {time.perf_counter() - t:.2f}s")
     elapsed = time.perf_counter() - t
     stats = ctx.kernel_stats()
     ctx.set_profiling(False)
-
-    nq = len(qs) * args.steps
-    qps = nq / elapsed
+    qps = len(qs) * args.steps / elapsed
     eps = emb / elapsed
-    sat = sum(1 for r in results if r["saturated"])
-    unfinished = sum(1 for r in results if r["work_limited"])
+    log("[bench] batched kernels:", json.dumps(kernel_table(stats)))
 
     # roofline: the kernel with the most time in the timed region
     dom_name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else 0.0
-    kernels = {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
-                   "GB/s": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else 0.0}
-               for k, v in stats.items() if v["launches"]}
-    log("[bench] kernels:", json.dumps(kernels))
 
-    # CPU baseline: the oracle on a bounded prefix of the same queries
+    # ---- plain path, one pass
+    plain = None
+    if not args.no_plain:
+        t = time.perf_counter()
+        plain_res = []
+        for i, q in enumerate(qs):
+            plain_res.append(ctx.match_full(q, cap=cap))
+            if (i + 1) % 10 == 0:
+                log(f"[bench]   plain {i + 1}/{len(qs)} at {time.perf_counter() - t:.1f}s")
+        pt = time.perf_counter() - t
+        both = [(a, b) for a, b in zip(batched, plain_res) if not a["work_limited"] and not b["work_limited"]]
+        agree = sum(1 for a, b in both if (a["count"], a["saturated"]) == (b["count"], b["saturated"]))
+        plain = {"queries_per_s": round(len(qs) / pt, 4), "embeddings_per_s": round(
+            sum(r["count"] for r in plain_res) / pt, 1), "seconds": round(pt, 3),
+                 "unfinished": sum(r["work_limited"] for r in plain_res),
+                 "counts_equal_to_batched": f"{agree}/{len(both)} queries finished by both"}
+        log(f"[bench] plain: {plain}")
+
+    # ---- CPU baseline: the oracle on a bounded prefix of the same queries
     cpu = None
     if args.cpu_budget > 0:
         import oracle
@@ -140,11 +153,8 @@ def main():
                 abandoned += 1
                 continue
             done += 1
-            gr = results[i]
-            if gr["work_limited"]:
-                continue
-            if (r["count"], r["saturated"]) != (gr["count"], gr["saturated"]) or \
-                    list(r["order"]) != list(gr["order"]):
+            gr = batched[i]
+            if not gr["work_limited"] and (r["count"], r["saturated"]) != (gr["count"], gr["saturated"]):
                 mism += 1
         threads = oracle.max_threads()
         cpu = {"value": round(done / spent, 4) if spent > 0 else 0.0, "unit": "queries/s", "cores": threads,
@@ -168,11 +178,13 @@ def main():
         "dtype": "u32",
         "data": "synthetic (seeded generators)",
         "config": {"workload": args.workload, "description": DESCR[args.workload], "queries": len(qs),
-                   "path": "plain (GEM order, edge-at-a-time join)", "cap": cap},
+                   "path": "batched vertex-at-a-time join, GEM-derived vertex order", "cap": cap,
+                   "work_limit_rows": wl},
         "embeddings_per_s": round(eps, 1),
-        "saturated_queries": sat,
-        "unfinished_queries": unfinished,
-        "work_limit_rows": args.work_limit,
+        "saturated_queries": sum(r["saturated"] for r in batched),
+        "unfinished_queries": sum(r["work_limited"] for r in batched),
+        "plain": plain,
+        "speedup_vs_plain": round(qps / plain["queries_per_s"], 3) if plain else None,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,

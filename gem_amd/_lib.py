@@ -91,7 +91,22 @@ SIGNATURES = [
     ("gem_hip_match_batch", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, C.POINTER(u32p),
                                       C.POINTER(BatchOpts), C.POINTER(JoinResult), u64p]),
     ("gem_hip_vjoin", C.c_int, [C.c_void_p, u32p, C.c_uint64, C.POINTER(JoinResult), u64p]),
+    ("gem_mlp_create", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("gem_mlp_destroy", None, [C.c_void_p]),
+    ("gem_mlp_num_params", C.c_uint64, [C.c_void_p]),
+    ("gem_mlp_get_params", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    ("gem_mlp_set_params", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    ("gem_mlp_predict", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_uint32, C.POINTER(C.c_float)]),
+    ("gem_mlp_loss_grad", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_uint32,
+                                    C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("gem_mlp_train", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_uint32, C.c_uint32,
+                                C.c_float, C.POINTER(C.c_float)]),
+    ("gem_hip_order_candidates", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, u32p, C.POINTER(C.c_int32), u32p]),
+    ("gem_hip_match_batch_learned", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, C.c_void_p,
+                                              C.c_uint32, C.POINTER(BatchOpts), C.POINTER(JoinResult), u64p, u32p]),
 ]
+
+GEM_FEAT_DIM = 272
 
 _lib = None
 

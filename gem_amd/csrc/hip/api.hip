@@ -245,6 +245,23 @@ int gem_hip_vjoin(gem_ctx *c, const uint32_t *vertex_order, uint64_t cap, gem_jo
   });
 }
 
+int gem_hip_order_candidates(gem_ctx *c, uint32_t max_orders, uint64_t seed, uint32_t *orders, int32_t *feats,
+                             uint32_t *n) {
+  return guard(c, [&]() {
+    if (!n) throw Error{GEM_E_INVALID, "null count"};
+    run_order_candidates(c, max_orders, seed, orders, feats, n);
+  });
+}
+
+int gem_hip_match_batch_learned(gem_ctx *c, const gem_query_view *queries, uint32_t nq, gem_mlp *mlp,
+                                uint32_t max_orders, const gem_batch_opts *opts, gem_join_result *results,
+                                uint64_t *level_rows, uint32_t *chosen) {
+  return guard(c, [&]() {
+    if (nq && (!queries || !results)) throw Error{GEM_E_INVALID, "null argument"};
+    run_match_batch_learned(c, queries, nq, mlp, max_orders, opts, results, level_rows, chosen);
+  });
+}
+
 int gem_hip_phase_times(const gem_ctx *c, gem_phase_times *out) {
   if (!c || !out) return GEM_E_INVALID;
   *out = c->times;

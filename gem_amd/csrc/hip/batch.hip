@@ -8,23 +8,45 @@
 
 namespace gemd {
 
+// Vertex order of first appearance along GEM's edge order.
+static std::vector<uint32_t> gem_vertex_order(const QueryDev &Qd) {
+  const gemh::QueryInfo &Q = Qd.Q;
+  std::vector<uint64_t> sz(Q.ne);
+  for (uint32_t e = 0; e < Q.ne; ++e) {
+    const TableDev &a = Qd.tab[2 * e], &b = Qd.tab[2 * e + 1];
+    sz[e] = a.present ? a.size : b.size;
+  }
+  std::vector<uint32_t> order(Q.ne), vorder;
+  int st = gemh::gem_order(Q, sz.data(), order.data());
+  if (st) throw Error{st, "GEM ordering failed"};
+  st = gemh::vertex_order_from_edges(Q, order.data(), &vorder);
+  if (st) throw Error{st, "vertex order from edge order failed"};
+  return vorder;
+}
+
+static gemh::QueryStats query_stats(const gem_ctx *c, const QueryDev &Qd) {
+  const gemh::QueryInfo &Q = Qd.Q;
+  gemh::QueryStats s;
+  s.ncan.assign(Qd.ncan.begin(), Qd.ncan.end());
+  s.tsize.assign(2 * Q.ne, 0);
+  for (uint32_t d = 0; d < 2 * Q.ne; ++d) s.tsize[d] = Qd.tab[d].present ? Qd.tab[d].size : 0;
+  s.label_freq.assign(Q.nv, 0);
+  for (uint32_t u = 0; u < Q.nv; ++u) {
+    auto it = c->g.label_id.find(Q.lab[u]);
+    if (it != c->g.label_id.end()) s.label_freq[u] = c->g.label_freq[it->second];
+  }
+  s.data_v = c->g.n;
+  s.data_e = c->g.slots / 2;
+  return s;
+}
+
 static void plan_query(gem_ctx *c, VQueryState &S, const QueryDev &Qd, const uint32_t *vorder_in) {
   const gemh::QueryInfo &Q = Qd.Q;
   std::vector<uint32_t> vorder;
-  if (vorder_in) {
+  if (vorder_in)
     vorder.assign(vorder_in, vorder_in + Q.nv);
-  } else {
-    std::vector<uint64_t> sz(Q.ne);
-    for (uint32_t e = 0; e < Q.ne; ++e) {
-      const TableDev &a = Qd.tab[2 * e], &b = Qd.tab[2 * e + 1];
-      sz[e] = a.present ? a.size : b.size;
-    }
-    std::vector<uint32_t> order(Q.ne);
-    int st = gemh::gem_order(Q, sz.data(), order.data());
-    if (st) throw Error{st, "GEM ordering failed"};
-    st = gemh::vertex_order_from_edges(Q, order.data(), &vorder);
-    if (st) throw Error{st, "vertex order from edge order failed"};
-  }
+  else
+    vorder = gem_vertex_order(Qd);
   if (gemh::make_vertex_plan(Q, vorder, &S.plan))
     throw Error{GEM_E_INVALID, "invalid vertex order (leaf first, or a vertex without an earlier neighbour)"};
   S.qd = &Qd;
@@ -62,6 +84,86 @@ void run_match_batch(gem_ctx *c, const gem_query_view *qs, uint32_t nq, const ui
       run_tables(c, qd[i]);
       plan_query(c, st[i], qd[i], vorders ? vorders[g0 + i] : nullptr);
       ptrs.push_back(&st[i]);
+    }
+    VBatchRunner runner(c, o.cap, o.work_limit, budget);
+    runner.run(ptrs);
+    for (uint32_t i = 0; i < gn; ++i)
+      fill_result(st[i], o.cap, &results[g0 + i], level_rows ? level_rows + (uint64_t)(g0 + i) * GEM_MAX_QV : nullptr);
+  }
+  GEM_HIP_CHECK(hipEventRecord(c->ev[7], s));
+  GEM_HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0;
+  GEM_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+  c->times.join_ms = ms;
+  prof_flush(c);
+}
+
+void run_order_candidates(gem_ctx *c, uint32_t max_orders, uint64_t seed, uint32_t *orders, int32_t *feats,
+                          uint32_t *n) {
+  const QueryDev &Qd = c->q;
+  if (!Qd.tabled) throw Error{GEM_E_STATE, "edge tables have not been built"};
+  const gemh::QueryStats st = query_stats(c, Qd);
+  std::vector<std::vector<uint32_t>> cand;
+  gemh::candidate_orders(Qd.Q, st, gem_vertex_order(Qd), max_orders, seed, &cand);
+  for (size_t i = 0; i < cand.size(); ++i) {
+    if (orders) std::copy(cand[i].begin(), cand[i].end(), orders + i * Qd.Q.nv);
+    if (feats) gemh::order_features(Qd.Q, st, cand[i], feats + i * gemh::FEAT_DIM);
+  }
+  *n = (uint32_t)cand.size();
+}
+
+void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, gem_mlp *mlp, uint32_t max_orders,
+                             const gem_batch_opts *opts, gem_join_result *results, uint64_t *level_rows,
+                             uint32_t *chosen) {
+  if (!c->g.loaded) throw Error{GEM_E_STATE, "no data graph loaded"};
+  gem_batch_opts o{};
+  if (opts) o = *opts;
+  const uint32_t group = o.group ? o.group : std::max<uint32_t>(nq, 1);
+  max_orders = std::max<uint32_t>(1, max_orders);
+  hipStream_t s = c->stream;
+  GEM_HIP_CHECK(hipEventRecord(c->ev[6], s));
+  for (uint32_t g0 = 0; g0 < nq; g0 += group) {
+    const uint32_t gn = std::min(group, nq - g0);
+    const uint64_t budget = o.level_budget ? o.level_budget : std::max<uint64_t>(64ull << 20, (16ull << 30) / gn);
+    std::vector<QueryDev> qd(gn);
+    std::vector<VQueryState> st(gn);
+    std::vector<VQueryState *> ptrs;
+    std::vector<std::vector<std::vector<uint32_t>>> cand(gn);
+    std::vector<int32_t> feats;
+    std::vector<uint32_t> owner;
+    for (uint32_t i = 0; i < gn; ++i) {
+      run_filter(c, qd[i], &qs[g0 + i], 0);
+      run_tables(c, qd[i]);
+      const gemh::QueryStats qsx = query_stats(c, qd[i]);
+      gemh::candidate_orders(qd[i].Q, qsx, gem_vertex_order(qd[i]), mlp ? max_orders : 1, 0x9e3779b9ull + g0 + i,
+                             &cand[i]);
+      if (mlp) {
+        for (auto &vo : cand[i]) {
+          const size_t off = feats.size();
+          feats.resize(off + gemh::FEAT_DIM);
+          gemh::order_features(qd[i].Q, qsx, vo, feats.data() + off);
+          owner.push_back(i);
+        }
+      }
+    }
+    std::vector<uint32_t> pick(gn, 0);
+    if (mlp && !owner.empty()) {
+      std::vector<float> pred(owner.size());
+      const int rc = gem_mlp_predict(mlp, feats.data(), (uint32_t)owner.size(), pred.data());
+      if (rc) throw Error{rc, "order model: " + c->err};
+      std::vector<float> best(gn, 0.f);
+      size_t k = 0;
+      for (uint32_t i = 0; i < gn; ++i)
+        for (uint32_t j = 0; j < cand[i].size(); ++j, ++k)
+          if (j == 0 || pred[k] < best[i]) {
+            best[i] = pred[k];
+            pick[i] = j;
+          }
+    }
+    for (uint32_t i = 0; i < gn; ++i) {
+      plan_query(c, st[i], qd[i], cand[i][pick[i]].data());
+      ptrs.push_back(&st[i]);
+      if (chosen) chosen[g0 + i] = pick[i];
     }
     VBatchRunner runner(c, o.cap, o.work_limit, budget);
     runner.run(ptrs);

@@ -281,6 +281,11 @@ void run_join(gem_ctx *c, const uint32_t *order, uint64_t cap, JoinOut *out, gem
 void run_match_batch(gem_ctx *c, const gem_query_view *qs, uint32_t nq, const uint32_t *const *vorders,
                      const gem_batch_opts *opts, gem_join_result *results, uint64_t *level_rows);
 void run_vjoin(gem_ctx *c, const uint32_t *vorder, uint64_t cap, gem_join_result *out, uint64_t *level_rows);
+void run_order_candidates(gem_ctx *c, uint32_t max_orders, uint64_t seed, uint32_t *orders, int32_t *feats,
+                          uint32_t *n);
+void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, gem_mlp *mlp, uint32_t max_orders,
+                             const gem_batch_opts *opts, gem_join_result *results, uint64_t *level_rows,
+                             uint32_t *chosen);
 
 // ------------------------------------------------------------------ scans
 // Exclusive scan of n u64 values in place-free form: out[i] = sum(in[0..i)),

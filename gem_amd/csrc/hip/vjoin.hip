@@ -25,6 +25,9 @@
 // is processed in block ranges, the deepest pending range first, which bounds
 // memory; a query stops as soon as its count exceeds the cap.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <memory>
 
 #include "ctx.h"
@@ -346,12 +349,22 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
     S->stack.push_back(VStackItem{1, buf, n0, 1, {}, 0, false});
   }
   DevBuf counters;
+  // GEM_VERBOSE=1: a progress line on stderr every ~2 s of a long batch
+  const char *verbose_env = std::getenv("GEM_VERBOSE");
+  const bool verbose = verbose_env && verbose_env[0] == '1';
+  auto last_print = std::chrono::steady_clock::now();
   for (;;) {
     std::vector<VQueryState *> act;
     for (VQueryState *S : qs)
       if (!S->done && !S->stack.empty()) act.push_back(S);
     if (act.empty()) break;
     ++iterations_;
+    if (verbose && std::chrono::steady_clock::now() - last_print > std::chrono::seconds(2)) {
+      last_print = std::chrono::steady_clock::now();
+      std::fprintf(stderr, "[vjoin] iteration %llu: %zu of %zu queries active\n", (unsigned long long)iterations_,
+                   act.size(), qs.size());
+      std::fflush(stderr);
+    }
     // ---- length pass for unmeasured, non-final top items
     Launch lenL;
     std::vector<VQueryState *> len_owner;

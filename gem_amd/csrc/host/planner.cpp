@@ -23,7 +23,10 @@
 #include "planner.h"
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
+
+#include "host_graph.h"
 
 namespace gemh {
 
@@ -272,6 +275,180 @@ int make_vertex_plan(const QueryInfo &q, const std::vector<uint32_t> &vorder, Ve
       if (q.lab[vorder[c]] == q.lab[u]) L.dup_col[L.ndup++] = c;
   }
   return GEM_OK;
+}
+
+static int32_t q16(double x) { return (int32_t)std::llround(16.0 * std::log2(1.0 + std::max(0.0, x))); }
+
+// fanout of the table w -> u: average row length
+static double fanout(const QueryInfo &q, const QueryStats &s, uint32_t w, uint32_t u) {
+  for (auto [n, e] : q.adj[w])
+    if (n == u) {
+      const uint32_t d = 2 * e + (q.edges[e].first == w ? 0 : 1);
+      uint64_t t = s.tsize[d];
+      if (t == 0) t = s.tsize[d ^ 1];  // leaf-keyed direction: same total
+      return (double)t / (double)std::max<uint64_t>(1, s.ncan[w]);
+    }
+  return 0.0;
+}
+
+void order_features(const QueryInfo &q, const QueryStats &s, const std::vector<uint32_t> &vo, int32_t *feat) {
+  std::fill(feat, feat + FEAT_DIM, 0);
+  std::vector<int> col(q.nv, -1);
+  for (uint32_t i = 0; i < vo.size(); ++i) col[vo[i]] = (int)i;
+  double est = (double)s.ncan[vo[0]], est_sum = 0;
+  for (uint32_t i = 0; i < vo.size() && i < FEAT_LEVELS; ++i) {
+    const uint32_t u = vo[i];
+    int32_t *f = feat + i * FEAT_PER_LEVEL;
+    uint32_t nb = 0, ndup = 0;
+    double fmin = 1e300, fmax = 0, sel = 1.0;
+    std::vector<double> fans;
+    for (auto [w, e] : q.adj[u])
+      if (col[w] >= 0 && col[w] < (int)i) {
+        fans.push_back(fanout(q, s, w, u));
+        ++nb;
+      }
+    for (uint32_t c = 0; c < i; ++c)
+      if (q.lab[vo[c]] == q.lab[u]) ++ndup;
+    if (nb) {
+      std::sort(fans.begin(), fans.end());
+      fmin = fans.front();
+      fmax = fans.back();
+      for (size_t j = 1; j < fans.size(); ++j) sel *= std::min(1.0, fans[j] / std::max<double>(1, s.ncan[u]));
+      est = est * fmin * sel;
+    } else {
+      fmin = 0;
+    }
+    est_sum += est;
+    f[0] = q16((double)s.ncan[u]);
+    f[1] = (int32_t)(16 * nb);
+    f[2] = (int32_t)(16 * ndup);
+    f[3] = q16(fmin);
+    f[4] = q16(fmax);
+    f[5] = q16(est);
+    f[6] = (int32_t)(16 * q.deg[u]);
+    f[7] = q16((double)s.label_freq[u]);
+  }
+  int32_t *g = feat + FEAT_LEVELS * FEAT_PER_LEVEL;
+  uint64_t sum_can = 0, max_can = 0, sum_t = 0;
+  for (uint32_t u = 0; u < q.nv; ++u) {
+    sum_can += s.ncan[u];
+    max_can = std::max(max_can, s.ncan[u]);
+  }
+  for (uint64_t t : s.tsize) sum_t += t;
+  std::vector<uint32_t> labs(q.lab);
+  std::sort(labs.begin(), labs.end());
+  const uint32_t nlab = (uint32_t)(std::unique(labs.begin(), labs.end()) - labs.begin());
+  g[0] = (int32_t)(16 * q.nv);
+  g[1] = (int32_t)(16 * q.ne);
+  g[2] = (int32_t)(16 * nlab);
+  g[3] = q16((double)sum_can);
+  g[4] = q16((double)max_can);
+  g[5] = q16((double)sum_t);
+  g[6] = q16(est);
+  g[7] = q16(est_sum);
+  g[8] = (2 * q.ne >= 3 * q.nv) ? 16 : 0;
+  g[9] = q16((double)s.data_v);
+  g[10] = q16((double)s.data_e);
+}
+
+static bool valid_start(const QueryInfo &q, uint32_t u) { return q.deg[u] != 1; }
+
+void candidate_orders(const QueryInfo &q, const QueryStats &s, const std::vector<uint32_t> &gem_vo,
+                      uint32_t max_orders, uint64_t seed, std::vector<std::vector<uint32_t>> *out) {
+  out->clear();
+  auto add = [&](const std::vector<uint32_t> &vo) {
+    if (out->size() >= max_orders) return;
+    for (auto &o : *out)
+      if (o == vo) return;
+    out->push_back(vo);
+  };
+  add(gem_vo);
+  // greedy by the running estimate: prefer vertices closing more edges, then
+  // the smallest estimated growth
+  std::vector<uint32_t> starts;
+  for (uint32_t u = 0; u < q.nv; ++u)
+    if (valid_start(q, u)) starts.push_back(u);
+  std::sort(starts.begin(), starts.end(), [&](uint32_t a, uint32_t b) {
+    return s.ncan[a] != s.ncan[b] ? s.ncan[a] < s.ncan[b] : a < b;
+  });
+  for (uint32_t si = 0; si < starts.size() && si < 4; ++si) {
+    std::vector<uint32_t> vo{starts[si]};
+    std::vector<char> in(q.nv, 0);
+    in[starts[si]] = 1;
+    while (vo.size() < q.nv) {
+      double best = 1e300;
+      uint32_t pick = UINT32_MAX, pick_nb = 0;
+      for (uint32_t u = 0; u < q.nv; ++u) {
+        if (in[u]) continue;
+        uint32_t nb = 0;
+        double fmin = 1e300, sel = 1.0;
+        std::vector<double> fans;
+        for (auto [w, e] : q.adj[u])
+          if (in[w]) fans.push_back(fanout(q, s, w, u));
+        nb = (uint32_t)fans.size();
+        if (!nb) continue;
+        std::sort(fans.begin(), fans.end());
+        fmin = fans.front();
+        for (size_t j = 1; j < fans.size(); ++j) sel *= std::min(1.0, fans[j] / std::max<double>(1, s.ncan[u]));
+        const double growth = fmin * sel;
+        if (nb > pick_nb || (nb == pick_nb && growth < best)) {
+          best = growth;
+          pick = u;
+          pick_nb = nb;
+        }
+      }
+      vo.push_back(pick);
+      in[pick] = 1;
+    }
+    add(vo);
+    // a second variant: pure smallest-growth (ignoring closing-edge count)
+    std::vector<uint32_t> vo2{starts[si]};
+    std::fill(in.begin(), in.end(), 0);
+    in[starts[si]] = 1;
+    while (vo2.size() < q.nv) {
+      double best = 1e300;
+      uint32_t pick = UINT32_MAX;
+      for (uint32_t u = 0; u < q.nv; ++u) {
+        if (in[u]) continue;
+        std::vector<double> fans;
+        for (auto [w, e] : q.adj[u])
+          if (in[w]) fans.push_back(fanout(q, s, w, u));
+        if (fans.empty()) continue;
+        std::sort(fans.begin(), fans.end());
+        double g = fans.front();
+        for (size_t j = 1; j < fans.size(); ++j) g *= std::min(1.0, fans[j] / std::max<double>(1, s.ncan[u]));
+        if (g < best) {
+          best = g;
+          pick = u;
+        }
+      }
+      vo2.push_back(pick);
+      in[pick] = 1;
+    }
+    add(vo2);
+  }
+  // seeded random connected orders
+  Rng rng(seed);
+  for (uint32_t tries = 0; out->size() < max_orders && tries < 8 * max_orders; ++tries) {
+    std::vector<uint32_t> vo{starts[rng.below(starts.size())]};
+    std::vector<char> in(q.nv, 0);
+    in[vo[0]] = 1;
+    while (vo.size() < q.nv) {
+      std::vector<uint32_t> fr;
+      for (uint32_t u = 0; u < q.nv; ++u) {
+        if (in[u]) continue;
+        for (auto [w, e] : q.adj[u])
+          if (in[w]) {
+            fr.push_back(u);
+            break;
+          }
+      }
+      const uint32_t u = fr[rng.below(fr.size())];
+      vo.push_back(u);
+      in[u] = 1;
+    }
+    add(vo);
+  }
 }
 
 }  // namespace gemh

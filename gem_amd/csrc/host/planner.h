@@ -86,4 +86,24 @@ int vertex_order_from_edges(const QueryInfo &q, const uint32_t *edge_order, std:
 // earlier neighbour.
 int make_vertex_plan(const QueryInfo &q, const std::vector<uint32_t> &vorder, VertexPlan *plan);
 
+// ------------------------------------------------------- learned ordering
+// Statistics of a filtered query that the order features are built from.
+struct QueryStats {
+  std::vector<uint64_t> ncan;        // |Can(u)|
+  std::vector<uint64_t> tsize;       // |T_d| per directed edge (0 if absent)
+  std::vector<uint64_t> label_freq;  // data frequency of u's label
+  uint64_t data_v = 0, data_e = 0;
+};
+constexpr uint32_t FEAT_PER_LEVEL = 8;
+constexpr uint32_t FEAT_LEVELS = GEM_MAX_QV;
+constexpr uint32_t FEAT_GLOBAL = 16;
+constexpr uint32_t FEAT_DIM = FEAT_LEVELS * FEAT_PER_LEVEL + FEAT_GLOBAL;  // 272
+// int32 features of one vertex order (log2-quantised counts, x16).
+void order_features(const QueryInfo &q, const QueryStats &s, const std::vector<uint32_t> &vorder, int32_t *feat);
+// Up to `max_orders` distinct valid vertex orders: the GEM-derived order
+// first, then greedy orders by the independence estimate from several
+// starts, then seeded random connected orders.
+void candidate_orders(const QueryInfo &q, const QueryStats &s, const std::vector<uint32_t> &gem_vorder,
+                      uint32_t max_orders, uint64_t seed, std::vector<std::vector<uint32_t>> *out);
+
 }  // namespace gemh

@@ -262,6 +262,45 @@ int gem_hip_match_batch(gem_ctx *ctx, const gem_query_view *queries, uint32_t nq
 int gem_hip_vjoin(gem_ctx *ctx, const uint32_t *vertex_order, uint64_t cap, gem_join_result *out,
                   uint64_t *level_rows);
 
+/* --------------------------------------------------- learned join order */
+/* No reference counterpart: GEM orders by a fixed score
+ * (in-core/lib/structure/src/unifiedTrie.cu:939-1018).  Here a small MLP
+ * (in -> 128 -> 128 -> 1, ReLU, bf16 weights, fp32 accumulation), trained on
+ * the GPU by the library's own kernels on join costs the matcher logs,
+ * predicts the cost of candidate vertex orders; the batched path runs the
+ * cheapest predicted one.  GEM's order is always a candidate, and the order
+ * used when no model is given. */
+#define GEM_FEAT_DIM 272u
+typedef struct gem_mlp gem_mlp;
+int gem_mlp_create(gem_ctx *ctx, uint32_t in_dim, uint32_t hidden, uint64_t seed, gem_mlp **out);
+void gem_mlp_destroy(gem_mlp *m);
+uint64_t gem_mlp_num_params(const gem_mlp *m);
+/* flat fp32 parameters: W1[in*h], b1[h], W2[h*h], b2[h], W3[h], b3[1] */
+int gem_mlp_get_params(gem_mlp *m, float *out);
+int gem_mlp_set_params(gem_mlp *m, const float *in);
+int gem_mlp_predict(gem_mlp *m, const int32_t *feats, uint32_t n, float *out);
+/* loss = mean((y - t)^2) / 2 over the batch; grad[num_params] (nullable) */
+int gem_mlp_loss_grad(gem_mlp *m, const int32_t *feats, const float *targets, uint32_t n, float *loss,
+                      float *grad);
+/* full-batch Adam for `iters` steps; losses[iters] (nullable) */
+int gem_mlp_train(gem_mlp *m, const int32_t *feats, const float *targets, uint32_t n, uint32_t iters, float lr,
+                  float *losses);
+
+/* For the current query (after gem_hip_filter and gem_hip_build_edge_tables):
+ * up to max_orders candidate vertex orders (orders[max_orders * num_v], the
+ * GEM-derived order first) and their int32 features
+ * (feats[max_orders * GEM_FEAT_DIM]); *n receives the number produced. */
+int gem_hip_order_candidates(gem_ctx *ctx, uint32_t max_orders, uint64_t seed, uint32_t *orders, int32_t *feats,
+                             uint32_t *n);
+
+/* Batched path with model-chosen orders: for each query the candidates are
+ * scored by `mlp` and the lowest predicted cost is run (mlp == NULL: GEM
+ * order).  chosen (nullable) receives, per query, the index of the chosen
+ * candidate (0 = GEM's order). */
+int gem_hip_match_batch_learned(gem_ctx *ctx, const gem_query_view *queries, uint32_t nq, gem_mlp *mlp,
+                                uint32_t max_orders, const gem_batch_opts *opts, gem_join_result *results,
+                                uint64_t *level_rows, uint32_t *chosen);
+
 /* Whole plain pipeline for one query: filter -> index -> tables -> GEM order
  * -> join.  Equivalent to the reference's main() for one query at -t 0. */
 int gem_hip_match(gem_ctx *ctx, const gem_query_view *q, uint64_t cap,

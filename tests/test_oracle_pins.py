@@ -1,5 +1,5 @@
 """Pins for the CPU oracle: closed-form counts, brute force, and the filter /
-table definitions.  The oracle is the judge of GPU parity, so each of its
+table definitions.  The oracle decides GPU parity, so each of its
 functions is checked against something it does not compute itself."""
 import itertools
 import math
