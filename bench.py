@@ -67,6 +67,9 @@ def main():
                     help="rows a query's join may process before it is reported unfinished")
     ap.add_argument("--group", type=int, default=0, help="queries resident at once in the batched path (0 = all)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path pass")
+    ap.add_argument("--model", default=os.path.join(ROOT, "gem_amd", "models", "order_mlp_rmat20.npz"),
+                    help="trained order model (.npz); absent = GEM-derived orders")
+    ap.add_argument("--orders", type=int, default=16, help="candidate orders scored per query")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-oracle work (0 = skip)")
     ap.add_argument("--cpu-work-limit", type=float, default=2e9,
                     help="intermediate rows after which one oracle query is abandoned")
@@ -94,17 +97,30 @@ def main():
     ctx.load_data(g)
     ctx.set_work_limit(wl)
 
-    # ---- batched path (the measured value)
+    # ---- batched path with the learned order (the measured value)
+    from gem_amd.learned import OrderModel, match_batch_learned
+    model = None
+    if args.model and os.path.exists(args.model):
+        model = OrderModel.load(ctx, args.model)
+        log(f"[bench] order model {args.model}: trained on {model.meta.get('workload')} "
+            f"({model.meta.get('tuples')} tuples)")
+    else:
+        log("[bench] no order model: batched path runs GEM-derived orders")
+
+    def batched_pass():
+        return match_batch_learned(ctx, qs, model, max_orders=args.orders, cap=cap, work_limit=wl,
+                                   group=args.group)
+
     for w in range(args.warmup):
         t = time.perf_counter()
-        ctx.match_batch(qs, cap=cap, work_limit=wl, group=args.group)
+        batched_pass()
         log(f"[bench] batched warmup pass {w}: {time.perf_counter() - t:.2f}s")
     ctx.set_profiling(True)
     t = time.perf_counter()
     batched = None
     emb = 0
     for s in range(args.steps):
-        res = ctx.match_batch(qs, cap=cap, work_limit=wl, group=args.group)
+        res = batched_pass()
         emb += sum(r["count"] for r in res)
         if batched is None:
             batched = res
@@ -115,6 +131,21 @@ def main():
     qps = len(qs) * args.steps / elapsed
     eps = emb / elapsed
     log("[bench] batched kernels:", json.dumps(kernel_table(stats)))
+
+    # ablation: the same batched path with GEM-derived orders, one pass
+    gem_batched = None
+    if model is not None:
+        t = time.perf_counter()
+        res_g = match_batch_learned(ctx, qs, None, cap=cap, work_limit=wl, group=args.group)
+        tg = time.perf_counter() - t
+        gem_batched = {"queries_per_s": round(len(qs) / tg, 4), "seconds": round(tg, 3),
+                       "rows_processed": int(sum(r["rows_processed"] for r in res_g)),
+                       "rows_processed_learned": int(sum(r["rows_processed"] for r in batched)),
+                       "unfinished": sum(r["work_limited"] for r in res_g),
+                       "counts_equal": sum(1 for a, b in zip(res_g, batched)
+                                           if not a["work_limited"] and not b["work_limited"]
+                                           and (a["count"], a["saturated"]) == (b["count"], b["saturated"]))}
+        log(f"[bench] batched with GEM order: {gem_batched}")
 
     # roofline: the kernel with the most time in the timed region
     dom_name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
@@ -178,12 +209,15 @@ def main():
         "dtype": "u32",
         "data": "synthetic (seeded generators)",
         "config": {"workload": args.workload, "description": DESCR[args.workload], "queries": len(qs),
-                   "path": "batched vertex-at-a-time join, GEM-derived vertex order", "cap": cap,
+                   "path": "batched vertex-at-a-time join, " + (
+                       "learned order (MLP over candidate orders)" if model is not None else "GEM-derived order"),
+                   "cap": cap,
                    "work_limit_rows": wl},
         "embeddings_per_s": round(eps, 1),
         "saturated_queries": sum(r["saturated"] for r in batched),
         "unfinished_queries": sum(r["work_limited"] for r in batched),
         "plain": plain,
+        "batched_gem_order": gem_batched,
         "speedup_vs_plain": round(qps / plain["queries_per_s"], 3) if plain else None,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},

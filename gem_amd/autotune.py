@@ -1,0 +1,96 @@
+"""Autotuner for the batched path.
+
+    python -m gem_amd.autotune --workload rmat20 --out gem_amd/tuning/rmat20.json
+
+Times the batched matcher on a sample of the workload's queries for every
+combination of the tunables and keeps the fastest:
+  * persistent grid size of the work-queue kernels (occupancy),
+  * level budget (bytes of pending level chunks shared by running queries;
+    sets the chunk size of every level),
+  * group size (queries resident on the GPU at once),
+  * candidate orders scored per query by the order model.
+Counts do not depend on any of them (checked on every trial).  The chosen
+configuration is stored as JSON and read by bench.py.
+"""
+import argparse
+import itertools
+import json
+import os
+import sys
+import time
+
+from . import GemContext
+from .learned import OrderModel, match_batch_learned
+from .workloads import CAP, make_graph, make_queries
+
+SPACE = {
+    "persistent_blocks": [1024, 2048, 4096],
+    "level_budget": [4 << 30, 16 << 30, 48 << 30],
+    "group": [0, 32],
+    "max_orders": [8, 16],
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_tuning(path):
+    if path and os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("best", {})
+    return {}
+
+
+def apply(ctx, cfg):
+    if "persistent_blocks" in cfg:
+        ctx.set_persistent_blocks(cfg["persistent_blocks"])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="rmat20")
+    ap.add_argument("--sample", type=int, default=40, help="queries timed per trial")
+    ap.add_argument("--cap", type=float, default=CAP)
+    ap.add_argument("--work-limit", type=float, default=2e10)
+    ap.add_argument("--model", default=os.path.join(os.path.dirname(__file__), "models", "order_mlp_rmat20.npz"))
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    out = args.out or os.path.join(os.path.dirname(__file__), "tuning", f"{args.workload}.json")
+
+    g = make_graph(args.workload)
+    qs = make_queries(args.workload, g)[:args.sample]
+    ctx = GemContext(0)
+    ctx.load_data(g)
+    model = OrderModel.load(ctx, args.model) if os.path.exists(args.model) else None
+    cap, wl = int(args.cap), int(args.work_limit)
+    match_batch_learned(ctx, qs, model, cap=cap, work_limit=wl)  # warm up
+    trials, ref = [], None
+    keys = list(SPACE)
+    for vals in itertools.product(*(SPACE[k] for k in keys)):
+        cfg = dict(zip(keys, vals))
+        if model is None and cfg["max_orders"] != SPACE["max_orders"][0]:
+            continue
+        apply(ctx, cfg)
+        t = time.perf_counter()
+        res = match_batch_learned(ctx, qs, model, max_orders=cfg["max_orders"], cap=cap, work_limit=wl,
+                                  level_budget=cfg["level_budget"], group=cfg["group"])
+        dt = time.perf_counter() - t
+        sig = [(r["count"], r["saturated"], r["work_limited"]) for r in res]
+        if ref is None:
+            ref = sig
+        same = all(a == b for a, b in zip(sig, ref) if not a[2] and not b[2])
+        trials.append({**cfg, "seconds": dt, "counts_equal": same})
+        log(f"[tune] {cfg} -> {dt:.3f}s counts_equal={same}")
+    ok = [t for t in trials if t["counts_equal"]]
+    best = min(ok, key=lambda t: t["seconds"])
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+    with open(out, "w") as f:
+        json.dump({"workload": args.workload, "sample": len(qs), "best": {k: best[k] for k in keys},
+                   "best_seconds": best["seconds"], "trials": trials}, f, indent=1)
+    print(json.dumps({"best": {k: best[k] for k in keys}, "seconds": best["seconds"], "out": out}))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -79,6 +79,12 @@ int gem_hip_set_level_budget(gem_ctx *c, uint64_t bytes) {
   return GEM_OK;
 }
 
+int gem_hip_set_persistent_blocks(gem_ctx *c, uint32_t blocks) {
+  if (!c || blocks == 0 || blocks > (1u << 20)) return GEM_E_INVALID;
+  c->persistent_blocks = blocks;
+  return GEM_OK;
+}
+
 int gem_hip_set_work_limit(gem_ctx *c, uint64_t rows) {
   if (!c) return GEM_E_INVALID;
   c->work_limit_rows = rows;

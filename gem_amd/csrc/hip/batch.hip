@@ -75,7 +75,7 @@ void run_match_batch(gem_ctx *c, const gem_query_view *qs, uint32_t nq, const ui
   GEM_HIP_CHECK(hipEventRecord(c->ev[6], s));
   for (uint32_t g0 = 0; g0 < nq; g0 += group) {
     const uint32_t gn = std::min(group, nq - g0);
-    const uint64_t budget = o.level_budget ? o.level_budget : std::max<uint64_t>(64ull << 20, (16ull << 30) / gn);
+    const uint64_t budget = o.level_budget ? o.level_budget : (16ull << 30);  // shared by the running queries
     std::vector<QueryDev> qd(gn);
     std::vector<VQueryState> st(gn);
     std::vector<VQueryState *> ptrs;
@@ -124,7 +124,7 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
   GEM_HIP_CHECK(hipEventRecord(c->ev[6], s));
   for (uint32_t g0 = 0; g0 < nq; g0 += group) {
     const uint32_t gn = std::min(group, nq - g0);
-    const uint64_t budget = o.level_budget ? o.level_budget : std::max<uint64_t>(64ull << 20, (16ull << 30) / gn);
+    const uint64_t budget = o.level_budget ? o.level_budget : (16ull << 30);  // shared by the running queries
     std::vector<QueryDev> qd(gn);
     std::vector<VQueryState> st(gn);
     std::vector<VQueryState *> ptrs;

@@ -187,6 +187,7 @@ struct gem_ctx {
   gem_phase_times times{};
   uint64_t level_budget_bytes = 1ull << 31;  // per join level buffer
   uint64_t work_limit_rows = 0;             // 0 = none
+  uint32_t persistent_blocks = 2048;        // grid of the batched path's persistent kernels
   void *pinned = nullptr;                   // small pinned staging area
   size_t pinned_bytes = 0;
   gemd::Prof prof;

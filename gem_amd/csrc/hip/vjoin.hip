@@ -322,16 +322,22 @@ void upload(gem_ctx *c, const Launch &L, DevLaunch &D) {
   GEM_HIP_CHECK(hipMemsetAsync(D.queue.p, 0, 4, s));
 }
 
+// persistent grid: enough blocks to fill the chip (tunable cap), never more
+// than the tasks
+uint32_t g_persistent_blocks = 2048;
 uint32_t grid_for(uint64_t blocks) {
-  // persistent: enough blocks to fill the chip, never more than the tasks
-  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 2048));
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks, g_persistent_blocks));
 }
 
 }  // namespace
 
 void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   hipStream_t s = c_->stream;
-  const uint64_t budget = std::max<uint64_t>(budget_bytes_, 1ull << 20);
+  g_persistent_blocks = c_->persistent_blocks;
+  // budget_bytes_ is shared by the queries still running: a query's chunk may
+  // use total / active (at most 4 GiB, at least 1 MiB), so a long-running last
+  // query is not held to a small slice.
+  const uint64_t total_budget = std::max<uint64_t>(budget_bytes_, 1ull << 20);
   // seed each query with the identity level of its first vertex
   for (VQueryState *S : qs) {
     const uint32_t a = S->plan.vorder[0];
@@ -359,6 +365,14 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
       if (!S->done && !S->stack.empty()) act.push_back(S);
     if (act.empty()) break;
     ++iterations_;
+    // live pending rows of every query count against the budget; what is
+    // left is shared by the running queries (4 GiB per query at most, 1 MiB
+    // at least so that depth-first progress, which frees memory, continues)
+    uint64_t live = 0;
+    for (VQueryState *S : qs)
+      for (const VStackItem &it : S->stack) live += it.rows ? it.rows->bytes : 0;
+    const uint64_t left = total_budget > live ? total_budget - live : 0;
+    const uint64_t budget = std::max<uint64_t>(std::min<uint64_t>(left / act.size(), 4ull << 30), 1ull << 20);
     if (verbose && std::chrono::steady_clock::now() - last_print > std::chrono::seconds(2)) {
       last_print = std::chrono::steady_clock::now();
       std::fprintf(stderr, "[vjoin] iteration %llu: %zu of %zu queries active\n", (unsigned long long)iterations_,
@@ -371,7 +385,9 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
     for (VQueryState *S : act) {
       VStackItem &top = S->stack.back();
       const bool final_level = top.level + 1 == S->qd->Q.nv;
-      if (final_level || top.measured) continue;
+      // final levels are measured only when they walk lists (nb > 1) under a
+      // cap: their blocks are then counted a cap-sized piece at a time
+      if (top.measured || (final_level && (S->max_nb[top.level] == 1 || !cap_))) continue;
       VItem it{};
       it.lv = S->levels.as<VLevel>() + top.level;
       it.rows = top.rows->as<uint32_t>();
@@ -437,8 +453,18 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
       it.k = top.k;
       keep_alive.push_back(top.rows);
       if (final_level) {
+        uint64_t b1 = nblocks;
+        if (top.measured) {
+          // count a piece whose survivor bound is about twice what is still
+          // missing to exceed the cap (at least 2^26 list entries of work)
+          const uint64_t need = cap_ + 1 > S->count ? cap_ + 1 - S->count : 1;
+          const uint64_t target = std::max<uint64_t>(2 * need, 1ull << 26);
+          b1 = top.cursor + 1;
+          while (b1 < nblocks && top.block_u[b1] - top.block_u[top.cursor] < target) ++b1;
+        }
         it.row_begin = (uint64_t)top.cursor * VB;
-        it.row_end = top.nrows;
+        it.row_end = std::min<uint64_t>(b1 * VB, top.nrows);
+        top.cursor = b1;
         it.counter = reinterpret_cast<unsigned long long *>((uintptr_t)nctr);  // patched below
         ft.push_back(FTask{S, nctr++});
         S->processed += it.row_end - it.row_begin;
@@ -446,7 +472,7 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
           fin1.add(it, 1, S->max_ndup[top.level]);
         else
           fin.add(it, S->max_nb[top.level], S->max_ndup[top.level]);
-        S->stack.pop_back();
+        if (top.cursor >= nblocks) S->stack.pop_back();
         continue;
       }
       // one chunk of blocks whose output bound fits the per-query budget

@@ -46,6 +46,15 @@ class OrderModel:
     def __del__(self):
         self.close()
 
+    @classmethod
+    def load(cls, ctx, path):
+        """A model saved by gem_amd.train_order_model."""
+        z = np.load(path, allow_pickle=False)
+        m = cls(ctx, seed=0, in_dim=int(z["in_dim"]), hidden=int(z["hidden"]))
+        m.set_params(z["params"])
+        m.meta = {k: z[k].item() if z[k].shape == () else z[k] for k in z.files if k != "params"}
+        return m
+
     def get_params(self):
         out = np.zeros(self.num_params, np.float32)
         self.ctx._check(self._lib.gem_mlp_get_params(self._h, _pf32(out)), "gem_mlp_get_params")

@@ -139,6 +139,9 @@ class GemContext:
         return {"filter_ms": pt.filter_ms, "index_ms": pt.index_ms, "tables_ms": pt.tables_ms,
                 "join_ms": pt.join_ms}
 
+    def set_persistent_blocks(self, blocks):
+        self._check(self._lib.gem_hip_set_persistent_blocks(self._h, int(blocks)), "gem_hip_set_persistent_blocks")
+
     def set_work_limit(self, rows):
         """Stop a join after `rows` processed rows (0 = no limit)."""
         self._check(self._lib.gem_hip_set_work_limit(self._h, int(rows)), "gem_hip_set_work_limit")

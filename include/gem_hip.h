@@ -243,7 +243,8 @@ int gem_hip_kernel_stats(gem_ctx *ctx, gem_kernel_stat *out, uint32_t max, uint3
 typedef struct gem_batch_opts {
   uint64_t cap;          /* per-query count cap (0 = none)                  */
   uint64_t work_limit;   /* rows a query may process (0 = none)             */
-  uint64_t level_budget; /* bytes of one pending level chunk per query (0 = auto) */
+  uint64_t level_budget; /* bytes for pending level chunks, shared by the
+                            group's running queries (0 = 16 GiB)           */
   uint32_t group;        /* queries resident on the GPU at once (0 = all)   */
   uint32_t reserved;
 } gem_batch_opts;
@@ -256,6 +257,11 @@ typedef struct gem_batch_opts {
 int gem_hip_match_batch(gem_ctx *ctx, const gem_query_view *queries, uint32_t nq,
                         const uint32_t *const *vertex_orders, const gem_batch_opts *opts,
                         gem_join_result *results, uint64_t *level_rows);
+
+/* Grid size (workgroups of 256 threads) of the batched path's persistent
+ * kernels; they pull tasks from a device counter, so this sets occupancy,
+ * not the amount of work (default 2048).  Set by the autotuner. */
+int gem_hip_set_persistent_blocks(gem_ctx *ctx, uint32_t blocks);
 
 /* Vertex-at-a-time join of the current query (after gem_hip_filter and
  * gem_hip_build_edge_tables) along vertex_order[num_v]. */
