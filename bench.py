@@ -67,6 +67,9 @@ def main():
                     help="rows a query's join may process before it is reported unfinished")
     ap.add_argument("--group", type=int, default=0, help="queries resident at once in the batched path (0 = all)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path pass")
+    ap.add_argument("--plain-sample", type=int, default=64,
+                    help="run the plain path on a systematic sample of this many queries (every k-th; "
+                         "0 = all queries)")
     ap.add_argument("--model", default=os.path.join(ROOT, "gem_amd", "models", "order_mlp_rmat20.npz"),
                     help="trained order model (.npz); absent = GEM-derived orders")
     ap.add_argument("--orders", type=int, default=16, help="candidate orders scored per query")
@@ -151,21 +154,33 @@ def main():
     dom_name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else 0.0
 
-    # ---- plain path, one pass
+    # ---- plain path, one pass over a systematic sample (every k-th query)
     plain = None
     if not args.no_plain:
+        n = len(qs)
+        k = max(1, -(-n // args.plain_sample)) if args.plain_sample else 1
+        idx = list(range(0, n, k))
+        sample = [qs[i] for i in idx]
         t = time.perf_counter()
         plain_res = []
-        for i, q in enumerate(qs):
+        for j, q in enumerate(sample):
             plain_res.append(ctx.match_full(q, cap=cap))
-            if (i + 1) % 10 == 0:
-                log(f"[bench]   plain {i + 1}/{len(qs)} at {time.perf_counter() - t:.1f}s")
+            if (j + 1) % 10 == 0:
+                log(f"[bench]   plain {j + 1}/{len(sample)} at {time.perf_counter() - t:.1f}s")
         pt = time.perf_counter() - t
-        both = [(a, b) for a, b in zip(batched, plain_res) if not a["work_limited"] and not b["work_limited"]]
+        # the batched learned path over exactly the same queries, for a matched ratio
+        t = time.perf_counter()
+        matched = match_batch_learned(ctx, sample, model, max_orders=args.orders, cap=cap, work_limit=wl,
+                                      group=args.group)
+        mt = time.perf_counter() - t
+        both = [(batched[i], b) for i, b in zip(idx, plain_res)
+                if not batched[i]["work_limited"] and not b["work_limited"]]
         agree = sum(1 for a, b in both if (a["count"], a["saturated"]) == (b["count"], b["saturated"]))
-        plain = {"queries_per_s": round(len(qs) / pt, 4), "embeddings_per_s": round(
+        plain = {"queries_per_s": round(len(sample) / pt, 4), "embeddings_per_s": round(
             sum(r["count"] for r in plain_res) / pt, 1), "seconds": round(pt, 3),
+                 "sample": f"every {k}-th query: {len(sample)} of {n}",
                  "unfinished": sum(r["work_limited"] for r in plain_res),
+                 "batched_on_sample_queries_per_s": round(len(sample) / mt, 4),
                  "counts_equal_to_batched": f"{agree}/{len(both)} queries finished by both"}
         log(f"[bench] plain: {plain}")
 

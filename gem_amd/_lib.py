@@ -78,6 +78,7 @@ SIGNATURES = [
     ("gem_hip_set_level_budget", C.c_int, [C.c_void_p, C.c_uint64]),
     ("gem_hip_set_work_limit", C.c_int, [C.c_void_p, C.c_uint64]),
     ("gem_hip_set_persistent_blocks", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("gem_hip_set_graph", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_load_data_csr", C.c_int, [C.c_void_p, C.POINTER(CsrView)]),
     ("gem_hip_filter", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, u32p, u32p, C.c_uint64]),
     ("gem_hip_position_lookup", C.c_int, [C.c_void_p, C.c_uint32, u32p, C.c_uint64, u32p]),

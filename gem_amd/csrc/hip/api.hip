@@ -61,6 +61,8 @@ void gem_hip_destroy(gem_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (gem_mlp *m : c->mlps) gemd::mlp_detach(m);
+  c->mlps.clear();
   c->q = QueryDev();
   c->g = GraphDev();
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -82,6 +84,12 @@ int gem_hip_set_level_budget(gem_ctx *c, uint64_t bytes) {
 int gem_hip_set_persistent_blocks(gem_ctx *c, uint32_t blocks) {
   if (!c || blocks == 0 || blocks > (1u << 20)) return GEM_E_INVALID;
   c->persistent_blocks = blocks;
+  return GEM_OK;
+}
+
+int gem_hip_set_graph(gem_ctx *c, int on) {
+  if (!c) return GEM_E_INVALID;
+  c->use_graph = on != 0;
   return GEM_OK;
 }
 

@@ -188,10 +188,19 @@ struct gem_ctx {
   uint64_t level_budget_bytes = 1ull << 31;  // per join level buffer
   uint64_t work_limit_rows = 0;             // 0 = none
   uint32_t persistent_blocks = 2048;        // grid of the batched path's persistent kernels
+  bool use_graph = false;                   // replay the batched extend step as a hipGraph
   void *pinned = nullptr;                   // small pinned staging area
   size_t pinned_bytes = 0;
   gemd::Prof prof;
+  // order models created on this context: destroying the context releases
+  // their device memory and detaches them (a later gem_mlp_destroy only frees
+  // the host object, every other gem_mlp_* call returns GEM_E_STATE)
+  std::vector<gem_mlp *> mlps;
 };
+
+namespace gemd {
+void mlp_detach(gem_mlp *m);  // frees m's device buffers, sets m->ctx = nullptr
+}
 
 namespace gemd {
 
@@ -212,7 +221,7 @@ struct ProfLaunch {
   int id;
   hipEvent_t b = nullptr;
   ProfLaunch(gem_ctx *ctx, int kid) : c(ctx), id(kid) {
-    if (c->prof.on) {
+    if (c->prof.on && kid >= 0) {
       b = prof_event(c);
       GEM_HIP_CHECK(hipEventRecord(b, c->stream));
     }

@@ -8,10 +8,11 @@
 // Parameters (fp32 master copy, flat): W1[in x h], b1[h], W2[h x h], b2[h],
 // W3[h], b3[1]; forward and backward use bf16 copies of W1..W3 (rounded to
 // nearest even).  Loss: mean squared error / 2.  Optimiser: Adam.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <random>
 
+#include "../host/host_graph.h"
 #include "ctx.h"
 
 namespace gemd {
@@ -252,6 +253,7 @@ float backward(gem_mlp *m, const float *h_target, uint32_t n, Acts &a) {
 template <class F>
 int mguard(gem_mlp *m, F &&f) {
   if (!m) return GEM_E_INVALID;
+  if (!m->ctx) return GEM_E_STATE;  // its context was destroyed first
   try {
     GEM_HIP_CHECK(hipSetDevice(m->ctx->device));
     f();
@@ -266,6 +268,17 @@ int mguard(gem_mlp *m, F &&f) {
 }
 
 }  // namespace
+
+namespace gemd {
+void mlp_detach(gem_mlp *m) {
+  m->params.reset();
+  m->grads.reset();
+  m->adam_m.reset();
+  m->adam_v.reset();
+  m->wbf.reset();
+  m->ctx = nullptr;
+}
+}  // namespace gemd
 
 extern "C" {
 
@@ -291,11 +304,10 @@ int gem_mlp_create(gem_ctx *ctx, uint32_t in_dim, uint32_t hidden, uint64_t seed
     m->wbf.alloc(((size_t)in_dim * hidden + (size_t)hidden * hidden + hidden) * 2, s);
     // He-uniform weights, zero biases (seeded, host side)
     std::vector<float> p(m->P, 0.f);
-    std::mt19937_64 rng(seed);
+    gemh::Rng rng(seed);
     auto fill = [&](size_t o, size_t cnt, double fan_in) {
       const double lim = std::sqrt(6.0 / fan_in);
-      std::uniform_real_distribution<double> U(-lim, lim);
-      for (size_t i = 0; i < cnt; ++i) p[o + i] = (float)U(rng);
+      for (size_t i = 0; i < cnt; ++i) p[o + i] = (float)((2.0 * rng.uniform() - 1.0) * lim);
     };
     fill(m->oW1, (size_t)in_dim * hidden, in_dim);
     fill(m->oW2, (size_t)hidden * hidden, hidden);
@@ -310,13 +322,18 @@ int gem_mlp_create(gem_ctx *ctx, uint32_t in_dim, uint32_t hidden, uint64_t seed
     delete m;
     return st;
   }
+  ctx->mlps.push_back(m);
   *out = m;
   return GEM_OK;
 }
 
 void gem_mlp_destroy(gem_mlp *m) {
   if (!m) return;
-  (void)hipStreamSynchronize(m->ctx->stream);
+  if (gem_ctx *c = m->ctx) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->mlps.erase(std::remove(c->mlps.begin(), c->mlps.end(), m), c->mlps.end());
+  }
   delete m;
 }
 

@@ -28,6 +28,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <memory>
 
 #include "ctx.h"
@@ -97,11 +98,14 @@ __global__ void __launch_bounds__(VB) k_vlen(const VItem *__restrict__ items, ui
 // ------------------------------------------------------ extend (walk) pass
 // Dynamic LDS per wave: other lists (nb-1) x 64 x (u64 base + u32 len), dup
 // positions ndup x 64 u32.
+// The item count is read from device memory so that one captured graph can
+// replay the launch for any number of items.
 template <bool FINAL>
-__global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, uint32_t nitems,
+__global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p,
                                             const uint64_t *__restrict__ blk_prefix, unsigned int *__restrict__ queue,
                                             uint32_t max_nb, uint32_t max_ndup,
                                             unsigned long long *__restrict__ prof_items) {
+  const uint32_t nitems = *nitems_p;
   __shared__ uint32_t task;
   __shared__ uint32_t st[VW][65];
   __shared__ uint64_t drv[VW][64];
@@ -211,9 +215,10 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, ui
 }
 
 // Last level, one backward list: |list| - equal-label columns present.
-__global__ void __launch_bounds__(VB) k_vcount1(const VItem *__restrict__ items, uint32_t nitems,
+__global__ void __launch_bounds__(VB) k_vcount1(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p,
                                                const uint64_t *__restrict__ blk_prefix,
                                                unsigned int *__restrict__ queue) {
+  const uint32_t nitems = *nitems_p;
   __shared__ uint32_t task;
   __shared__ uint64_t sm[VW];
   const uint64_t total = blk_prefix[nitems];
@@ -329,6 +334,120 @@ uint32_t grid_for(uint64_t blocks) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks, g_persistent_blocks));
 }
 
+// Fixed buffers for the extend step of every iteration of one run: item
+// tables and block prefixes of the three launches (write, final walk, final
+// single-list count), and a header with the item counts, work counters and
+// per-task result counters.  Host side: pinned mirrors.  Because every
+// pointer, grid and LDS size is fixed for the run, the step
+//   H2D header, items, prefixes -> 3 kernels -> D2H header
+// is captured once as a hipGraph and replayed each iteration (no memset node:
+// the zeroed counters travel in the header copy).
+struct ExtSlot {
+  static constexpr size_t HDR_NITEMS = 0, HDR_QUEUE = 16, HDR_PITEMS = 32, HDR_CTR = 48;
+  uint32_t cap = 0;  // items per launch = counters
+  DevBuf items[3], blk[3], hdr;
+  VItem *h_items[3] = {};
+  uint64_t *h_blk[3] = {};
+  unsigned char *h_hdr = nullptr;
+  size_t hdr_bytes = 0;
+  uint32_t max_nb = 1, max_ndup = 0;
+  size_t lds = 0;
+  uint32_t grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  void init(gem_ctx *c, uint32_t capacity, uint32_t mnb, uint32_t mnd) {
+    hipStream_t s = c->stream;
+    cap = std::max<uint32_t>(capacity, 1);
+    max_nb = std::max<uint32_t>(mnb, 1);
+    max_ndup = mnd;
+    lds = (size_t)VW * 64 * ((max_nb - 1) * 12 + max_ndup * 4);
+    if (lds > 60 * 1024) throw Error{GEM_E_LIMIT, "a level has too many backward edges for the vjoin LDS layout"};
+    grid = g_persistent_blocks;
+    hdr_bytes = HDR_CTR + 8ull * cap;
+    hdr.alloc(hdr_bytes, s);
+    GEM_HIP_CHECK(hipHostMalloc((void **)&h_hdr, hdr_bytes, hipHostMallocDefault));
+    for (int i = 0; i < 3; ++i) {
+      items[i].alloc(sizeof(VItem) * cap, s);
+      blk[i].alloc(8ull * (cap + 1), s);
+      GEM_HIP_CHECK(hipHostMalloc((void **)&h_items[i], sizeof(VItem) * cap, hipHostMallocDefault));
+      GEM_HIP_CHECK(hipHostMalloc((void **)&h_blk[i], 8ull * (cap + 1), hipHostMallocDefault));
+      std::memset(h_items[i], 0, sizeof(VItem) * cap);
+      std::memset(h_blk[i], 0, 8ull * (cap + 1));
+    }
+    std::memset(h_hdr, 0, hdr_bytes);
+  }
+  ~ExtSlot() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (h_hdr) (void)hipHostFree(h_hdr);
+    for (int i = 0; i < 3; ++i) {
+      if (h_items[i]) (void)hipHostFree(h_items[i]);
+      if (h_blk[i]) (void)hipHostFree(h_blk[i]);
+    }
+  }
+  uint32_t *nitems(int i) { return reinterpret_cast<uint32_t *>(h_hdr + HDR_NITEMS) + i; }
+  uint64_t *counters() { return reinterpret_cast<uint64_t *>(h_hdr + HDR_CTR); }
+  uint64_t *pitems() { return reinterpret_cast<uint64_t *>(h_hdr + HDR_PITEMS); }
+  unsigned long long *dev_counter(size_t i) {
+    return reinterpret_cast<unsigned long long *>(hdr.as<unsigned char>() + HDR_CTR) + i;
+  }
+
+  // The step's work, enqueued on the stream (eagerly or under capture).
+  void enqueue(gem_ctx *c, bool prof_kernels, bool count_items) {
+    hipStream_t s = c->stream;
+    unsigned char *d = hdr.as<unsigned char>();
+    GEM_HIP_CHECK(hipMemcpyAsync(d, h_hdr, hdr_bytes, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < 3; ++i) {
+      GEM_HIP_CHECK(hipMemcpyAsync(items[i].p, h_items[i], sizeof(VItem) * cap, hipMemcpyHostToDevice, s));
+      GEM_HIP_CHECK(hipMemcpyAsync(blk[i].p, h_blk[i], 8ull * (cap + 1), hipMemcpyHostToDevice, s));
+    }
+    const uint32_t *dn = reinterpret_cast<const uint32_t *>(d + HDR_NITEMS);
+    unsigned int *dq = reinterpret_cast<unsigned int *>(d + HDR_QUEUE);
+    unsigned long long *dp = reinterpret_cast<unsigned long long *>(d + HDR_PITEMS);
+    {
+      ProfLaunch pl(c, prof_kernels ? K_VEXT_WRITE : -1);
+      k_vext<false><<<grid, VB, lds, s>>>(items[0].as<VItem>(), dn + 0, blk[0].as<uint64_t>(), dq + 0, max_nb,
+                                          max_ndup, count_items ? dp : nullptr);
+      GEM_LAUNCH_CHECK("k_vext<write>");
+    }
+    {
+      ProfLaunch pl(c, prof_kernels ? K_VEXT_FINAL : -1);
+      k_vext<true><<<grid, VB, lds, s>>>(items[1].as<VItem>(), dn + 1, blk[1].as<uint64_t>(), dq + 1, max_nb,
+                                         max_ndup, count_items ? dp + 1 : nullptr);
+      GEM_LAUNCH_CHECK("k_vext<final>");
+    }
+    {
+      ProfLaunch pl(c, prof_kernels ? K_VCOUNT1 : -1);
+      k_vcount1<<<grid, VB, 0, s>>>(items[2].as<VItem>(), dn + 2, blk[2].as<uint64_t>(), dq + 2);
+      GEM_LAUNCH_CHECK("k_vcount1");
+    }
+    GEM_HIP_CHECK(hipMemcpyAsync(h_hdr, d, hdr_bytes, hipMemcpyDeviceToHost, s));
+  }
+
+  void launch(gem_ctx *c, bool use_graph) {
+    if (!use_graph) {
+      enqueue(c, c->prof.on, c->prof.on);
+      return;
+    }
+    if (!exec) {
+      hipStream_t s = c->stream;
+      GEM_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue(c, false, false);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      GEM_HIP_CHECK(hipStreamEndCapture(s, &graph));
+      GEM_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    GEM_HIP_CHECK(hipGraphLaunch(exec, c->stream));
+  }
+};
+
 }  // namespace
 
 void VBatchRunner::run(std::vector<VQueryState *> &qs) {
@@ -338,6 +457,17 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   // use total / active (at most 4 GiB, at least 1 MiB), so a long-running last
   // query is not held to a small slice.
   const uint64_t total_budget = std::max<uint64_t>(budget_bytes_, 1ull << 20);
+  // fixed launch buffers for the run (one item per query per launch at most)
+  ExtSlot slot;
+  {
+    uint32_t mnb = 1, mnd = 0;
+    for (VQueryState *S : qs)
+      for (size_t i = 0; i < S->max_nb.size(); ++i) {
+        mnb = std::max(mnb, S->max_nb[i]);
+        mnd = std::max(mnd, S->max_ndup[i]);
+      }
+    slot.init(c_, (uint32_t)qs.size(), mnb, mnd);
+  }
   // seed each query with the identity level of its first vertex
   for (VQueryState *S : qs) {
     const uint32_t a = S->plan.vorder[0];
@@ -354,7 +484,6 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
     GEM_LAUNCH_CHECK("k_viota");
     S->stack.push_back(VStackItem{1, buf, n0, 1, {}, 0, false});
   }
-  DevBuf counters;
   // GEM_VERBOSE=1: a progress line on stderr every ~2 s of a long batch
   const char *verbose_env = std::getenv("GEM_VERBOSE");
   const bool verbose = verbose_env && verbose_env[0] == '1';
@@ -499,56 +628,30 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
       if (top.cursor >= nblocks) S->stack.pop_back();
     }
     if (nctr == 0) continue;
-    counters.alloc(nctr * 8, s);
-    GEM_HIP_CHECK(hipMemsetAsync(counters.p, 0, nctr * 8, s));
-    auto patch = [&](Launch &L) {
-      for (auto &it : L.items)
-        it.counter = counters.as<unsigned long long>() + (size_t)(uintptr_t)it.counter;
-    };
-    patch(wr);
-    patch(fin);
-    patch(fin1);
-    DevLaunch Dw, Df, Df1;
-    auto ext_lds = [](const Launch &L) {
-      const size_t b = (size_t)VW * 64 * ((L.max_nb > 0 ? L.max_nb - 1 : 0) * 12 + L.max_ndup * 4);
-      if (b > 60 * 1024) throw Error{GEM_E_LIMIT, "a level has too many backward edges for the vjoin LDS layout"};
-      return b;
-    };
-    // profiling: list entries walked per launch (written to counters[nctr])
+    if (nctr > slot.cap || wr.items.size() > slot.cap || fin.items.size() > slot.cap || fin1.items.size() > slot.cap)
+      throw Error{GEM_E_STATE, "more vjoin tasks than launch slots"};
+    // fill the pinned launch tables: counters point into the device header,
+    // counters / work queues / walked-entry counts travel zeroed in the copy
+    Launch *L3[3] = {&wr, &fin, &fin1};
+    for (int i = 0; i < 3; ++i) {
+      Launch &L = *L3[i];
+      for (size_t t = 0; t < L.items.size(); ++t) {
+        VItem it = L.items[t];
+        it.counter = slot.dev_counter((size_t)(uintptr_t)it.counter);
+        slot.h_items[i][t] = it;
+      }
+      for (size_t t = 0; t < L.blk.size(); ++t) slot.h_blk[i][t] = L.blk[t];
+      if (L.blk.empty()) slot.h_blk[i][0] = 0;
+      *slot.nitems(i) = (uint32_t)L.items.size();
+    }
+    std::memset(slot.h_hdr + ExtSlot::HDR_QUEUE, 0, slot.hdr_bytes - ExtSlot::HDR_QUEUE);
     const bool prof = c_->prof.on;
-    DevBuf pitems;
-    if (prof) {
-      pitems.alloc(16, s);
-      GEM_HIP_CHECK(hipMemsetAsync(pitems.p, 0, 16, s));
-    }
-    unsigned long long *pw = prof ? pitems.as<unsigned long long>() : nullptr;
-    if (!wr.empty()) {
-      upload(c_, wr, Dw);
-      ProfLaunch pl(c_, K_VEXT_WRITE);
-      k_vext<false><<<grid_for(wr.blk.back()), VB, ext_lds(wr), s>>>(
-          Dw.items.as<VItem>(), (uint32_t)wr.items.size(), Dw.blk.as<uint64_t>(), Dw.queue.as<unsigned int>(),
-          wr.max_nb, wr.max_ndup, pw);
-      GEM_LAUNCH_CHECK("k_vext<write>");
-    }
-    if (!fin.empty()) {
-      upload(c_, fin, Df);
-      ProfLaunch pl(c_, K_VEXT_FINAL);
-      k_vext<true><<<grid_for(fin.blk.back()), VB, ext_lds(fin), s>>>(
-          Df.items.as<VItem>(), (uint32_t)fin.items.size(), Df.blk.as<uint64_t>(), Df.queue.as<unsigned int>(),
-          fin.max_nb, fin.max_ndup, prof ? pw + 1 : nullptr);
-      GEM_LAUNCH_CHECK("k_vext<final>");
-    }
-    if (!fin1.empty()) {
-      upload(c_, fin1, Df1);
-      ProfLaunch pl(c_, K_VCOUNT1);
-      k_vcount1<<<grid_for(fin1.blk.back()), VB, 0, s>>>(Df1.items.as<VItem>(), (uint32_t)fin1.items.size(),
-                                                        Df1.blk.as<uint64_t>(), Df1.queue.as<unsigned int>());
-      GEM_LAUNCH_CHECK("k_vcount1");
-    }
-    std::vector<uint64_t> hc(nctr + 2, 0);
-    GEM_HIP_CHECK(hipMemcpyAsync(hc.data(), counters.p, nctr * 8, hipMemcpyDeviceToHost, s));
-    if (prof) GEM_HIP_CHECK(hipMemcpyAsync(hc.data() + nctr, pitems.p, 16, hipMemcpyDeviceToHost, s));
+    slot.launch(c_, c_->use_graph && !prof);
     GEM_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<uint64_t> hc(nctr + 2, 0);
+    for (size_t i = 0; i < nctr; ++i) hc[i] = slot.counters()[i];
+    hc[nctr] = slot.pitems()[0];
+    hc[nctr + 1] = slot.pitems()[1];
     if (prof) {
       // algorithmic bytes: rows read (4 B per column + an offset pair per
       // backward list), 4 B per list entry walked, output rows written

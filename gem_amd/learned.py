@@ -37,6 +37,7 @@ class OrderModel:
         self._h = h
         self.in_dim, self.hidden = in_dim, hidden
         self.num_params = int(self._lib.gem_mlp_num_params(h))
+        ctx._models.add(self)  # closed before the context is
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -4,6 +4,7 @@
 Every call runs HIP kernels; there is no CPU path.
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -31,11 +32,14 @@ class GemContext:
         if st != 0:
             raise GemError(st, f"gem_hip_create(device={device})")
         self._h = h
+        self._models = weakref.WeakSet()
         self.graph = None
         self.query = None
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
+            for m in list(getattr(self, "_models", ())):
+                m.close()
             self._lib.gem_hip_destroy(self._h)
             self._h = None
 
@@ -141,6 +145,10 @@ class GemContext:
 
     def set_persistent_blocks(self, blocks):
         self._check(self._lib.gem_hip_set_persistent_blocks(self._h, int(blocks)), "gem_hip_set_persistent_blocks")
+
+    def set_graph(self, on=True):
+        """Replay the batched path's extend step as a captured hipGraph."""
+        self._check(self._lib.gem_hip_set_graph(self._h, 1 if on else 0), "gem_hip_set_graph")
 
     def set_work_limit(self, rows):
         """Stop a join after `rows` processed rows (0 = no limit)."""

@@ -262,6 +262,11 @@ int gem_hip_match_batch(gem_ctx *ctx, const gem_query_view *queries, uint32_t nq
  * kernels; they pull tasks from a device counter, so this sets occupancy,
  * not the amount of work (default 2048).  Set by the autotuner. */
 int gem_hip_set_persistent_blocks(gem_ctx *ctx, uint32_t blocks);
+/* on != 0: the batched path's per-iteration extend step (table uploads, three
+ * persistent kernels, result download) is captured once per run as a hipGraph
+ * and replayed; off (default): launched eagerly.  Kernel profiling forces the
+ * eager form. */
+int gem_hip_set_graph(gem_ctx *ctx, int on);
 
 /* Vertex-at-a-time join of the current query (after gem_hip_filter and
  * gem_hip_build_edge_tables) along vertex_order[num_v]. */
@@ -278,6 +283,9 @@ int gem_hip_vjoin(gem_ctx *ctx, const uint32_t *vertex_order, uint64_t cap, gem_
  * used when no model is given. */
 #define GEM_FEAT_DIM 272u
 typedef struct gem_mlp gem_mlp;
+/* The model lives on ctx's device.  Destroy it before ctx; if ctx is destroyed
+ * first, the model's device memory is released with it and every later call
+ * on the model except gem_mlp_destroy returns GEM_E_STATE. */
 int gem_mlp_create(gem_ctx *ctx, uint32_t in_dim, uint32_t hidden, uint64_t seed, gem_mlp **out);
 void gem_mlp_destroy(gem_mlp *m);
 uint64_t gem_mlp_num_params(const gem_mlp *m);

@@ -82,6 +82,31 @@ def test_batch_equals_plain_with_cap(gpu_ctx, rmat_graph):
         assert (r["count"], r["saturated"]) == (c, s)
 
 
+def test_graph_replays_match_eager(gpu_ctx, rmat_graph):
+    """The extend step replayed as a captured hipGraph (once per iteration,
+    with eager length passes and synchronisations between replays) gives the
+    eager path's counts and level rows, on several consecutive runs."""
+    gpu_ctx.load_data(rmat_graph)
+    qs = _queries(rmat_graph, 12, [4, 5, 6, 7], 13001)
+    cap = 10 ** 8
+    gpu_ctx.set_graph(False)
+    eager = gpu_ctx.match_batch(qs, cap=cap, level_budget=64 << 20)
+    try:
+        gpu_ctx.set_graph(True)
+        for rep in range(3):
+            got = gpu_ctx.match_batch(qs, cap=cap, level_budget=64 << 20)
+            for a, b in zip(eager, got):
+                assert (a["count"], a["saturated"]) == (b["count"], b["saturated"])
+                if not a["saturated"]:
+                    assert np.array_equal(a["level_rows"], b["level_rows"])
+            # eager work between graph runs
+            gpu_ctx.set_graph(False)
+            assert [r["count"] for r in gpu_ctx.match_batch(qs[:3], cap=cap)] == [r["count"] for r in eager[:3]]
+            gpu_ctx.set_graph(True)
+    finally:
+        gpu_ctx.set_graph(False)
+
+
 def test_batch_invariance(gpu_ctx, rmat_graph):
     """Group size, level budget and vertex order do not change any count."""
     gpu_ctx.load_data(rmat_graph)

@@ -86,3 +86,28 @@ def test_learned_orders_gain_and_keep_counts(gpu_ctx, rmat_graph):
     for a, b in zip(gem, lrn):
         if not a["work_limited"] and not b["work_limited"]:
             assert (a["count"], a["saturated"]) == (b["count"], b["saturated"])
+
+
+def test_model_outliving_its_context():
+    """Destroying a context before its model releases the model's device
+    memory; the model then reports GEM_E_STATE and can still be destroyed."""
+    import ctypes as C
+    from gem_amd import GemContext
+    from gem_amd._lib import load
+    lib = load()
+    ctx = GemContext(0)
+    h = C.c_void_p()
+    assert lib.gem_mlp_create(ctx._h, F, H, 1, C.byref(h)) == 0
+    lib.gem_hip_destroy(ctx._h)   # context first, through the C ABI
+    ctx._h = None
+    feats = np.zeros((2, F), np.int32)
+    out = np.zeros(2, np.float32)
+    st = lib.gem_mlp_predict(h, feats.ctypes.data_as(C.POINTER(C.c_int32)), 2,
+                             out.ctypes.data_as(C.POINTER(C.c_float)))
+    assert st == 5  # GEM_E_STATE
+    lib.gem_mlp_destroy(h)
+    # the Python wrappers close models before their context
+    ctx2 = GemContext(0)
+    m = OrderModel(ctx2, seed=2)
+    ctx2.close()
+    assert m._h is None
