@@ -49,6 +49,12 @@ class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 40), ("launches", C.c_uint64), ("ms", C.c_double), ("bytes", C.c_double)]
 
 
+class StreamStats(C.Structure):
+    _fields_ = [("table_bytes", C.c_uint64), ("h2d_bytes", C.c_uint64), ("blocks_loaded", C.c_uint64),
+                ("blocks_hit", C.c_uint64), ("groups", C.c_uint64), ("serial_groups", C.c_uint64),
+                ("copy_ms", C.c_double), ("step_ms", C.c_double)]
+
+
 class PhaseTimes(C.Structure):
     _fields_ = [("filter_ms", C.c_float), ("index_ms", C.c_float), ("tables_ms", C.c_float),
                 ("join_ms", C.c_float)]
@@ -79,6 +85,8 @@ SIGNATURES = [
     ("gem_hip_set_work_limit", C.c_int, [C.c_void_p, C.c_uint64]),
     ("gem_hip_set_persistent_blocks", C.c_int, [C.c_void_p, C.c_uint32]),
     ("gem_hip_set_graph", C.c_int, [C.c_void_p, C.c_int]),
+    ("gem_hip_set_table_stream", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
+    ("gem_hip_stream_stats", C.c_int, [C.c_void_p, C.POINTER(StreamStats)]),
     ("gem_hip_load_data_csr", C.c_int, [C.c_void_p, C.POINTER(CsrView)]),
     ("gem_hip_filter", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, u32p, u32p, C.c_uint64]),
     ("gem_hip_position_lookup", C.c_int, [C.c_void_p, C.c_uint32, u32p, C.c_uint64, u32p]),

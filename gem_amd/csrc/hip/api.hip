@@ -93,6 +93,26 @@ int gem_hip_set_graph(gem_ctx *c, int on) {
   return GEM_OK;
 }
 
+int gem_hip_set_table_stream(gem_ctx *c, uint64_t cache_bytes, uint64_t slot_bytes) {
+  if (!c) return GEM_E_INVALID;
+  if (slot_bytes == 0) slot_bytes = 1ull << 20;
+  // slots hold whole u32 lists and are 256-byte aligned; two slots at least
+  if (slot_bytes % 256 || (cache_bytes && cache_bytes < 2 * slot_bytes)) {
+    c->err = "slot_bytes must be a multiple of 256 and cache_bytes at least two slots";
+    return GEM_E_INVALID;
+  }
+  c->stream_cache_bytes = cache_bytes;
+  c->stream_slot_bytes = slot_bytes;
+  return GEM_OK;
+}
+
+int gem_hip_stream_stats(gem_ctx *c, gem_stream_stats *out) {
+  if (!c || !out) return GEM_E_INVALID;
+  *out = c->sstats;
+  c->sstats = gem_stream_stats{};
+  return GEM_OK;
+}
+
 int gem_hip_set_work_limit(gem_ctx *c, uint64_t rows) {
   if (!c) return GEM_E_INVALID;
   c->work_limit_rows = rows;

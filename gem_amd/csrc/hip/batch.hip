@@ -4,6 +4,7 @@
 #include <algorithm>
 
 #include "ctx.h"
+#include "tcache.h"
 #include "vjoin.h"
 
 namespace gemd {
@@ -71,6 +72,7 @@ void run_match_batch(gem_ctx *c, const gem_query_view *qs, uint32_t nq, const ui
   gem_batch_opts o{};
   if (opts) o = *opts;
   const uint32_t group = o.group ? o.group : std::max<uint32_t>(nq, 1);
+  const bool streamed = c->stream_cache_bytes != 0;
   hipStream_t s = c->stream;
   GEM_HIP_CHECK(hipEventRecord(c->ev[6], s));
   for (uint32_t g0 = 0; g0 < nq; g0 += group) {
@@ -83,9 +85,10 @@ void run_match_batch(gem_ctx *c, const gem_query_view *qs, uint32_t nq, const ui
       run_filter(c, qd[i], &qs[g0 + i], 0);
       run_tables(c, qd[i]);
       plan_query(c, st[i], qd[i], vorders ? vorders[g0 + i] : nullptr);
+      if (streamed) spill_tables(c, qd[i], c->stream_slot_bytes);
       ptrs.push_back(&st[i]);
     }
-    VBatchRunner runner(c, o.cap, o.work_limit, budget);
+    VBatchRunner runner(c, o.cap, o.work_limit, budget, streamed);
     runner.run(ptrs);
     for (uint32_t i = 0; i < gn; ++i)
       fill_result(st[i], o.cap, &results[g0 + i], level_rows ? level_rows + (uint64_t)(g0 + i) * GEM_MAX_QV : nullptr);
@@ -120,6 +123,7 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
   if (opts) o = *opts;
   const uint32_t group = o.group ? o.group : std::max<uint32_t>(nq, 1);
   max_orders = std::max<uint32_t>(1, max_orders);
+  const bool streamed = c->stream_cache_bytes != 0;
   hipStream_t s = c->stream;
   GEM_HIP_CHECK(hipEventRecord(c->ev[6], s));
   for (uint32_t g0 = 0; g0 < nq; g0 += group) {
@@ -134,6 +138,7 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
     for (uint32_t i = 0; i < gn; ++i) {
       run_filter(c, qd[i], &qs[g0 + i], 0);
       run_tables(c, qd[i]);
+      if (streamed) spill_tables(c, qd[i], c->stream_slot_bytes);
       const gemh::QueryStats qsx = query_stats(c, qd[i]);
       gemh::candidate_orders(qd[i].Q, qsx, gem_vertex_order(qd[i]), mlp ? max_orders : 1, 0x9e3779b9ull + g0 + i,
                              &cand[i]);
@@ -165,7 +170,7 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
       ptrs.push_back(&st[i]);
       if (chosen) chosen[g0 + i] = pick[i];
     }
-    VBatchRunner runner(c, o.cap, o.work_limit, budget);
+    VBatchRunner runner(c, o.cap, o.work_limit, budget, streamed);
     runner.run(ptrs);
     for (uint32_t i = 0; i < gn; ++i)
       fill_result(st[i], o.cap, &results[g0 + i], level_rows ? level_rows + (uint64_t)(g0 + i) * GEM_MAX_QV : nullptr);

@@ -110,12 +110,56 @@ struct FilterQuery {
   uint8_t req_cnt[GEM_MAX_QV][GEM_MAX_QV];
 };
 
+// Pinned host allocation (hipHostMalloc), move-only.
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  PinnedBuf(PinnedBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; }
+  PinnedBuf &operator=(PinnedBuf &&o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+    }
+    return *this;
+  }
+  ~PinnedBuf() { reset(); }
+  void alloc(size_t n) {
+    reset();
+    if (n == 0) return;
+    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error{GEM_E_NOMEM, "hipHostMalloc(" + std::to_string(n) + " bytes): " + hipGetErrorString(e)};
+    }
+    bytes = n;
+  }
+  void reset() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
 struct TableDev {
   bool present = false;
   uint32_t key_u = 0, val_u = 0;
   uint64_t size = 0;
   DevBuf off;  // u64[|Can(key)| + 1]
-  DevBuf val;  // u32[size], positions in Can(val_u)
+  DevBuf val;  // u32[size], positions in Can(val_u); empty once streamed
+  // streamed form (values in pinned host memory): keys [b << kshift,
+  // (b + 1) << kshift) form block b, whose values are blk_off[b]..blk_off[b+1]
+  const uint32_t *hval = nullptr;
+  uint32_t kshift = 0;
+  std::vector<uint64_t> blk_off;  // nblk + 1 entries
 };
 
 struct QueryDev {
@@ -130,6 +174,8 @@ struct QueryDev {
   DevBuf fq;                           // FilterQuery
   DevBuf slot_of;                      // u8[num data labels]: dense label -> query slot
   std::vector<TableDev> tab;           // per directed edge
+  bool streamed = false;               // table values live in host_vals
+  PinnedBuf host_vals;
 };
 
 // Kernel-level profiling: HIP events around launches on the context stream
@@ -189,6 +235,12 @@ struct gem_ctx {
   uint64_t work_limit_rows = 0;             // 0 = none
   uint32_t persistent_blocks = 2048;        // grid of the batched path's persistent kernels
   bool use_graph = false;                   // replay the batched extend step as a hipGraph
+  // batched path with streamed edge tables: values in pinned host memory,
+  // a device cache of stream_cache_bytes in slots of stream_slot_bytes
+  // (0 = tables stay in HBM)
+  uint64_t stream_cache_bytes = 0;
+  uint64_t stream_slot_bytes = 1ull << 20;
+  gem_stream_stats sstats{};
   void *pinned = nullptr;                   // small pinned staging area
   size_t pinned_bytes = 0;
   gemd::Prof prof;

@@ -1,0 +1,97 @@
+// Streamed edge tables for the batched join (out-of-core form).
+//
+// After a query's tables are built in HBM, spill_tables() moves their values
+// to pinned host memory and keeps only the offsets on the device.  Each table
+// is cut into key-range blocks (2^kshift keys each) whose values fit one cache
+// slot.  During the join, every iteration of the batched runner
+//   1. marks, per block, the first and last 256-row task that reads it
+//      (k_vmark: atomicMin / atomicMax of the task index),
+//   2. cuts its tasks into groups such that the blocks live in two
+//      consecutive groups fit the cache together,
+//   3. for each group, copies the missing blocks host -> device on a second
+//      stream while the previous group's kernels run, rewrites their entries
+//      of the block map (vmap: block -> device address of its values, minus
+//      the block's first offset, so that address = vmap[block] + 4 * offset),
+//      and lets the group's kernels start once its copies are done.
+// Absent blocks map to a zeroed guard slot, so a read of a block that is not
+// resident stays inside device memory (and shows up as a wrong count, never
+// as a fault).  The cache keeps blocks across iterations (least recently used
+// ones are replaced first; blocks needed later in the same iteration last).
+#pragma once
+
+#include <vector>
+
+#include "ctx.h"
+#include "vjoin.h"
+
+namespace gemd {
+
+// Move the values of Qd's tables to pinned host memory (block layout for
+// slot_bytes-sized cache slots).  Throws GEM_E_LIMIT if one list is longer
+// than a slot.
+void spill_tables(gem_ctx *c, QueryDev &Qd, uint64_t slot_bytes);
+
+class TableCache {
+ public:
+  struct Group {
+    uint64_t a = 0, b = 0;  // task range [a, b) of the iteration
+    bool serial = false;    // its copies wait for the previous group's kernels
+    size_t p0 = 0, p1 = 0;  // its entries in the patch list
+    std::vector<std::pair<uint64_t, uint32_t>> loads;  // (block, slot)
+  };
+
+  TableCache(gem_ctx *c, uint64_t cache_bytes, uint64_t slot_bytes);
+  ~TableCache();
+  TableCache(const TableCache &) = delete;
+  TableCache &operator=(const TableCache &) = delete;
+
+  // Assign global block ids to the queries' tables, then patch and upload
+  // their level descriptors (vmap, kshift, gblk).
+  void attach(std::vector<VQueryState *> &qs);
+
+  uint32_t *d_first() const { return first_.as<uint32_t>(); }
+  uint32_t *d_last() const { return last_.as<uint32_t>(); }
+
+  // On the context stream: first = ~0, last = 0 for every block.
+  void reset_marks();
+  // Read the marks back (synchronises the context stream), then cut
+  // total_tasks into groups and decide every copy and eviction.
+  void plan(uint64_t total_tasks);
+  const std::vector<Group> &groups() const { return groups_; }
+  // Device queue counters for the group launches (3 per group), zeroed.
+  unsigned int *queues();
+  // Copy stream: group g's block copies and map updates; the context stream
+  // waits for them.
+  void issue(size_t g);
+  // Context stream: mark the end of group g's kernels.
+  void done(size_t g);
+  // After the iteration's stream synchronisation: accumulate statistics.
+  void finish();
+
+ private:
+  gem_ctx *c_;
+  uint64_t slot_bytes_;
+  uint32_t nslots_;
+  hipStream_t copy_ = nullptr;
+  DevBuf pool_;  // (nslots + 1) slots; the last is the zeroed guard
+  DevBuf vmap_, first_, last_, dpatch_, queues_;
+  PinnedBuf h_first_, h_last_, h_patch_;
+  uint64_t nblk_ = 0;
+  std::vector<const uint32_t *> bsrc_;  // host address of block values
+  std::vector<uint64_t> boff_, blen_;   // first offset and length (entries)
+  std::vector<uint32_t> blk_slot_;      // block -> slot or NONE
+  std::vector<uint64_t> slot_blk_;      // slot -> block or NONE
+  std::vector<uint64_t> stamp_;         // slot -> last use
+  std::vector<int64_t> live_in_;        // block -> id of the last group it was live in
+  int64_t gbase_ = 0;                   // id of this iteration's first group
+  uint64_t clock_ = 0;
+  std::vector<Group> groups_;
+  std::vector<hipEvent_t> ev_;          // 3 per group: copy begin, copy end, kernels end
+  hipEvent_t ev_step0_ = nullptr;
+  size_t npatch_ = 0;
+  uint64_t hits_ = 0;
+  uint64_t guard_addr() const;
+  uint64_t slot_addr(uint32_t s) const;
+};
+
+}  // namespace gemd

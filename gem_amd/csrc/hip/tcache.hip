@@ -1,0 +1,350 @@
+// Streamed edge tables: spill to pinned host memory, block cache in HBM,
+// per-iteration copy plan.  The join kernels that read through the block map
+// and the marking kernel are in vjoin.hip.
+#include <algorithm>
+#include <cstring>
+
+#include "tcache.h"
+
+namespace gemd {
+
+namespace {
+
+constexpr uint32_t NONE32 = 0xFFFFFFFFu;
+constexpr uint64_t NONE64 = ~0ull;
+
+struct Patch {
+  uint64_t idx, val;
+};
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_patch(const Patch *__restrict__ pt, uint64_t n, uint64_t *__restrict__ vmap) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) vmap[pt[i].idx] = pt[i].val;
+}
+
+uint32_t fill_grid(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 4096)); }
+
+}  // namespace
+
+// ------------------------------------------------------------------- spill
+void spill_tables(gem_ctx *c, QueryDev &Qd, uint64_t slot_bytes) {
+  hipStream_t s = c->stream;
+  const uint64_t slot_entries = slot_bytes / 4;
+  uint64_t total = 0;
+  for (const TableDev &T : Qd.tab)
+    if (T.present) total += T.size * 4;
+  Qd.host_vals.alloc(std::max<uint64_t>(total, 4));
+  unsigned char *dst = Qd.host_vals.as<unsigned char>();
+  for (TableDev &T : Qd.tab) {
+    if (!T.present) continue;
+    const uint64_t nkeys = Qd.ncan[T.key_u];
+    std::vector<uint64_t> off(nkeys + 1);
+    GEM_HIP_CHECK(hipMemcpyAsync(off.data(), T.off.p, 8 * (nkeys + 1), hipMemcpyDeviceToHost, s));
+    if (T.size) GEM_HIP_CHECK(hipMemcpyAsync(dst, T.val.p, T.size * 4, hipMemcpyDeviceToHost, s));
+    GEM_HIP_CHECK(hipStreamSynchronize(s));
+    T.hval = reinterpret_cast<const uint32_t *>(dst);
+    dst += T.size * 4;
+    c->sstats.table_bytes += T.size * 4;
+    uint64_t longest = 0;
+    for (uint64_t k = 0; k < nkeys; ++k) longest = std::max(longest, off[k + 1] - off[k]);
+    if (longest > slot_entries)
+      throw Error{GEM_E_LIMIT, "a table list of " + std::to_string(longest) + " entries is longer than a " +
+                                   std::to_string(slot_bytes) + "-byte cache slot"};
+    // the widest key range whose every block fits a slot (halving a block
+    // that fits gives blocks that fit, and single keys fit)
+    uint32_t ks = 20;
+    for (;; --ks) {
+      bool ok = true;
+      for (uint64_t k0 = 0; k0 < nkeys && ok; k0 += 1ull << ks)
+        ok = off[std::min<uint64_t>(k0 + (1ull << ks), nkeys)] - off[k0] <= slot_entries;
+      if (ok || ks == 0) break;
+    }
+    T.kshift = ks;
+    const uint64_t nblk = (nkeys + (1ull << ks) - 1) >> ks;
+    T.blk_off.resize(nblk + 1);
+    for (uint64_t b = 0; b <= nblk; ++b) T.blk_off[b] = off[std::min<uint64_t>(b << ks, nkeys)];
+    T.val.reset();
+  }
+  Qd.streamed = true;
+}
+
+// ------------------------------------------------------------------- cache
+TableCache::TableCache(gem_ctx *c, uint64_t cache_bytes, uint64_t slot_bytes) : c_(c), slot_bytes_(slot_bytes) {
+  nslots_ = (uint32_t)std::max<uint64_t>(cache_bytes / slot_bytes, 2);
+  hipStream_t s = c->stream;
+  pool_.alloc((uint64_t)(nslots_ + 1) * slot_bytes_, s);
+  GEM_HIP_CHECK(hipMemsetAsync(pool_.as<unsigned char>() + (uint64_t)nslots_ * slot_bytes_, 0, slot_bytes_, s));
+  GEM_HIP_CHECK(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking));
+  GEM_HIP_CHECK(hipEventCreate(&ev_step0_));
+  slot_blk_.assign(nslots_, NONE64);
+  stamp_.assign(nslots_, 0);
+}
+
+TableCache::~TableCache() {
+  if (copy_) {
+    (void)hipStreamSynchronize(copy_);
+    (void)hipStreamDestroy(copy_);
+  }
+  if (c_->stream) (void)hipStreamSynchronize(c_->stream);
+  for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+  if (ev_step0_) (void)hipEventDestroy(ev_step0_);
+}
+
+uint64_t TableCache::guard_addr() const {
+  return (uint64_t)(uintptr_t)(pool_.as<unsigned char>() + (uint64_t)nslots_ * slot_bytes_);
+}
+uint64_t TableCache::slot_addr(uint32_t s) const {
+  return (uint64_t)(uintptr_t)(pool_.as<unsigned char>() + (uint64_t)s * slot_bytes_);
+}
+
+void TableCache::attach(std::vector<VQueryState *> &qs) {
+  hipStream_t s = c_->stream;
+  // global block ids, table by table
+  std::vector<std::vector<uint64_t>> base(qs.size());
+  nblk_ = 0;
+  for (size_t i = 0; i < qs.size(); ++i) {
+    const QueryDev &Qd = *qs[i]->qd;
+    if (!Qd.streamed) throw Error{GEM_E_STATE, "streamed join over tables that were not spilled"};
+    base[i].assign(Qd.tab.size(), 0);
+    for (size_t d = 0; d < Qd.tab.size(); ++d) {
+      const TableDev &T = Qd.tab[d];
+      if (!T.present) continue;
+      base[i][d] = nblk_;
+      const uint64_t nb = T.blk_off.size() - 1;
+      for (uint64_t b = 0; b < nb; ++b) {
+        bsrc_.push_back(T.hval + T.blk_off[b]);
+        boff_.push_back(T.blk_off[b]);
+        blen_.push_back(T.blk_off[b + 1] - T.blk_off[b]);
+      }
+      nblk_ += nb;
+    }
+  }
+  if (nblk_ >= NONE32) throw Error{GEM_E_LIMIT, "too many table blocks for one batch"};
+  blk_slot_.assign(nblk_, NONE32);
+  live_in_.assign(nblk_, -1);
+  const uint64_t n = std::max<uint64_t>(nblk_, 1);
+  vmap_.alloc(8 * n, s);
+  first_.alloc(4 * n, s);
+  last_.alloc(4 * n, s);
+  h_first_.alloc(4 * n);
+  h_last_.alloc(4 * n);
+  std::vector<uint64_t> hv(n, 0);
+  for (uint64_t b = 0; b < nblk_; ++b) hv[b] = guard_addr() - 4 * boff_[b];
+  GEM_HIP_CHECK(hipMemcpyAsync(vmap_.p, hv.data(), 8 * n, hipMemcpyHostToDevice, s));
+  // level descriptors read the map
+  for (size_t i = 0; i < qs.size(); ++i) {
+    VQueryState &S = *qs[i];
+    const QueryDev &Qd = *S.qd;
+    for (uint32_t lv = 1; lv < Qd.Q.nv; ++lv) {
+      const gemh::VLevelPlan &lp = S.plan.levels[lv];
+      VLevel &L = S.hlv[lv];
+      for (uint32_t j = 0; j < lp.nb; ++j) {
+        const uint32_t d = lp.directed[j];
+        L.vmap[j] = vmap_.as<uint64_t>() + base[i][d];
+        L.kshift[j] = Qd.tab[d].kshift;
+        L.gblk[j] = (uint32_t)base[i][d];
+      }
+    }
+    GEM_HIP_CHECK(hipMemcpyAsync(S.levels.p, S.hlv.data(), sizeof(VLevel) * S.hlv.size(), hipMemcpyHostToDevice, s));
+  }
+  // the copy stream uses buffers allocated on the context stream
+  GEM_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void TableCache::reset_marks() {
+  if (!nblk_) return;
+  hipStream_t s = c_->stream;
+  k_fill_u32<<<fill_grid(nblk_), 256, 0, s>>>(first_.as<uint32_t>(), nblk_, NONE32);
+  k_fill_u32<<<fill_grid(nblk_), 256, 0, s>>>(last_.as<uint32_t>(), nblk_, 0);
+  GEM_LAUNCH_CHECK("k_fill_u32");
+}
+
+void TableCache::plan(uint64_t total) {
+  hipStream_t s = c_->stream;
+  groups_.clear();
+  npatch_ = 0;
+  hits_ = 0;
+  if (nblk_) {
+    GEM_HIP_CHECK(hipMemcpyAsync(h_first_.p, first_.p, 4 * nblk_, hipMemcpyDeviceToHost, s));
+    GEM_HIP_CHECK(hipMemcpyAsync(h_last_.p, last_.p, 4 * nblk_, hipMemcpyDeviceToHost, s));
+  }
+  GEM_HIP_CHECK(hipStreamSynchronize(s));
+  if (total == 0) return;
+  const uint32_t *fi = h_first_.as<uint32_t>(), *la = h_last_.as<uint32_t>();
+  // needed blocks (empty blocks are never copied: their lists read nothing)
+  std::vector<uint64_t> need;
+  std::vector<uint32_t> scount(total + 1, 0), ecount(total + 1, 0);
+  for (uint64_t b = 0; b < nblk_; ++b) {
+    if (fi[b] == NONE32 || blen_[b] == 0) continue;
+    if (fi[b] >= total || la[b] >= total || la[b] < fi[b]) throw Error{GEM_E_STATE, "block marks out of range"};
+    need.push_back(b);
+    scount[fi[b] + 1]++;
+    ecount[la[b] + 1]++;
+  }
+  // S[t] = #needed blocks first read before task t, E[t] = #last read before t
+  for (uint64_t t = 1; t <= total; ++t) {
+    scount[t] += scount[t - 1];
+    ecount[t] += ecount[t - 1];
+  }
+  auto S = [&](uint64_t t) { return (uint64_t)scount[t]; };
+  auto E = [&](uint64_t t) { return (uint64_t)ecount[t]; };
+  // groups: the blocks live in the previous and the current group fit the
+  // cache together (copies overlap the previous group's kernels); at least
+  // four groups when everything fits, so copies still overlap
+  const uint64_t target = std::max<uint64_t>(1, (total + 3) / 4);
+  uint64_t a_prev = 0, a = 0;
+  while (a < total) {
+    Group G;
+    G.a = a;
+    uint64_t lo = a_prev;
+    if (S(a + 1) - E(lo) > nslots_) {
+      G.serial = true;
+      lo = a;
+      if (S(a + 1) - E(a) > nslots_)
+        throw Error{GEM_E_LIMIT, "the table blocks read by one 256-row task do not fit the device cache (" +
+                                     std::to_string(S(a + 1) - E(a)) + " blocks, " + std::to_string(nslots_) +
+                                     " slots)"};
+    }
+    uint64_t b = a + 1;
+    while (b < total && b - a < target && S(b + 1) - E(lo) <= nslots_) ++b;
+    G.b = b;
+    groups_.push_back(std::move(G));
+    a_prev = a;
+    a = b;
+  }
+  // simulate the cache over the groups: which blocks each group copies and
+  // which it evicts (every change is one map patch)
+  std::sort(need.begin(), need.end(), [&](uint64_t x, uint64_t y) { return fi[x] < fi[y]; });
+  std::vector<Patch> patches;
+  std::vector<uint64_t> live;  // needed blocks first read before the group's end
+  size_t nx = 0;
+  for (size_t g = 0; g < groups_.size(); ++g) {
+    Group &G = groups_[g];
+    while (nx < need.size() && fi[need[nx]] < G.b) live.push_back(need[nx++]);
+    live.erase(std::remove_if(live.begin(), live.end(), [&](uint64_t b) { return la[b] < G.a; }), live.end());
+    const int64_t gid = gbase_ + (int64_t)g;  // unique over the run's iterations
+    for (uint64_t b : live) live_in_[b] = gid;
+    ++clock_;
+    std::vector<uint64_t> miss;
+    for (uint64_t b : live) {
+      if (blk_slot_[b] != NONE32) {
+        stamp_[blk_slot_[b]] = clock_;
+        ++hits_;
+      } else {
+        miss.push_back(b);
+      }
+    }
+    G.p0 = patches.size();
+    if (!miss.empty()) {
+      // victims: free slots, then slots whose block is not live in this or
+      // (unless serial) the previous group: blocks not read again in this
+      // iteration first (least recently used first), then those read latest
+      const int64_t keep_from = G.serial ? gid : gid - 1;
+      std::vector<std::pair<uint64_t, uint32_t>> cand;  // (priority, slot)
+      for (uint32_t sl = 0; sl < nslots_; ++sl) {
+        const uint64_t b = slot_blk_[sl];
+        if (b == NONE64) {
+          cand.push_back({0, sl});
+          continue;
+        }
+        if (live_in_[b] >= keep_from) continue;
+        const bool later = fi[b] != NONE32 && la[b] >= G.b;
+        cand.push_back({later ? (2ull << 62) - fi[b] : (1ull << 62) + stamp_[sl], sl});
+      }
+      if (cand.size() < miss.size()) throw Error{GEM_E_STATE, "table cache plan ran out of slots"};
+      std::partial_sort(cand.begin(), cand.begin() + miss.size(), cand.end());
+      for (size_t k = 0; k < miss.size(); ++k) {
+        const uint32_t sl = cand[k].second;
+        const uint64_t old = slot_blk_[sl];
+        if (old != NONE64) {
+          blk_slot_[old] = NONE32;
+          patches.push_back({old, guard_addr() - 4 * boff_[old]});
+        }
+        const uint64_t b = miss[k];
+        slot_blk_[sl] = b;
+        blk_slot_[b] = sl;
+        stamp_[sl] = clock_;
+        patches.push_back({b, slot_addr(sl) - 4 * boff_[b]});
+        G.loads.push_back({b, sl});
+      }
+    }
+    G.p1 = patches.size();
+  }
+  // one unused id between iterations: the previous iteration's last group is
+  // never "the previous group" of this one's first
+  gbase_ += (int64_t)groups_.size() + 1;
+  npatch_ = patches.size();
+  // patch lists: pinned (async copy source) and device copies; growing them
+  // is safe here because the previous iteration has been synchronised
+  const size_t need_bytes = std::max<size_t>(npatch_, 1) * sizeof(Patch);
+  if (h_patch_.bytes < need_bytes) h_patch_.alloc(need_bytes * 2);
+  if (dpatch_.bytes < need_bytes) {
+    dpatch_.alloc(need_bytes * 2, s);
+    GEM_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  if (npatch_) std::memcpy(h_patch_.p, patches.data(), npatch_ * sizeof(Patch));
+  // events: 3 per group
+  while (ev_.size() < 3 * groups_.size()) {
+    hipEvent_t e;
+    GEM_HIP_CHECK(hipEventCreate(&e));
+    ev_.push_back(e);
+  }
+  GEM_HIP_CHECK(hipEventRecord(ev_step0_, s));
+}
+
+unsigned int *TableCache::queues() {
+  hipStream_t s = c_->stream;
+  const size_t need = 4 * 3 * std::max<size_t>(groups_.size(), 1);
+  if (queues_.bytes < need) queues_.alloc(need * 2, s);
+  GEM_HIP_CHECK(hipMemsetAsync(queues_.p, 0, need, s));
+  return queues_.as<unsigned int>();
+}
+
+void TableCache::issue(size_t g) {
+  const Group &G = groups_[g];
+  hipStream_t s = c_->stream;
+  // blocks this group may evict were last read by group g-2 (g-1 if serial)
+  const int64_t wait_for = G.serial ? (int64_t)g - 1 : (int64_t)g - 2;
+  if (g == 0) GEM_HIP_CHECK(hipStreamWaitEvent(copy_, ev_step0_, 0));
+  if (wait_for >= 0) GEM_HIP_CHECK(hipStreamWaitEvent(copy_, ev_[3 * wait_for + 2], 0));
+  GEM_HIP_CHECK(hipEventRecord(ev_[3 * g], copy_));
+  for (const auto &ld : G.loads) {
+    const uint64_t b = ld.first;
+    GEM_HIP_CHECK(hipMemcpyAsync(pool_.as<unsigned char>() + (uint64_t)ld.second * slot_bytes_, bsrc_[b],
+                                 4 * blen_[b], hipMemcpyHostToDevice, copy_));
+    c_->sstats.h2d_bytes += 4 * blen_[b];
+  }
+  c_->sstats.blocks_loaded += G.loads.size();
+  const size_t np = G.p1 - G.p0;
+  if (np) {
+    Patch *dp = dpatch_.as<Patch>() + G.p0;
+    GEM_HIP_CHECK(hipMemcpyAsync(dp, h_patch_.as<Patch>() + G.p0, np * sizeof(Patch), hipMemcpyHostToDevice, copy_));
+    k_patch<<<(uint32_t)((np + 255) / 256), 256, 0, copy_>>>(dp, np, vmap_.as<uint64_t>());
+    GEM_LAUNCH_CHECK("k_patch");
+  }
+  GEM_HIP_CHECK(hipEventRecord(ev_[3 * g + 1], copy_));
+  GEM_HIP_CHECK(hipStreamWaitEvent(s, ev_[3 * g + 1], 0));
+  c_->sstats.groups += 1;
+  c_->sstats.serial_groups += G.serial ? 1 : 0;
+}
+
+void TableCache::done(size_t g) { GEM_HIP_CHECK(hipEventRecord(ev_[3 * g + 2], c_->stream)); }
+
+void TableCache::finish() {
+  c_->sstats.blocks_hit += hits_;
+  if (groups_.empty()) return;
+  GEM_HIP_CHECK(hipStreamSynchronize(copy_));
+  for (size_t g = 0; g < groups_.size(); ++g) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ev_[3 * g], ev_[3 * g + 1]) == hipSuccess) c_->sstats.copy_ms += ms;
+  }
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, ev_step0_, ev_[3 * (groups_.size() - 1) + 2]) == hipSuccess) c_->sstats.step_ms += ms;
+}
+
+}  // namespace gemd

@@ -14,11 +14,16 @@ struct VLevel {
   uint32_t nb, ndup;
   uint32_t key_col[GEM_MAX_QV];
   const uint64_t *toff[GEM_MAX_QV];
-  const uint32_t *tval[GEM_MAX_QV];
+  const uint32_t *tval[GEM_MAX_QV];  // in-HBM tables
   uint32_t dup_col[GEM_MAX_QV];
   const uint32_t *dup_can[GEM_MAX_QV];
   const uint64_t *bm_new;
   const uint32_t *rank_new;
+  // streamed tables: block map of table j (address of the values of key p =
+  // vmap[j][p >> kshift[j]] + 4 * toff[j][p]) and its first global block id
+  const uint64_t *vmap[GEM_MAX_QV];
+  uint32_t kshift[GEM_MAX_QV];
+  uint32_t gblk[GEM_MAX_QV];
 };
 
 // One unit of a launch: a row range of one pending level of one query.
@@ -46,10 +51,14 @@ struct VQueryState {
   const QueryDev *qd = nullptr;
   gemh::VertexPlan plan;
   DevBuf levels;  // VLevel[nv]
+  std::vector<VLevel> hlv;  // host copy of `levels`
   std::vector<uint32_t> max_nb, max_ndup;
   std::vector<VStackItem> stack;
   std::vector<uint64_t> level_rows;  // rows after each level
   uint64_t count = 0, processed = 0;
+  // rows read plus (upper bound of) list entries walked: what the work limit
+  // bounds, since one row with hub lists can cost 10^5 entries
+  uint64_t work = 0;
   bool saturated = false, done = false, work_limited = false;
 };
 
@@ -57,14 +66,17 @@ void build_vlevels(gem_ctx *c, VQueryState &S);
 
 class VBatchRunner {
  public:
-  VBatchRunner(gem_ctx *c, uint64_t cap, uint64_t work_limit, uint64_t budget_bytes)
-      : c_(c), cap_(cap), work_limit_(work_limit), budget_bytes_(budget_bytes) {}
+  // streamed: the queries' tables were spilled to host memory (spill_tables)
+  // and are read through a device block cache of the context's size
+  VBatchRunner(gem_ctx *c, uint64_t cap, uint64_t work_limit, uint64_t budget_bytes, bool streamed = false)
+      : c_(c), cap_(cap), work_limit_(work_limit), budget_bytes_(budget_bytes), streamed_(streamed) {}
   void run(std::vector<VQueryState *> &qs);
   uint64_t iterations() const { return iterations_; }
 
  private:
   gem_ctx *c_;
   uint64_t cap_, work_limit_, budget_bytes_;
+  bool streamed_;
   uint64_t iterations_ = 0;
 };
 

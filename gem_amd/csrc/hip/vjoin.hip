@@ -32,6 +32,7 @@
 #include <memory>
 
 #include "ctx.h"
+#include "tcache.h"
 #include "vjoin.h"
 #include "wave.h"
 
@@ -95,21 +96,30 @@ __global__ void __launch_bounds__(VB) k_vlen(const VItem *__restrict__ items, ui
   }
 }
 
+// Device address of the list of key p in backward table j (b = its first
+// offset): in HBM, or through the streamed tables' block map.
+template <bool OOC>
+__device__ __forceinline__ uint64_t list_addr(const VLevel &L, uint32_t j, uint32_t p, uint64_t b) {
+  if (OOC) return L.vmap[j][p >> L.kshift[j]] + 4 * b;
+  return (uint64_t)(uintptr_t)(L.tval[j] + b);
+}
+
 // ------------------------------------------------------ extend (walk) pass
-// Dynamic LDS per wave: other lists (nb-1) x 64 x (u64 base + u32 len), dup
-// positions ndup x 64 u32.
+// Dynamic LDS per wave: other lists (nb-1) x 64 x (u64 address + u32 len),
+// dup positions ndup x 64 u32.
 // The item count is read from device memory so that one captured graph can
-// replay the launch for any number of items.
-template <bool FINAL>
+// replay the launch for any number of items.  Tasks [g_begin, g_end) of the
+// launch are processed (streamed tables run a launch in task groups).
+template <bool FINAL, bool OOC>
 __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p,
                                             const uint64_t *__restrict__ blk_prefix, unsigned int *__restrict__ queue,
                                             uint32_t max_nb, uint32_t max_ndup,
-                                            unsigned long long *__restrict__ prof_items) {
+                                            unsigned long long *__restrict__ prof_items, uint64_t g_begin,
+                                            uint64_t g_end) {
   const uint32_t nitems = *nitems_p;
   __shared__ uint32_t task;
   __shared__ uint32_t st[VW][65];
-  __shared__ uint64_t drv[VW][64];
-  __shared__ uint32_t dlist[VW][64];  // which backward list drives each row
+  __shared__ uint64_t drv[VW][64];  // address of each row's driving list
   extern __shared__ unsigned char vlds[];
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const uint32_t per_wave = 64 * ((max_nb > 0 ? max_nb - 1 : 0) * 12 + max_ndup * 4);
@@ -117,11 +127,11 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
   uint64_t *obase = reinterpret_cast<uint64_t *>(mine);                                  // [nb-1][64]
   uint32_t *olen = reinterpret_cast<uint32_t *>(mine + (size_t)64 * 8 * (max_nb - 1));   // [nb-1][64]
   uint32_t *dpos = reinterpret_cast<uint32_t *>(mine + (size_t)64 * 12 * (max_nb - 1));  // [ndup][64]
-  const uint64_t total = blk_prefix[nitems];
+  const uint64_t total = blk_prefix[nitems] < g_end ? blk_prefix[nitems] : g_end;
   for (;;) {
     if (threadIdx.x == 0) task = atomicAdd(queue, 1u);
     __syncthreads();
-    const uint64_t g = task;
+    const uint64_t g = g_begin + task;
     __syncthreads();
     if (g >= total) break;
     const uint32_t t = find_item(blk_prefix, nitems, g);
@@ -133,8 +143,8 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
     const bool valid = r < it.row_end;
     const uint32_t *row = it.rows + r * k;
     // per-row lists: the shortest one drives, the others are probed
-    uint32_t dlen = 0, dj = 0;
-    uint64_t dbase = 0;
+    uint32_t dlen = 0, dj = 0, dp = 0;
+    uint64_t dbase = 0, daddr = 0;
     if (valid) {
       dlen = 0xFFFFFFFFu;
       for (uint32_t j = 0; j < nb; ++j) {
@@ -145,14 +155,16 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
           dlen = len;
           dj = j;
           dbase = b;
+          dp = p;
         }
       }
+      daddr = list_addr<OOC>(L, dj, dp, dbase);
       uint32_t o = 0;
       for (uint32_t j = 0; j < nb; ++j) {
         if (j == dj) continue;
         const uint32_t p = row[L.key_col[j]];
         const uint64_t b = L.toff[j][p];
-        obase[o * 64 + l] = b;
+        obase[o * 64 + l] = list_addr<OOC>(L, j, p, b);
         olen[o * 64 + l] = (uint32_t)(L.toff[j][p + 1] - b);
         ++o;
       }
@@ -165,8 +177,7 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
     const uint32_t wtotal = readlane_u32(incl, 63);
     st[w][l] = incl - dlen;
     if (l == 63) st[w][64] = wtotal;
-    drv[w][l] = dbase;
-    dlist[w][l] = dj;
+    drv[w][l] = daddr;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     uint64_t survivors = 0;
@@ -177,17 +188,12 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
       uint32_t val = 0, rw = 0;
       if (iv) {
         rw = seg_of(st[w], item);
-        const uint32_t j0 = dlist[w][rw];
-        val = L.tval[j0][drv[w][rw] + (item - st[w][rw])];
+        val = reinterpret_cast<const uint32_t *>(drv[w][rw])[item - st[w][rw]];
         keep = true;
         for (uint32_t c = 0; c < ndup && keep; ++c)
           if (dpos[c * 64 + rw] == val) keep = false;
-        uint32_t o = 0;
-        for (uint32_t j = 0; j < nb && keep; ++j) {
-          if (j == j0) continue;
-          keep = sorted_contains(L.tval[j] + obase[o * 64 + rw], olen[o * 64 + rw], val);
-          ++o;
-        }
+        for (uint32_t o = 0; o + 1 < nb && keep; ++o)
+          keep = sorted_contains(reinterpret_cast<const uint32_t *>(obase[o * 64 + rw]), olen[o * 64 + rw], val);
       }
       const uint64_t m = __ballot(keep);
       const uint32_t nk = (uint32_t)__popcll(m);
@@ -215,17 +221,18 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
 }
 
 // Last level, one backward list: |list| - equal-label columns present.
+template <bool OOC>
 __global__ void __launch_bounds__(VB) k_vcount1(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p,
                                                const uint64_t *__restrict__ blk_prefix,
-                                               unsigned int *__restrict__ queue) {
+                                               unsigned int *__restrict__ queue, uint64_t g_begin, uint64_t g_end) {
   const uint32_t nitems = *nitems_p;
   __shared__ uint32_t task;
   __shared__ uint64_t sm[VW];
-  const uint64_t total = blk_prefix[nitems];
+  const uint64_t total = blk_prefix[nitems] < g_end ? blk_prefix[nitems] : g_end;
   for (;;) {
     if (threadIdx.x == 0) task = atomicAdd(queue, 1u);
     __syncthreads();
-    const uint64_t g = task;
+    const uint64_t g = g_begin + task;
     __syncthreads();
     if (g >= total) break;
     const uint32_t t = find_item(blk_prefix, nitems, g);
@@ -238,9 +245,11 @@ __global__ void __launch_bounds__(VB) k_vcount1(const VItem *__restrict__ items,
       const uint32_t p = row[L.key_col[0]];
       const uint64_t b = L.toff[0][p], e = L.toff[0][p + 1];
       cnt = e - b;
+      const uint32_t *lst = nullptr;
+      if (L.ndup) lst = reinterpret_cast<const uint32_t *>(list_addr<OOC>(L, 0, p, b));
       for (uint32_t c = 0; c < L.ndup; ++c) {
         const uint32_t pos = vposition_in(L.bm_new, L.rank_new, L.dup_can[c][row[L.dup_col[c]]]);
-        if (pos != NOT_FOUND && sorted_contains(L.tval[0] + b, e - b, pos)) --cnt;
+        if (pos != NOT_FOUND && sorted_contains(lst, e - b, pos)) --cnt;
       }
     }
     cnt = wave_sum_u64(cnt);
@@ -251,6 +260,31 @@ __global__ void __launch_bounds__(VB) k_vcount1(const VItem *__restrict__ items,
       if (s) atomicAdd(it.counter, (unsigned long long)s);
     }
     __syncthreads();
+  }
+}
+
+// Streamed tables: for every table block a row of the launch will read, the
+// first and last task (launch task + base) that reads it.  Blocks (256-row
+// tasks) are strided over the grid.
+__global__ void __launch_bounds__(VB) k_vmark(const VItem *__restrict__ items, uint32_t nitems,
+                                             const uint64_t *__restrict__ blk_prefix, uint64_t base,
+                                             bool count_only, uint32_t *__restrict__ first,
+                                             uint32_t *__restrict__ last) {
+  const uint64_t total = blk_prefix[nitems];
+  for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
+    const uint32_t t = find_item(blk_prefix, nitems, g);
+    const VItem it = items[t];
+    const VLevel &L = *it.lv;
+    const uint64_t r = it.row_begin + (g - blk_prefix[t]) * VB + threadIdx.x;
+    // k_vcount1 reads list values only to remove equal-label columns
+    if (r >= it.row_end || (count_only && L.ndup == 0)) continue;
+    const uint32_t *row = it.rows + r * it.k;
+    const uint32_t gt = (uint32_t)(base + g);
+    for (uint32_t j = 0; j < L.nb; ++j) {
+      const uint32_t b = L.gblk[j] + (row[L.key_col[j]] >> L.kshift[j]);
+      if (first[b] > gt) atomicMin(&first[b], gt);
+      if (last[b] < gt) atomicMax(&last[b], gt);
+    }
   }
 }
 
@@ -294,6 +328,8 @@ void build_vlevels(gem_ctx *c, VQueryState &S) {
     S.max_nb[i] = lv[i].nb;
     S.max_ndup[i] = lv[i].ndup;
   }
+  // the copy above reads lv asynchronously: keep it in the query state
+  S.hlv = std::move(lv);
 }
 
 namespace {
@@ -404,23 +440,81 @@ struct ExtSlot {
     }
     const uint32_t *dn = reinterpret_cast<const uint32_t *>(d + HDR_NITEMS);
     unsigned int *dq = reinterpret_cast<unsigned int *>(d + HDR_QUEUE);
-    unsigned long long *dp = reinterpret_cast<unsigned long long *>(d + HDR_PITEMS);
-    {
+    launch_kernels<false>(c, grid, dn, dq, prof_kernels, count_items, 0, ~0ull, 0, ~0ull, 0, ~0ull);
+    GEM_HIP_CHECK(hipMemcpyAsync(h_hdr, d, hdr_bytes, hipMemcpyDeviceToHost, s));
+  }
+
+  // The three kernels over task ranges [b_i, e_i) of launch i (queues dq[0..2]).
+  template <bool OOC>
+  void launch_kernels(gem_ctx *c, uint32_t grd, const uint32_t *dn, unsigned int *dq, bool prof_kernels,
+                      bool count_items, uint64_t b0, uint64_t e0, uint64_t b1, uint64_t e1, uint64_t b2,
+                      uint64_t e2) {
+    hipStream_t s = c->stream;
+    unsigned long long *dp = reinterpret_cast<unsigned long long *>(hdr.as<unsigned char>() + HDR_PITEMS);
+    if (b0 < e0) {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_WRITE : -1);
-      k_vext<false><<<grid, VB, lds, s>>>(items[0].as<VItem>(), dn + 0, blk[0].as<uint64_t>(), dq + 0, max_nb,
-                                          max_ndup, count_items ? dp : nullptr);
+      k_vext<false, OOC><<<grd, VB, lds, s>>>(items[0].as<VItem>(), dn + 0, blk[0].as<uint64_t>(), dq + 0, max_nb,
+                                               max_ndup, count_items ? dp : nullptr, b0, e0);
       GEM_LAUNCH_CHECK("k_vext<write>");
     }
-    {
+    if (b1 < e1) {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_FINAL : -1);
-      k_vext<true><<<grid, VB, lds, s>>>(items[1].as<VItem>(), dn + 1, blk[1].as<uint64_t>(), dq + 1, max_nb,
-                                         max_ndup, count_items ? dp + 1 : nullptr);
+      k_vext<true, OOC><<<grd, VB, lds, s>>>(items[1].as<VItem>(), dn + 1, blk[1].as<uint64_t>(), dq + 1, max_nb,
+                                              max_ndup, count_items ? dp + 1 : nullptr, b1, e1);
       GEM_LAUNCH_CHECK("k_vext<final>");
     }
-    {
+    if (b2 < e2) {
       ProfLaunch pl(c, prof_kernels ? K_VCOUNT1 : -1);
-      k_vcount1<<<grid, VB, 0, s>>>(items[2].as<VItem>(), dn + 2, blk[2].as<uint64_t>(), dq + 2);
+      k_vcount1<OOC><<<grd, VB, 0, s>>>(items[2].as<VItem>(), dn + 2, blk[2].as<uint64_t>(), dq + 2, b2, e2);
       GEM_LAUNCH_CHECK("k_vcount1");
+    }
+  }
+
+  // Streamed tables: the same step, its tasks cut into groups whose table
+  // blocks are copied in while the previous group runs (eager launches only).
+  void launch_streamed(gem_ctx *c, TableCache &tc) {
+    hipStream_t s = c->stream;
+    unsigned char *d = hdr.as<unsigned char>();
+    GEM_HIP_CHECK(hipMemcpyAsync(d, h_hdr, hdr_bytes, hipMemcpyHostToDevice, s));
+    uint64_t T[3], base[3];
+    uint64_t total = 0;
+    for (int i = 0; i < 3; ++i) {
+      GEM_HIP_CHECK(hipMemcpyAsync(items[i].p, h_items[i], sizeof(VItem) * cap, hipMemcpyHostToDevice, s));
+      GEM_HIP_CHECK(hipMemcpyAsync(blk[i].p, h_blk[i], 8ull * (cap + 1), hipMemcpyHostToDevice, s));
+      T[i] = *nitems(i) ? h_blk[i][*nitems(i)] : 0;
+      base[i] = total;
+      total += T[i];
+    }
+    if (total >= 0xFFFFFFFFull) throw Error{GEM_E_LIMIT, "too many tasks in one streamed iteration"};
+    tc.reset_marks();
+    for (int i = 0; i < 3; ++i) {
+      if (!T[i]) continue;
+      k_vmark<<<(uint32_t)std::min<uint64_t>(T[i], 4096), VB, 0, s>>>(
+          items[i].as<VItem>(), *nitems(i), blk[i].as<uint64_t>(), base[i], i == 2, tc.d_first(), tc.d_last());
+      GEM_LAUNCH_CHECK("k_vmark");
+    }
+    tc.plan(total);
+    const auto &G = tc.groups();
+    unsigned int *q = tc.queues();
+    const uint32_t *dn = reinterpret_cast<const uint32_t *>(d + HDR_NITEMS);
+    const bool prof = c->prof.on;
+    for (size_t g = 0; g < G.size(); ++g) {
+      tc.issue(g);
+      uint64_t lo[3], hi[3];
+      for (int i = 0; i < 3; ++i) {
+        lo[i] = std::max(G[g].a, base[i]);
+        hi[i] = std::min(G[g].b, base[i] + T[i]);
+        if (lo[i] < hi[i]) {
+          lo[i] -= base[i];
+          hi[i] -= base[i];
+        } else {
+          lo[i] = hi[i] = 0;
+        }
+      }
+      const uint64_t most = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+      const uint32_t grd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(grid, most));
+      launch_kernels<true>(c, grd, dn, q + 3 * g, prof, prof, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+      tc.done(g);
     }
     GEM_HIP_CHECK(hipMemcpyAsync(h_hdr, d, hdr_bytes, hipMemcpyDeviceToHost, s));
   }
@@ -467,6 +561,13 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
         mnd = std::max(mnd, S->max_ndup[i]);
       }
     slot.init(c_, (uint32_t)qs.size(), mnb, mnd);
+  }
+  // streamed tables: a block cache shared by the run's queries
+  std::unique_ptr<TableCache> tc;
+  if (streamed_) {
+    if (!c_->stream_cache_bytes) throw Error{GEM_E_STATE, "streamed join without a table cache size"};
+    tc = std::make_unique<TableCache>(c_, c_->stream_cache_bytes, c_->stream_slot_bytes);
+    tc->attach(qs);
   }
   // seed each query with the identity level of its first vertex
   for (VQueryState *S : qs) {
@@ -593,10 +694,12 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
         }
         it.row_begin = (uint64_t)top.cursor * VB;
         it.row_end = std::min<uint64_t>(b1 * VB, top.nrows);
+        if (top.measured) S->work += top.block_u[b1] - top.block_u[top.cursor];
         top.cursor = b1;
         it.counter = reinterpret_cast<unsigned long long *>((uintptr_t)nctr);  // patched below
         ft.push_back(FTask{S, nctr++});
         S->processed += it.row_end - it.row_begin;
+        S->work += it.row_end - it.row_begin;
         if (S->max_nb[top.level] == 1)
           fin1.add(it, 1, S->max_ndup[top.level]);
         else
@@ -613,6 +716,7 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
       it.row_begin = b0 * VB;
       it.row_end = std::min<uint64_t>(b1 * VB, top.nrows);
       S->processed += it.row_end - it.row_begin;
+      S->work += it.row_end - it.row_begin + ub;
       top.cursor = b1;
       if (ub == 0) {
         if (top.cursor >= nblocks) S->stack.pop_back();
@@ -646,8 +750,12 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
     }
     std::memset(slot.h_hdr + ExtSlot::HDR_QUEUE, 0, slot.hdr_bytes - ExtSlot::HDR_QUEUE);
     const bool prof = c_->prof.on;
-    slot.launch(c_, c_->use_graph && !prof);
+    if (tc)
+      slot.launch_streamed(c_, *tc);
+    else
+      slot.launch(c_, c_->use_graph && !prof);
     GEM_HIP_CHECK(hipStreamSynchronize(s));
+    if (tc) tc->finish();
     std::vector<uint64_t> hc(nctr + 2, 0);
     for (size_t i = 0; i < nctr; ++i) hc[i] = slot.counters()[i];
     hc[nctr] = slot.pitems()[0];
@@ -683,7 +791,7 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
         S->saturated = true;
         S->done = true;
         S->stack.clear();
-      } else if (work_limit_ && S->processed > work_limit_) {
+      } else if (work_limit_ && S->work > work_limit_) {
         S->work_limited = true;
         S->done = true;
         S->stack.clear();

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <thread>
@@ -68,6 +69,28 @@ void parallel_chunks(uint64_t n, F f, unsigned threads = 0) {
   for (auto &th : ts) th.join();
 }
 
+// Run f(begin, end) over [0, n) in chunks of `grain` handed out from a shared
+// counter (for skewed per-item work, e.g. the hub lists of R-MAT graphs).
+template <class F>
+void parallel_dynamic(uint64_t n, uint64_t grain, F f, unsigned threads = 0) {
+  if (threads == 0) threads = hw_threads();
+  if (n <= grain || threads == 1) {
+    f((uint64_t)0, n);
+    return;
+  }
+  std::atomic<uint64_t> next{0};
+  std::vector<std::thread> ts;
+  for (unsigned t = 0; t < threads; ++t)
+    ts.emplace_back([&]() {
+      for (;;) {
+        const uint64_t b = next.fetch_add(grain);
+        if (b >= n) break;
+        f(b, std::min<uint64_t>(n, b + grain));
+      }
+    });
+  for (auto &th : ts) th.join();
+}
+
 // Build a symmetric, sorted, duplicate-free CSR from a generator of directed
 // pairs.  `emit(i, &s, &d)` returns false to drop sample i; each kept sample
 // contributes s->d and d->s.  The generator is called twice per sample (count
@@ -100,7 +123,7 @@ void build_symmetric_csr(uint32_t n, uint64_t num_samples, Emit emit, gem_graph 
   cur.clear();
   cur.shrink_to_fit();
   // sort + dedup each list (deterministic whatever order the scatter produced)
-  parallel_chunks(n, [&](uint64_t b, uint64_t e) {
+  parallel_dynamic(n, 1024, [&](uint64_t b, uint64_t e) {
     for (uint64_t v = b; v < e; ++v) {
       uint32_t *p = adj.data() + off[v];
       uint32_t len = deg[v];

@@ -8,7 +8,8 @@ import weakref
 
 import numpy as np
 
-from ._lib import GEM_MAX_QV, BatchOpts, GemError, JoinResult, KernelStat, PhaseTimes, QueryView, StepLog, load
+from ._lib import (GEM_MAX_QV, BatchOpts, GemError, JoinResult, KernelStat, PhaseTimes, QueryView, StepLog,
+                   StreamStats, load)
 
 NOT_FOUND = 0xFFFFFFFF
 
@@ -149,6 +150,18 @@ class GemContext:
     def set_graph(self, on=True):
         """Replay the batched path's extend step as a captured hipGraph."""
         self._check(self._lib.gem_hip_set_graph(self._h, 1 if on else 0), "gem_hip_set_graph")
+
+    def set_table_stream(self, cache_bytes, slot_bytes=0):
+        """Batched paths: keep edge-table values in pinned host memory and
+        stream them through a device cache of `cache_bytes` (0 = in HBM)."""
+        self._check(self._lib.gem_hip_set_table_stream(self._h, int(cache_bytes), int(slot_bytes)),
+                    "gem_hip_set_table_stream")
+
+    def stream_stats(self):
+        """Streamed-table statistics since the last call (then reset)."""
+        st = StreamStats()
+        self._check(self._lib.gem_hip_stream_stats(self._h, C.byref(st)), "gem_hip_stream_stats")
+        return {k: getattr(st, k) for k, _ in StreamStats._fields_}
 
     def set_work_limit(self, rows):
         """Stop a join after `rows` processed rows (0 = no limit)."""

@@ -9,6 +9,10 @@ the parity tests.
   rmat22 R-MAT 2^22 vertices, average degree 16, 32 Zipf labels; 512
          random-walk queries with sizes uniform in [4, 32], mixed density.
   rmat20_learn  the rmat20 graph with 256 queries of 8-24 vertices.
+  ooc24  R-MAT 2^24 vertices, average degree 64, 64 Zipf labels; 32 sparse
+         (average degree < 3) random-walk 12-vertex queries: the out-of-core
+         shape (streamed edge tables) at a size one box generates in about a
+         minute (2^28 vertices take over ten).
 """
 import numpy as np
 
@@ -19,6 +23,7 @@ WORKLOADS = {
     "rmat20": dict(kind="rmat", scale=20, degree=16.0, labels=16, seed=1002),
     "rmat22": dict(kind="rmat", scale=22, degree=16.0, labels=32, seed=1003),
     "rmat20_learn": dict(kind="rmat", scale=20, degree=16.0, labels=16, seed=1002),
+    "ooc24": dict(kind="rmat", scale=24, degree=64.0, labels=64, seed=1005),
 }
 
 CAP = 10 ** 9  # enumeration cap for the R-MAT workloads
@@ -57,6 +62,15 @@ def make_queries(name, graph, count=None):
         # interleave so any prefix is mixed
         for d, s in zip(dense, sparse):
             qs += [d, s]
+        return qs
+    if name == "ooc24":
+        n = count or 32
+        i = 0
+        while len(qs) < n:
+            q = Query.random_walk(graph, 12, seed * 1000 + i)
+            i += 1
+            if q.num_e >= 2 and not q.dense:
+                qs.append(q)
         return qs
     lo, hi, n = (4, 32, count or 512) if name == "rmat22" else (8, 24, count or 256)
     rng = np.random.default_rng(seed)

@@ -242,7 +242,8 @@ int gem_hip_kernel_stats(gem_ctx *ctx, gem_kernel_stat *out, uint32_t max, uint3
  * Counts equal the plain path's (the count does not depend on the order). */
 typedef struct gem_batch_opts {
   uint64_t cap;          /* per-query count cap (0 = none)                  */
-  uint64_t work_limit;   /* rows a query may process (0 = none)             */
+  uint64_t work_limit;   /* work a query may do before it stops unfinished:
+                            rows read plus list entries walked (0 = none)  */
   uint64_t level_budget; /* bytes for pending level chunks, shared by the
                             group's running queries (0 = 16 GiB)           */
   uint32_t group;        /* queries resident on the GPU at once (0 = all)   */
@@ -267,6 +268,35 @@ int gem_hip_set_persistent_blocks(gem_ctx *ctx, uint32_t blocks);
  * and replayed; off (default): launched eagerly.  Kernel profiling forces the
  * eager form. */
 int gem_hip_set_graph(gem_ctx *ctx, int on);
+
+/* Out-of-core edge tables for the batched paths (replaces the reference's
+ * out-of-core MemoryManager / LRUEvictPolicy, out-of-core/lib/structure/src/
+ * memory_manager.cu:196-822 and policy.cpp:106-172, with its join hooks at
+ * out-of-core/lib/join/src/join_trie.cu:1088-1108, which swap intermediate
+ * columns; here the level budget already bounds intermediates, and what is
+ * streamed is the edge tables).  cache_bytes != 0: after a query's tables are
+ * built, their values move to pinned host memory in key-range blocks of at
+ * most slot_bytes (0 = 1 MiB); each join iteration marks the blocks its rows
+ * will read, and groups of its tasks run while the blocks of the next group
+ * are copied host -> device on a second stream into a device cache of
+ * cache_bytes.  Counts are unchanged.  cache_bytes == 0: tables stay in HBM
+ * (default).  A list longer than a slot, or a 256-row task whose blocks do
+ * not fit the cache, fails with GEM_E_LIMIT.  The single-query entry points
+ * (gem_hip_join, gem_hip_vjoin, gem_hip_match) always keep tables in HBM. */
+int gem_hip_set_table_stream(gem_ctx *ctx, uint64_t cache_bytes, uint64_t slot_bytes);
+
+typedef struct gem_stream_stats {
+  uint64_t table_bytes;    /* table values moved to host memory             */
+  uint64_t h2d_bytes;      /* block bytes copied host -> device              */
+  uint64_t blocks_loaded;  /* block copies                                   */
+  uint64_t blocks_hit;     /* needed blocks already resident                 */
+  uint64_t groups;         /* task groups launched                           */
+  uint64_t serial_groups;  /* groups whose copies could not overlap compute  */
+  double copy_ms;          /* time of the copy stream's work (events)        */
+  double step_ms;          /* time from first copy to last kernel (events)   */
+} gem_stream_stats;
+/* Statistics accumulated since the last call (then reset). */
+int gem_hip_stream_stats(gem_ctx *ctx, gem_stream_stats *out);
 
 /* Vertex-at-a-time join of the current query (after gem_hip_filter and
  * gem_hip_build_edge_tables) along vertex_order[num_v]. */
