@@ -32,7 +32,7 @@ class CsrView(C.Structure):
 class JoinResult(C.Structure):
     _fields_ = [("count", C.c_uint64), ("saturated", C.c_uint32), ("num_steps", C.c_uint32),
                 ("num_embeddings_written", C.c_uint64), ("rows_processed", C.c_uint64),
-                ("work_limited", C.c_uint32), ("reserved", C.c_uint32)]
+                ("work_limited", C.c_uint32), ("reserved", C.c_uint32), ("work", C.c_uint64)]
 
 
 class StepLog(C.Structure):
@@ -106,6 +106,7 @@ SIGNATURES = [
     ("gem_mlp_num_params", C.c_uint64, [C.c_void_p]),
     ("gem_mlp_get_params", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     ("gem_mlp_set_params", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    ("gem_mlp_set_margin", C.c_int, [C.c_void_p, C.c_float]),
     ("gem_mlp_predict", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_uint32, C.POINTER(C.c_float)]),
     ("gem_mlp_loss_grad", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_uint32,
                                     C.POINTER(C.c_float), C.POINTER(C.c_float)]),

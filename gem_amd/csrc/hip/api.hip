@@ -234,6 +234,7 @@ int gem_hip_join(gem_ctx *c, const uint32_t *edge_order, uint64_t cap, gem_join_
     out->rows_processed = jo.rows_processed;
     out->work_limited = jo.work_limited ? 1 : 0;
     out->reserved = 0;
+    out->work = jo.rows_processed;
   });
 }
 
@@ -326,6 +327,7 @@ int gem_hip_match(gem_ctx *c, const gem_query_view *q, uint64_t cap, gem_join_re
     out->rows_processed = jo.rows_processed;
     out->work_limited = jo.work_limited ? 1 : 0;
     out->reserved = 0;
+    out->work = jo.rows_processed;
   });
 }
 

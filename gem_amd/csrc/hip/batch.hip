@@ -62,6 +62,7 @@ static void fill_result(const VQueryState &S, uint64_t cap, gem_join_result *r, 
   r->rows_processed = S.processed;
   r->work_limited = S.work_limited ? 1 : 0;
   r->reserved = 0;
+  r->work = S.work;
   if (level_rows)
     for (uint32_t i = 0; i < GEM_MAX_QV; ++i) level_rows[i] = i < S.level_rows.size() ? S.level_rows[i] : 0;
 }
@@ -156,14 +157,20 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
       std::vector<float> pred(owner.size());
       const int rc = gem_mlp_predict(mlp, feats.data(), (uint32_t)owner.size(), pred.data());
       if (rc) throw Error{rc, "order model: " + c->err};
-      std::vector<float> best(gn, 0.f);
+      // the cheapest prediction wins if it beats GEM's order (candidate 0)
+      // by more than the model's margin; otherwise GEM's order runs
+      const float margin = mlp_margin(mlp);
       size_t k = 0;
-      for (uint32_t i = 0; i < gn; ++i)
+      for (uint32_t i = 0; i < gn; ++i) {
+        const float gem = pred[k];
+        float best = gem;
         for (uint32_t j = 0; j < cand[i].size(); ++j, ++k)
-          if (j == 0 || pred[k] < best[i]) {
-            best[i] = pred[k];
+          if (pred[k] < best) {
+            best = pred[k];
             pick[i] = j;
           }
+        if (!(best < gem - margin)) pick[i] = 0;
+      }
     }
     for (uint32_t i = 0; i < gn; ++i) {
       plan_query(c, st[i], qd[i], cand[i][pick[i]].data());

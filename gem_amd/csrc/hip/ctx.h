@@ -252,6 +252,7 @@ struct gem_ctx {
 
 namespace gemd {
 void mlp_detach(gem_mlp *m);  // frees m's device buffers, sets m->ctx = nullptr
+float mlp_margin(const gem_mlp *m);
 }
 
 namespace gemd {

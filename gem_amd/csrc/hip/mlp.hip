@@ -163,6 +163,7 @@ struct gem_mlp {
   size_t oW1, ob1, oW2, ob2, oW3, ob3;
   DevBuf params, grads, adam_m, adam_v, wbf;  // wbf: bf16 W1 | W2 | W3
   uint64_t adam_t = 0;
+  float margin = 0.f;  // a candidate replaces GEM's order only if predicted cheaper by more
 };
 
 namespace {
@@ -270,6 +271,8 @@ int mguard(gem_mlp *m, F &&f) {
 }  // namespace
 
 namespace gemd {
+float mlp_margin(const gem_mlp *m) { return m->margin; }
+
 void mlp_detach(gem_mlp *m) {
   m->params.reset();
   m->grads.reset();
@@ -338,6 +341,12 @@ void gem_mlp_destroy(gem_mlp *m) {
 }
 
 uint64_t gem_mlp_num_params(const gem_mlp *m) { return m ? m->P : 0; }
+
+int gem_mlp_set_margin(gem_mlp *m, float margin) {
+  if (!m || !(margin >= 0.f)) return GEM_E_INVALID;
+  m->margin = margin;
+  return GEM_OK;
+}
 
 int gem_mlp_get_params(gem_mlp *m, float *out) {
   return mguard(m, [&]() {

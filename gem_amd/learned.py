@@ -5,9 +5,11 @@ Training data are (order features, cost) pairs the matcher logs itself: for
 each training query the library proposes candidate vertex orders
 (gem_hip_order_candidates: GEM's order first, then estimate-greedy and seeded
 random orders), the vertex-at-a-time matcher runs each one, and the cost
-label is log2(1 + rows processed).  The network is fit by full-batch Adam on
-the GPU; at match time the batched path runs the candidate with the lowest
-predicted cost (GEM's order when no model is given).
+label is log2(1 + work), work = rows read plus list entries walked.  The
+network is fit by full-batch Adam on the GPU; at match time the batched path
+runs the candidate with the lowest predicted cost when it beats GEM's order by
+more than the model's margin, else GEM's order (also the order when no model
+is given).
 """
 import ctypes as C
 
@@ -54,7 +56,15 @@ class OrderModel:
         m = cls(ctx, seed=0, in_dim=int(z["in_dim"]), hidden=int(z["hidden"]))
         m.set_params(z["params"])
         m.meta = {k: z[k].item() if z[k].shape == () else z[k] for k in z.files if k != "params"}
+        m.set_margin(float(m.meta.get("margin", 0.0)))
         return m
+
+    def set_margin(self, log2_units):
+        """GEM's order is replaced only by a candidate predicted cheaper by
+        more than `log2_units` (log2 of the work; 1.0 = half the work)."""
+        sd = float(getattr(self, "meta", {}).get("sd", 1.0))
+        self.margin = float(log2_units)
+        self.ctx._check(self._lib.gem_mlp_set_margin(self._h, self.margin / sd), "gem_mlp_set_margin")
 
     def get_params(self):
         out = np.zeros(self.num_params, np.float32)
@@ -104,8 +114,10 @@ def order_candidates(ctx, query, max_orders=16, seed=0):
     return orders[:k * query.num_v].reshape(k, query.num_v), feats[:k * GEM_FEAT_DIM].reshape(k, GEM_FEAT_DIM)
 
 
-def cost_label(rows_processed):
-    return float(np.log2(1.0 + float(rows_processed)))
+def cost_label(work):
+    """Training label of one (query, order) run: log2(1 + work), work = rows
+    read plus list entries walked by the vertex-at-a-time join."""
+    return float(np.log2(1.0 + float(work)))
 
 
 def collect(ctx, queries, max_orders=16, cap=0, work_limit=0, seed=0, log=None):
@@ -123,7 +135,7 @@ def collect(ctx, queries, max_orders=16, cap=0, work_limit=0, seed=0, log=None):
         for ci in range(len(orders)):
             r = ctx.vjoin(orders[ci], cap=cap)
             F.append(feats[ci])
-            Y.append(cost_label(r["rows_processed"]))
+            Y.append(cost_label(r["work"]))
             QI.append(qi)
             CI.append(ci)
             CNT.append(-1 if r["work_limited"] else r["count"])
@@ -136,14 +148,18 @@ def collect(ctx, queries, max_orders=16, cap=0, work_limit=0, seed=0, log=None):
             np.array(CI), np.array(CNT, np.int64))
 
 
-def choose(model, feats, qidx):
-    """Per query, the candidate with the lowest predicted cost."""
+def choose(model, feats, qidx, margin=0.0):
+    """Per query, the candidate with the lowest predicted cost, if it beats the
+    query's first candidate (GEM's order) by more than `margin` (model output
+    units); else the first candidate."""
     pred = model.predict(feats)
-    best = {}
+    first, best = {}, {}
     for i, (q, p) in enumerate(zip(qidx, pred)):
+        if q not in first:
+            first[q] = (i, p)
         if q not in best or p < best[q][1]:
             best[q] = (i, p)
-    return {q: i for q, (i, p) in best.items()}
+    return {q: (best[q][0] if best[q][1] < first[q][1] - margin else first[q][0]) for q in best}
 
 
 def match_batch_learned(ctx, queries, model=None, max_orders=16, cap=0, work_limit=0, level_budget=0, group=0):
@@ -160,7 +176,7 @@ def match_batch_learned(ctx, queries, model=None, max_orders=16, cap=0, work_lim
                "gem_hip_match_batch_learned")
     return [{"count": int(res[i].count), "saturated": bool(res[i].saturated),
              "work_limited": bool(res[i].work_limited), "rows_processed": int(res[i].rows_processed),
-             "chosen": int(chosen[i])} for i in range(nq)]
+             "work": int(res[i].work), "chosen": int(chosen[i])} for i in range(nq)]
 
 
 __all__ = ["OrderModel", "order_candidates", "collect", "choose", "match_batch_learned", "cost_label", "GemError"]

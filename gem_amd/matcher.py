@@ -176,7 +176,7 @@ class GemContext:
         v = query.view()
         self._check(self._lib.gem_hip_match(self._h, C.byref(v), cap, C.byref(res), _p32(order)), "gem_hip_match")
         return {"count": int(res.count), "saturated": bool(res.saturated), "work_limited": bool(res.work_limited),
-                "rows_processed": int(res.rows_processed), "order": order}
+                "rows_processed": int(res.rows_processed), "work": int(res.work), "order": order}
 
     def vjoin(self, vertex_order=None, cap=0):
         """Vertex-at-a-time join of the current query (after filter and
@@ -187,7 +187,8 @@ class GemContext:
         self._check(self._lib.gem_hip_vjoin(self._h, _p32(vo) if vo is not None else None, cap, C.byref(res),
                                             _p64(lr)), "gem_hip_vjoin")
         return {"count": int(res.count), "saturated": bool(res.saturated), "work_limited": bool(res.work_limited),
-                "rows_processed": int(res.rows_processed), "level_rows": lr[:self.query.num_v].copy()}
+                "rows_processed": int(res.rows_processed), "work": int(res.work),
+                "level_rows": lr[:self.query.num_v].copy()}
 
     def match_batch(self, queries, vertex_orders=None, cap=0, work_limit=0, level_budget=0, group=0):
         """Batched vertex-at-a-time path over a list of queries.  Returns a
@@ -214,7 +215,7 @@ class GemContext:
         for i, q in enumerate(queries):
             r = res[i]
             out.append({"count": int(r.count), "saturated": bool(r.saturated), "work_limited": bool(r.work_limited),
-                        "rows_processed": int(r.rows_processed),
+                        "rows_processed": int(r.rows_processed), "work": int(r.work),
                         "level_rows": lr[i * GEM_MAX_QV:i * GEM_MAX_QV + q.num_v].copy()})
         return out
 

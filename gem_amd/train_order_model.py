@@ -5,11 +5,11 @@
 For every training query the library proposes candidate vertex orders (GEM's
 order first) with their int32 features; every (query, order) pair is run by
 the batched vertex-at-a-time matcher (many pairs per launch), which logs the
-rows each level produced and the rows it processed; the cost label is
-log2(1 + rows processed).  The MLP is then fit on the GPU (full-batch Adam,
-the library's own forward/backward kernels) on a random 80% of the queries and
-evaluated on the other 20%: total rows processed by the model's choice vs
-GEM's order.  The weights, the label normalisation and the evaluation are
+rows each level produced and the work it did (rows read plus list entries
+walked); the cost label is log2(1 + work).  The MLP is then fit on the GPU
+(full-batch Adam, the library's own forward/backward kernels) on a random 80%
+of the queries and evaluated on the other 20%: total work of the model's
+choice vs GEM's order.  The weights, the label normalisation and the evaluation are
 saved to an .npz file.
 """
 import argparse
@@ -51,23 +51,30 @@ def collect_batched(ctx, queries, max_orders, cap, work_limit, group, seed=0):
         res = ctx.match_batch([e[0] for e in part], vertex_orders=[e[1] for e in part], cap=cap,
                               work_limit=work_limit, group=group)
         for r in res:
-            costs.append(cost_label(r["rows_processed"]))
+            costs.append(cost_label(r["work"]))
             counts.append(-1 if r["work_limited"] else r["count"])
         log(f"[train]   ran {min(b + group, len(entries))}/{len(entries)} pairs, {time.perf_counter() - t:.1f}s")
     return (np.array(feats, np.int32).reshape(-1, GEM_FEAT_DIM), np.array(costs, np.float32), np.array(qidx),
             np.array(cidx), np.array(counts, np.int64))
 
 
-def evaluate(model, F, Y, Q, C, mu, sd, queries_sel):
+def evaluate(model, F, Y, Q, C, mu, sd, queries_sel, margin=0.0):
+    """Total work (2^label) over the selected queries of GEM's order, of the
+    model's choice (GEM's order unless a candidate is predicted cheaper by more
+    than `margin` log2 units) and of the best candidate."""
     pred = model.predict(F) * sd + mu
     gem = learned = best = 0.0
     for q in queries_sel:
         m = Q == q
         idx = np.flatnonzero(m)
-        gem += 2.0 ** Y[idx[C[idx] == 0][0]]
-        learned += 2.0 ** Y[idx[np.argmin(pred[idx])]]
+        g0 = idx[C[idx] == 0][0]
+        pick = idx[np.argmin(pred[idx])]
+        if not pred[pick] < pred[g0] - margin:
+            pick = g0
+        gem += 2.0 ** Y[g0]
+        learned += 2.0 ** Y[pick]
         best += 2.0 ** Y[idx].min()
-    return {"gem_rows": float(gem), "learned_rows": float(learned), "best_rows": float(best),
+    return {"gem_work": float(gem), "learned_work": float(learned), "best_work": float(best),
             "gain_vs_gem": float(gem / learned) if learned else 0.0}
 
 
@@ -85,6 +92,8 @@ def main():
     ap.add_argument("--iters", type=int, default=600)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--margin", type=float, default=1.0,
+                    help="log2 units by which a candidate must beat GEM's predicted cost to replace it")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "models", "order_mlp_rmat20.npz"))
     args = ap.parse_args()
 
@@ -113,14 +122,15 @@ def main():
     losses = model.train(F[tr], (Y[tr] - mu) / sd, iters=args.iters, lr=args.lr)
     log(f"[train] {args.iters} Adam steps on {int(tr.sum())} tuples in {time.perf_counter() - t:.2f}s, "
         f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
-    ev = evaluate(model, F, Y, Q, C, mu, sd, te_q)
-    ev_train = evaluate(model, F, Y, Q, C, mu, sd, sorted(tr_q))
+    ev = evaluate(model, F, Y, Q, C, mu, sd, te_q, args.margin)
+    ev_train = evaluate(model, F, Y, Q, C, mu, sd, sorted(tr_q), args.margin)
+    ev["no_margin"] = evaluate(model, F, Y, Q, C, mu, sd, te_q, 0.0)
     log(f"[train] held-out: {json.dumps(ev)}")
     # counts agree across all finished orders of each query
     consistent = all(len(set(CNT[(Q == q) & (CNT >= 0)].tolist())) <= 1 for q in range(len(qs)))
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.savez(args.out, params=model.get_params(), mu=mu, sd=sd, in_dim=GEM_FEAT_DIM, hidden=model.hidden,
-             workload=args.workload, tuples=len(Y), queries=len(qs), losses=losses,
+             workload=args.workload, tuples=len(Y), queries=len(qs), losses=losses, margin=args.margin,
              heldout=json.dumps(ev), train=json.dumps(ev_train), counts_consistent=consistent)
     print(json.dumps({"tuples": int(len(Y)), "queries": len(qs), "heldout": ev, "train": ev_train,
                       "loss": [float(losses[0]), float(losses[-1])], "counts_consistent": bool(consistent),

@@ -9,6 +9,8 @@ the parity tests.
   rmat22 R-MAT 2^22 vertices, average degree 16, 32 Zipf labels; 512
          random-walk queries with sizes uniform in [4, 32], mixed density.
   rmat20_learn  the rmat20 graph with 256 queries of 8-24 vertices.
+  rmat22_learn  the rmat22 graph with 256 other queries of 4-32 vertices
+         (training set of the order model used on rmat22).
   ooc24  R-MAT 2^24 vertices, average degree 64, 64 Zipf labels; 32 sparse
          (average degree < 3) random-walk 12-vertex queries: the out-of-core
          shape (streamed edge tables) at a size one box generates in about a
@@ -23,6 +25,7 @@ WORKLOADS = {
     "rmat20": dict(kind="rmat", scale=20, degree=16.0, labels=16, seed=1002),
     "rmat22": dict(kind="rmat", scale=22, degree=16.0, labels=32, seed=1003),
     "rmat20_learn": dict(kind="rmat", scale=20, degree=16.0, labels=16, seed=1002),
+    "rmat22_learn": dict(kind="rmat", scale=22, degree=16.0, labels=32, seed=1003, qseed=2003),
     "ooc24": dict(kind="rmat", scale=24, degree=64.0, labels=64, seed=1005),
 }
 
@@ -37,7 +40,7 @@ def make_graph(name):
 
 
 def make_queries(name, graph, count=None):
-    seed = WORKLOADS[name]["seed"]
+    seed = WORKLOADS[name].get("qseed", WORKLOADS[name]["seed"])
     qs = []
     if name == "er10k":
         n = count or 20
@@ -72,7 +75,8 @@ def make_queries(name, graph, count=None):
             if q.num_e >= 2 and not q.dense:
                 qs.append(q)
         return qs
-    lo, hi, n = (4, 32, count or 512) if name == "rmat22" else (8, 24, count or 256)
+    lo, hi, n = {"rmat22": (4, 32, count or 512), "rmat22_learn": (4, 32, count or 256)}.get(
+        name, (8, 24, count or 256))
     rng = np.random.default_rng(seed)
     sizes = rng.integers(lo, hi + 1, size=n)
     i = 0

@@ -181,6 +181,9 @@ typedef struct gem_join_result {
   uint64_t rows_processed; /* rows read by the join's steps              */
   uint32_t work_limited;   /* 1: stopped at the work limit, count unknown */
   uint32_t reserved;
+  uint64_t work;           /* what the work limit bounds: rows read (plain
+                              path); rows read + list entries walked, upper
+                              bound (batched path)                         */
 } gem_join_result;
 
 /* Stop a join once it has processed more than `rows` rows (0 = no limit);
@@ -322,6 +325,11 @@ uint64_t gem_mlp_num_params(const gem_mlp *m);
 /* flat fp32 parameters: W1[in*h], b1[h], W2[h*h], b2[h], W3[h], b3[1] */
 int gem_mlp_get_params(gem_mlp *m, float *out);
 int gem_mlp_set_params(gem_mlp *m, const float *in);
+/* The batched learned path runs the cheapest-predicted candidate only if its
+ * prediction is below GEM's order's by more than `margin` (in the model's
+ * output units; default 0): GEM's order stays the fallback when the model is
+ * not confident. */
+int gem_mlp_set_margin(gem_mlp *m, float margin);
 int gem_mlp_predict(gem_mlp *m, const int32_t *feats, uint32_t n, float *out);
 /* loss = mean((y - t)^2) / 2 over the batch; grad[num_params] (nullable) */
 int gem_mlp_loss_grad(gem_mlp *m, const int32_t *feats, const float *targets, uint32_t n, float *loss,
