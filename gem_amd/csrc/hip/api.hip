@@ -87,12 +87,6 @@ int gem_hip_set_persistent_blocks(gem_ctx *c, uint32_t blocks) {
   return GEM_OK;
 }
 
-int gem_hip_set_join_chunk(gem_ctx *c, uint32_t entries) {
-  if (!c || entries < 64 || entries > (1u << 24) || entries % 64) return GEM_E_INVALID;
-  c->join_chunk = entries;
-  return GEM_OK;
-}
-
 int gem_hip_set_graph(gem_ctx *c, int on) {
   if (!c) return GEM_E_INVALID;
   c->use_graph = on != 0;

@@ -234,7 +234,6 @@ struct gem_ctx {
   uint64_t level_budget_bytes = 1ull << 31;  // per join level buffer
   uint64_t work_limit_rows = 0;             // 0 = none
   uint32_t persistent_blocks = 2048;        // grid of the batched path's persistent kernels
-  uint32_t join_chunk = 4096;               // list entries walked per extend task
   bool use_graph = false;                   // replay the batched extend step as a hipGraph
   // batched path with streamed edge tables: values in pinned host memory,
   // a device cache of stream_cache_bytes in slots of stream_slot_bytes

@@ -35,11 +35,6 @@ struct VItem {
   uint32_t *out;
   uint64_t out_cap;
   unsigned long long *counter;
-  // walked-entry tasks (extend launches): gpre[g] = entries walked by the
-  // level's 64-row groups before group g; this item is groups [g0, g1),
-  // entries [gpre[g0], gpre[g1])
-  const uint64_t *gpre;
-  uint64_t g0, g1;
 };
 
 struct VStackItem {
@@ -47,11 +42,9 @@ struct VStackItem {
   std::shared_ptr<DevBuf> rows;
   uint64_t nrows;
   uint32_t k;                      // columns per row
-  std::vector<uint64_t> block_u;   // prefix over 64-row groups of list entries to walk
-  uint64_t cursor;                 // next group to process
+  std::vector<uint64_t> block_u;   // prefix of per-block output bounds
+  uint64_t cursor;                 // next block to process
   bool measured;
-  std::shared_ptr<DevBuf> gpre_buf;  // device copy of the same prefix (shared buffer)
-  const uint64_t *gpre = nullptr;
 };
 
 struct VQueryState {

@@ -62,24 +62,22 @@ __device__ __forceinline__ uint32_t find_item(const uint64_t *blk_prefix, uint32
 }
 
 // ------------------------------------------------------------ length pass
-// Per 64-row group (one wave): the sum over its rows of the shortest backward
-// list, i.e. the list entries the extend pass will walk.  group_u is indexed
-// by 256-row task * 4 + wave; groups past an item's last row hold 0.
 __global__ void __launch_bounds__(VB) k_vlen(const VItem *__restrict__ items, uint32_t nitems,
                                             const uint64_t *__restrict__ blk_prefix,
-                                            unsigned int *__restrict__ queue, uint64_t *__restrict__ group_u) {
-  const uint32_t l = threadIdx.x % 64;
+                                            unsigned int *__restrict__ queue, uint64_t *__restrict__ block_u) {
+  __shared__ uint64_t sm[VW];
+  __shared__ uint32_t task;
   const uint64_t total = blk_prefix[nitems];
   for (;;) {
-    uint32_t wt = 0;
-    if (l == 0) wt = atomicAdd(queue, 1u);
-    wt = readlane_u32(wt, 0);
-    if (wt >= total * VW) break;
-    const uint64_t g = wt / VW;
+    if (threadIdx.x == 0) task = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint64_t g = task;
+    __syncthreads();
+    if (g >= total) break;
     const uint32_t t = find_item(blk_prefix, nitems, g);
     const VItem it = items[t];
     const VLevel &L = *it.lv;
-    const uint64_t r = it.row_begin + (g - blk_prefix[t]) * VB + (wt % VW) * 64 + l;
+    const uint64_t r = it.row_begin + (g - blk_prefix[t]) * VB + threadIdx.x;
     uint64_t u = 0;
     if (r < it.row_end) {
       const uint32_t *row = it.rows + r * it.k;
@@ -91,7 +89,10 @@ __global__ void __launch_bounds__(VB) k_vlen(const VItem *__restrict__ items, ui
       }
     }
     u = wave_sum_u64(u);
-    if (l == 0) group_u[wt] = u;
+    if (threadIdx.x % 64 == 0) sm[threadIdx.x / 64] = u;
+    __syncthreads();
+    if (threadIdx.x == 0) block_u[g] = sm[0] + sm[1] + sm[2] + sm[3];
+    __syncthreads();
   }
 }
 
@@ -103,46 +104,22 @@ __device__ __forceinline__ uint64_t list_addr(const VLevel &L, uint32_t j, uint3
   return (uint64_t)(uintptr_t)(L.tval[j] + b);
 }
 
-// largest g in [lo, hi) with pre[g] <= x (pre ascending, pre[lo] <= x)
-__device__ __forceinline__ uint64_t group_of(const uint64_t *pre, uint64_t lo, uint64_t hi, uint64_t x) {
-  while (hi - lo > 1) {
-    const uint64_t mid = (lo + hi) >> 1;
-    if (pre[mid] <= x)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 // ------------------------------------------------------ extend (walk) pass
-// Tasks are ranges of `chunk` list entries to walk, not ranges of rows: the
-// length pass gives, per 64-row group, the entries its rows' driving lists
-// hold, so a hub row whose list has 10^5 entries is spread over many waves
-// and a task of light rows covers many groups.  Each wave pulls tasks from the
-// launch's queue on its own (no block barrier).  For every 64-row group the
-// task overlaps, the wave sets up the rows' lists (the shortest drives, the
-// others are binary-searched, equal-label columns are compared) and walks the
-// task's share of the group's entries, 64 per step.
 // Dynamic LDS per wave: other lists (nb-1) x 64 x (u64 address + u32 len),
 // dup positions ndup x 64 u32.
-// Survivors are written as whole rows: their (parent row, new value) pairs are
-// staged in LDS, then each half-wave copies one row (k + 1 consecutive words),
-// so a store instruction covers two contiguous output rows instead of 64
-// scattered words.
 // The item count is read from device memory so that one captured graph can
 // replay the launch for any number of items.  Tasks [g_begin, g_end) of the
 // launch are processed (streamed tables run a launch in task groups).
 template <bool FINAL, bool OOC>
 __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p,
                                             const uint64_t *__restrict__ blk_prefix, unsigned int *__restrict__ queue,
-                                            uint32_t max_nb, uint32_t max_ndup, uint32_t chunk,
+                                            uint32_t max_nb, uint32_t max_ndup,
                                             unsigned long long *__restrict__ prof_items, uint64_t g_begin,
                                             uint64_t g_end) {
   const uint32_t nitems = *nitems_p;
+  __shared__ uint32_t task;
   __shared__ uint32_t st[VW][65];
   __shared__ uint64_t drv[VW][64];  // address of each row's driving list
-  __shared__ uint32_t sv_row[VW][64], sv_val[VW][64];  // this step's survivors
   extern __shared__ unsigned char vlds[];
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const uint32_t per_wave = 64 * ((max_nb > 0 ? max_nb - 1 : 0) * 12 + max_ndup * 4);
@@ -151,123 +128,96 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
   uint32_t *olen = reinterpret_cast<uint32_t *>(mine + (size_t)64 * 8 * (max_nb - 1));   // [nb-1][64]
   uint32_t *dpos = reinterpret_cast<uint32_t *>(mine + (size_t)64 * 12 * (max_nb - 1));  // [ndup][64]
   const uint64_t total = blk_prefix[nitems] < g_end ? blk_prefix[nitems] : g_end;
-  uint64_t walked = 0;
   for (;;) {
-    uint32_t wt = 0;
-    if (l == 0) wt = atomicAdd(queue, 1u);
-    wt = readlane_u32(wt, 0);
-    const uint64_t g = g_begin + wt;
+    if (threadIdx.x == 0) task = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint64_t g = g_begin + task;
+    __syncthreads();
     if (g >= total) break;
     const uint32_t t = find_item(blk_prefix, nitems, g);
     const VItem it = items[t];
     const VLevel &L = *it.lv;
     const uint32_t nb = L.nb, ndup = L.ndup, k = it.k;
-    const uint64_t u_end = it.gpre[it.g1];
-    uint64_t x = it.gpre[it.g0] + (g - blk_prefix[t]) * chunk;
-    const uint64_t x1 = x + chunk < u_end ? x + chunk : u_end;
+    const uint64_t rbase = it.row_begin + (g - blk_prefix[t]) * VB + w * 64;
+    const uint64_t r = rbase + l;
+    const bool valid = r < it.row_end;
+    const uint32_t *row = it.rows + r * k;
+    // per-row lists: the shortest one drives, the others are probed
+    uint32_t dlen = 0, dj = 0, dp = 0;
+    uint64_t dbase = 0, daddr = 0;
+    if (valid) {
+      dlen = 0xFFFFFFFFu;
+      for (uint32_t j = 0; j < nb; ++j) {
+        const uint32_t p = row[L.key_col[j]];
+        const uint64_t b = L.toff[j][p];
+        const uint32_t len = (uint32_t)(L.toff[j][p + 1] - b);
+        if (len < dlen) {
+          dlen = len;
+          dj = j;
+          dbase = b;
+          dp = p;
+        }
+      }
+      daddr = list_addr<OOC>(L, dj, dp, dbase);
+      uint32_t o = 0;
+      for (uint32_t j = 0; j < nb; ++j) {
+        if (j == dj) continue;
+        const uint32_t p = row[L.key_col[j]];
+        const uint64_t b = L.toff[j][p];
+        obase[o * 64 + l] = list_addr<OOC>(L, j, p, b);
+        olen[o * 64 + l] = (uint32_t)(L.toff[j][p + 1] - b);
+        ++o;
+      }
+      for (uint32_t c = 0; c < ndup; ++c) {
+        const uint32_t vid = L.dup_can[c][row[L.dup_col[c]]];
+        dpos[c * 64 + l] = vposition_in(L.bm_new, L.rank_new, vid);
+      }
+    }
+    const uint32_t incl = wave_incl_scan_u32(dlen);
+    const uint32_t wtotal = readlane_u32(incl, 63);
+    st[w][l] = incl - dlen;
+    if (l == 63) st[w][64] = wtotal;
+    drv[w][l] = daddr;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     uint64_t survivors = 0;
-    uint64_t grp = it.g0;
-    while (x < x1) {
-      grp = group_of(it.gpre, grp, it.g1, x);
-      const uint64_t gbeg = it.gpre[grp], gend = it.gpre[grp + 1];
-      const uint64_t rbase = grp * 64;
-      const uint64_t r = rbase + l;
-      const bool valid = r < it.row_end;
-      const uint32_t *row = it.rows + r * k;
-      // per-row lists: the shortest one drives, the others are probed
-      uint32_t dlen = 0, dj = 0, dp = 0;
-      uint64_t dbase = 0, daddr = 0;
-      if (valid) {
-        dlen = 0xFFFFFFFFu;
-        for (uint32_t j = 0; j < nb; ++j) {
-          const uint32_t p = row[L.key_col[j]];
-          const uint64_t b = L.toff[j][p];
-          const uint32_t len = (uint32_t)(L.toff[j][p + 1] - b);
-          if (len < dlen) {
-            dlen = len;
-            dj = j;
-            dbase = b;
-            dp = p;
+    for (uint32_t s = 0; s < wtotal; s += 64) {
+      const uint32_t item = s + l;
+      const bool iv = item < wtotal;
+      bool keep = false;
+      uint32_t val = 0, rw = 0;
+      if (iv) {
+        rw = seg_of(st[w], item);
+        val = reinterpret_cast<const uint32_t *>(drv[w][rw])[item - st[w][rw]];
+        keep = true;
+        for (uint32_t c = 0; c < ndup && keep; ++c)
+          if (dpos[c * 64 + rw] == val) keep = false;
+        for (uint32_t o = 0; o + 1 < nb && keep; ++o)
+          keep = sorted_contains(reinterpret_cast<const uint32_t *>(obase[o * 64 + rw]), olen[o * 64 + rw], val);
+      }
+      const uint64_t m = __ballot(keep);
+      const uint32_t nk = (uint32_t)__popcll(m);
+      if (FINAL) {
+        survivors += nk;
+      } else if (nk) {
+        uint64_t base = 0;
+        if (l == 0) base = atomicAdd(it.counter, (unsigned long long)nk);
+        base = readlane_u64(base, 0);
+        if (keep) {
+          const uint64_t o = base + rank_below(m);
+          if (o < it.out_cap) {
+            const uint32_t *src = it.rows + (rbase + rw) * k;
+            uint32_t *dst = it.out + o * (k + 1);
+            for (uint32_t c = 0; c < k; ++c) dst[c] = src[c];
+            dst[k] = val;
           }
-        }
-        daddr = list_addr<OOC>(L, dj, dp, dbase);
-        uint32_t o = 0;
-        for (uint32_t j = 0; j < nb; ++j) {
-          if (j == dj) continue;
-          const uint32_t p = row[L.key_col[j]];
-          const uint64_t b = L.toff[j][p];
-          obase[o * 64 + l] = list_addr<OOC>(L, j, p, b);
-          olen[o * 64 + l] = (uint32_t)(L.toff[j][p + 1] - b);
-          ++o;
-        }
-        for (uint32_t c = 0; c < ndup; ++c) {
-          const uint32_t vid = L.dup_can[c][row[L.dup_col[c]]];
-          dpos[c * 64 + l] = vposition_in(L.bm_new, L.rank_new, vid);
         }
       }
-      const uint32_t incl = wave_incl_scan_u32(dlen);
-      st[w][l] = incl - dlen;
-      if (l == 63) st[w][64] = incl;
-      drv[w][l] = daddr;
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      // this task's entries of the group: [ws, we) in the group's walk order
-      // (the group holds gend - gbeg entries, the same sum the scan produced)
-      const uint32_t ws = (uint32_t)(x - gbeg);
-      const uint32_t we = (uint32_t)((x1 < gend ? x1 : gend) - gbeg);
-      walked += we - ws;
-      for (uint32_t s = ws; s < we; s += 64) {
-        const uint32_t item = s + l;
-        const bool iv = item < we;
-        bool keep = false;
-        uint32_t val = 0, rw = 0;
-        if (iv) {
-          rw = seg_of(st[w], item);
-          val = reinterpret_cast<const uint32_t *>(drv[w][rw])[item - st[w][rw]];
-          keep = true;
-          for (uint32_t c = 0; c < ndup && keep; ++c)
-            if (dpos[c * 64 + rw] == val) keep = false;
-          for (uint32_t o = 0; o + 1 < nb && keep; ++o)
-            keep = sorted_contains(reinterpret_cast<const uint32_t *>(obase[o * 64 + rw]), olen[o * 64 + rw], val);
-        }
-        const uint64_t m = __ballot(keep);
-        const uint32_t nk = (uint32_t)__popcll(m);
-        if (FINAL) {
-          survivors += nk;
-        } else if (nk) {
-          uint64_t base = 0;
-          if (l == 0) base = atomicAdd(it.counter, (unsigned long long)nk);
-          base = readlane_u64(base, 0);
-          if (keep) {
-            const uint32_t i = rank_below(m);
-            sv_row[w][i] = rw;
-            sv_val[w][i] = val;
-          }
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          // half-wave h copies survivor i + h: k parent columns, then the value
-          const uint32_t h = l >> 5, c = l & 31;
-          for (uint32_t i = 0; i < nk; i += 2) {
-            const uint32_t sidx = i + h;
-            const uint64_t o = base + sidx;
-            if (sidx < nk && c <= k && o < it.out_cap) {
-              const uint32_t v = c < k ? it.rows[(rbase + sv_row[w][sidx]) * k + c] : sv_val[w][sidx];
-              it.out[o * (k + 1) + c] = v;
-            }
-          }
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        }
-      }
-      // st / drv / LDS lists are rewritten for the next group
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      x = x1 < gend ? x1 : gend;
-      ++grp;
     }
     if (FINAL && l == 0 && survivors) atomicAdd(it.counter, (unsigned long long)survivors);
+    if (prof_items && l == 0 && wtotal) atomicAdd(prof_items, (unsigned long long)wtotal);
+    __syncthreads();
   }
-  if (prof_items && l == 0 && walked) atomicAdd(prof_items, (unsigned long long)walked);
 }
 
 // Last level, one backward list: |list| - equal-label columns present.
@@ -338,43 +288,6 @@ __global__ void __launch_bounds__(VB) k_vmark(const VItem *__restrict__ items, u
   }
 }
 
-// The same marks for an extend launch, whose tasks are ranges of `chunk`
-// walked entries: a wave per task marks every row of every 64-row group the
-// task overlaps (a superset of the rows whose lists it reads).
-__global__ void __launch_bounds__(VB) k_vmark_chunks(const VItem *__restrict__ items, uint32_t nitems,
-                                                    const uint64_t *__restrict__ blk_prefix, uint32_t chunk,
-                                                    uint64_t base, uint32_t *__restrict__ first,
-                                                    uint32_t *__restrict__ last) {
-  const uint32_t l = threadIdx.x % 64;
-  const uint64_t total = blk_prefix[nitems];
-  const uint64_t nw = (uint64_t)gridDim.x * VW;
-  for (uint64_t g = (uint64_t)blockIdx.x * VW + threadIdx.x / 64; g < total; g += nw) {
-    const uint32_t t = find_item(blk_prefix, nitems, g);
-    const VItem it = items[t];
-    const VLevel &L = *it.lv;
-    const uint64_t u_end = it.gpre[it.g1];
-    uint64_t x = it.gpre[it.g0] + (g - blk_prefix[t]) * chunk;
-    const uint64_t x1 = x + chunk < u_end ? x + chunk : u_end;
-    const uint32_t gt = (uint32_t)(base + g);
-    uint64_t grp = it.g0;
-    while (x < x1) {
-      grp = group_of(it.gpre, grp, it.g1, x);
-      const uint64_t gend = it.gpre[grp + 1];
-      const uint64_t r = grp * 64 + l;
-      if (r < it.row_end) {
-        const uint32_t *row = it.rows + r * it.k;
-        for (uint32_t j = 0; j < L.nb; ++j) {
-          const uint32_t b = L.gblk[j] + (row[L.key_col[j]] >> L.kshift[j]);
-          if (first[b] > gt) atomicMin(&first[b], gt);
-          if (last[b] < gt) atomicMax(&last[b], gt);
-        }
-      }
-      x = x1 < gend ? x1 : gend;
-      ++grp;
-    }
-  }
-}
-
 __global__ void k_viota(uint32_t *p, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = (uint32_t)i;
@@ -425,16 +338,10 @@ struct Launch {
   std::vector<VItem> items;
   std::vector<uint64_t> blk;  // prefix, items.size() + 1
   uint32_t max_nb = 1, max_ndup = 0;
-  // tasks of 256 rows
-  void add(const VItem &it, uint32_t nb, uint32_t ndup) { add_tasks(it, (it.row_end - it.row_begin + VB - 1) / VB, nb, ndup); }
-  // tasks of `chunk` walked entries (`entries` in the item)
-  void add_chunks(const VItem &it, uint64_t entries, uint32_t chunk, uint32_t nb, uint32_t ndup) {
-    add_tasks(it, (entries + chunk - 1) / chunk, nb, ndup);
-  }
-  void add_tasks(const VItem &it, uint64_t ntasks, uint32_t nb, uint32_t ndup) {
+  void add(const VItem &it, uint32_t nb, uint32_t ndup) {
     if (blk.empty()) blk.push_back(0);
     items.push_back(it);
-    blk.push_back(blk.back() + ntasks);
+    blk.push_back(blk.back() + (it.row_end - it.row_begin + VB - 1) / VB);
     max_nb = std::max(max_nb, nb);
     max_ndup = std::max(max_ndup, ndup);
   }
@@ -482,7 +389,6 @@ struct ExtSlot {
   uint32_t max_nb = 1, max_ndup = 0;
   size_t lds = 0;
   uint32_t grid = 0;
-  uint32_t chunk = 4096;  // list entries per extend task
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
@@ -494,7 +400,6 @@ struct ExtSlot {
     lds = (size_t)VW * 64 * ((max_nb - 1) * 12 + max_ndup * 4);
     if (lds > 60 * 1024) throw Error{GEM_E_LIMIT, "a level has too many backward edges for the vjoin LDS layout"};
     grid = g_persistent_blocks;
-    chunk = c->join_chunk;
     hdr_bytes = HDR_CTR + 8ull * cap;
     hdr.alloc(hdr_bytes, s);
     GEM_HIP_CHECK(hipHostMalloc((void **)&h_hdr, hdr_bytes, hipHostMallocDefault));
@@ -549,13 +454,13 @@ struct ExtSlot {
     if (b0 < e0) {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_WRITE : -1);
       k_vext<false, OOC><<<grd, VB, lds, s>>>(items[0].as<VItem>(), dn + 0, blk[0].as<uint64_t>(), dq + 0, max_nb,
-                                               max_ndup, chunk, count_items ? dp : nullptr, b0, e0);
+                                               max_ndup, count_items ? dp : nullptr, b0, e0);
       GEM_LAUNCH_CHECK("k_vext<write>");
     }
     if (b1 < e1) {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_FINAL : -1);
       k_vext<true, OOC><<<grd, VB, lds, s>>>(items[1].as<VItem>(), dn + 1, blk[1].as<uint64_t>(), dq + 1, max_nb,
-                                              max_ndup, chunk, count_items ? dp + 1 : nullptr, b1, e1);
+                                              max_ndup, count_items ? dp + 1 : nullptr, b1, e1);
       GEM_LAUNCH_CHECK("k_vext<final>");
     }
     if (b2 < e2) {
@@ -584,13 +489,8 @@ struct ExtSlot {
     tc.reset_marks();
     for (int i = 0; i < 3; ++i) {
       if (!T[i]) continue;
-      const uint32_t grd = (uint32_t)std::min<uint64_t>(i == 2 ? T[i] : (T[i] + VW - 1) / VW, 4096);
-      if (i == 2)
-        k_vmark<<<grd, VB, 0, s>>>(items[i].as<VItem>(), *nitems(i), blk[i].as<uint64_t>(), base[i], true,
-                                   tc.d_first(), tc.d_last());
-      else
-        k_vmark_chunks<<<grd, VB, 0, s>>>(items[i].as<VItem>(), *nitems(i), blk[i].as<uint64_t>(), chunk, base[i],
-                                          tc.d_first(), tc.d_last());
+      k_vmark<<<(uint32_t)std::min<uint64_t>(T[i], 4096), VB, 0, s>>>(
+          items[i].as<VItem>(), *nitems(i), blk[i].as<uint64_t>(), base[i], i == 2, tc.d_first(), tc.d_last());
       GEM_LAUNCH_CHECK("k_vmark");
     }
     tc.plan(total);
@@ -709,15 +609,15 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
                    act.size(), qs.size());
       std::fflush(stderr);
     }
-    // ---- length pass for unmeasured top items that walk lists (every level
-    // but a final one with a single backward list, which k_vcount1 counts
-    // from offsets alone)
+    // ---- length pass for unmeasured, non-final top items
     Launch lenL;
     std::vector<VQueryState *> len_owner;
     for (VQueryState *S : act) {
       VStackItem &top = S->stack.back();
       const bool final_level = top.level + 1 == S->qd->Q.nv;
-      if (top.measured || (final_level && S->max_nb[top.level] == 1)) continue;
+      // final levels are measured only when they walk lists (nb > 1) under a
+      // cap: their blocks are then counted a cap-sized piece at a time
+      if (top.measured || (final_level && (S->max_nb[top.level] == 1 || !cap_))) continue;
       VItem it{};
       it.lv = S->levels.as<VLevel>() + top.level;
       it.rows = top.rows->as<uint32_t>();
@@ -730,33 +630,26 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
     if (!lenL.empty()) {
       DevLaunch D;
       upload(c_, lenL, D);
-      // one entry per 64-row group (4 per 256-row task)
-      const uint64_t ngrp = lenL.blk.back() * VW;
-      DevBuf bu;
-      auto bp = std::make_shared<DevBuf>();
-      bu.alloc(ngrp * 8, s);
-      bp->alloc((ngrp + 1) * 8, s);
+      const uint64_t nblk = lenL.blk.back();
+      DevBuf bu, bp;
+      bu.alloc(nblk * 8, s);
+      bp.alloc((nblk + 1) * 8, s);
       {
         ProfLaunch pl(c_, K_VLEN);
-        k_vlen<<<grid_for(lenL.blk.back()), VB, 0, s>>>(D.items.as<VItem>(), (uint32_t)lenL.items.size(),
-                                                         D.blk.as<uint64_t>(), D.queue.as<unsigned int>(),
-                                                         bu.as<uint64_t>());
+        k_vlen<<<grid_for(nblk), VB, 0, s>>>(D.items.as<VItem>(), (uint32_t)lenL.items.size(), D.blk.as<uint64_t>(),
+                                              D.queue.as<unsigned int>(), bu.as<uint64_t>());
         GEM_LAUNCH_CHECK("k_vlen");
       }
-      scan_u64(c_, bu.as<uint64_t>(), bp->as<uint64_t>(), ngrp);
-      std::vector<uint64_t> pref(ngrp + 1);
-      GEM_HIP_CHECK(hipMemcpyAsync(pref.data(), bp->p, (ngrp + 1) * 8, hipMemcpyDeviceToHost, s));
+      scan_u64(c_, bu.as<uint64_t>(), bp.as<uint64_t>(), nblk);
+      std::vector<uint64_t> pref(nblk + 1);
+      GEM_HIP_CHECK(hipMemcpyAsync(pref.data(), bp.p, (nblk + 1) * 8, hipMemcpyDeviceToHost, s));
       GEM_HIP_CHECK(hipStreamSynchronize(s));
       for (size_t t = 0; t < len_owner.size(); ++t) {
         VStackItem &top = len_owner[t]->stack.back();
-        const uint64_t g0 = lenL.blk[t] * VW, ng = (top.nrows + 63) / 64;
-        top.block_u.assign(pref.begin() + g0, pref.begin() + g0 + ng + 1);
+        const uint64_t b0 = lenL.blk[t], b1 = lenL.blk[t + 1];
+        top.block_u.assign(pref.begin() + b0, pref.begin() + b1 + 1);
         const uint64_t base = top.block_u[0];
         for (auto &x : top.block_u) x -= base;
-        // the kernels read the same prefix (absolute values: only differences
-        // and comparisons are used)
-        top.gpre_buf = bp;
-        top.gpre = bp->as<uint64_t>() + g0;
         top.measured = true;
         // rows + one offset pair per backward list
         prof_bytes(c_, K_VLEN, (double)top.nrows * (4.0 * top.k + 16.0 * len_owner[t]->max_nb[top.level]));
@@ -783,60 +676,50 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
       VStackItem &top = S->stack.back();
       const uint32_t nv = S->qd->Q.nv;
       const bool final_level = top.level + 1 == nv;
-      const uint64_t ngroups = (top.nrows + 63) / 64;  // 64-row groups of the level
+      const uint64_t nblocks = (top.nrows + VB - 1) / VB;
       VItem it{};
       it.lv = S->levels.as<VLevel>() + top.level;
       it.rows = top.rows->as<uint32_t>();
       it.k = top.k;
-      it.gpre = top.gpre;
       keep_alive.push_back(top.rows);
-      if (top.gpre_buf) keep_alive.push_back(top.gpre_buf);
       if (final_level) {
-        uint64_t b1 = ngroups;
+        uint64_t b1 = nblocks;
         if (top.measured) {
           // count a piece whose survivor bound is about twice what is still
           // missing to exceed the cap (at least 2^26 list entries of work)
-          const uint64_t need = cap_ && cap_ + 1 > S->count ? cap_ + 1 - S->count : 1;
+          const uint64_t need = cap_ + 1 > S->count ? cap_ + 1 - S->count : 1;
           const uint64_t target = std::max<uint64_t>(2 * need, 1ull << 26);
           b1 = top.cursor + 1;
-          while (b1 < ngroups && top.block_u[b1] - top.block_u[top.cursor] < target) ++b1;
+          while (b1 < nblocks && top.block_u[b1] - top.block_u[top.cursor] < target) ++b1;
         }
-        const uint64_t b0 = top.cursor;
-        it.row_begin = b0 * 64;
-        it.row_end = std::min<uint64_t>(b1 * 64, top.nrows);
-        it.g0 = b0;
-        it.g1 = b1;
-        const uint64_t entries = top.measured ? top.block_u[b1] - top.block_u[b0] : 0;
-        S->work += entries + (it.row_end - it.row_begin);
-        S->processed += it.row_end - it.row_begin;
+        it.row_begin = (uint64_t)top.cursor * VB;
+        it.row_end = std::min<uint64_t>(b1 * VB, top.nrows);
+        if (top.measured) S->work += top.block_u[b1] - top.block_u[top.cursor];
         top.cursor = b1;
-        if (S->max_nb[top.level] == 1) {
-          it.counter = reinterpret_cast<unsigned long long *>((uintptr_t)nctr);  // patched below
-          ft.push_back(FTask{S, nctr++});
+        it.counter = reinterpret_cast<unsigned long long *>((uintptr_t)nctr);  // patched below
+        ft.push_back(FTask{S, nctr++});
+        S->processed += it.row_end - it.row_begin;
+        S->work += it.row_end - it.row_begin;
+        if (S->max_nb[top.level] == 1)
           fin1.add(it, 1, S->max_ndup[top.level]);
-        } else if (entries) {
-          it.counter = reinterpret_cast<unsigned long long *>((uintptr_t)nctr);
-          ft.push_back(FTask{S, nctr++});
-          fin.add_chunks(it, entries, slot.chunk, S->max_nb[top.level], S->max_ndup[top.level]);
-        }
-        if (top.cursor >= ngroups) S->stack.pop_back();
+        else
+          fin.add(it, S->max_nb[top.level], S->max_ndup[top.level]);
+        if (top.cursor >= nblocks) S->stack.pop_back();
         continue;
       }
-      // one range of groups whose output bound fits the per-query budget
+      // one chunk of blocks whose output bound fits the per-query budget
       const uint64_t row_bytes = 4ull * (top.k + 1);
       const uint64_t lim = std::max<uint64_t>(budget / row_bytes, 1);
       uint64_t b0 = top.cursor, b1 = b0 + 1;
-      while (b1 < ngroups && top.block_u[b1 + 1] - top.block_u[b0] <= lim) ++b1;
+      while (b1 < nblocks && top.block_u[b1 + 1] - top.block_u[b0] <= lim) ++b1;
       const uint64_t ub = top.block_u[b1] - top.block_u[b0];
-      it.row_begin = b0 * 64;
-      it.row_end = std::min<uint64_t>(b1 * 64, top.nrows);
-      it.g0 = b0;
-      it.g1 = b1;
+      it.row_begin = b0 * VB;
+      it.row_end = std::min<uint64_t>(b1 * VB, top.nrows);
       S->processed += it.row_end - it.row_begin;
       S->work += it.row_end - it.row_begin + ub;
       top.cursor = b1;
       if (ub == 0) {
-        if (top.cursor >= ngroups) S->stack.pop_back();
+        if (top.cursor >= nblocks) S->stack.pop_back();
         continue;
       }
       auto out = std::make_shared<DevBuf>();
@@ -845,8 +728,8 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
       it.out_cap = ub;
       it.counter = reinterpret_cast<unsigned long long *>((uintptr_t)nctr);
       wt.push_back(WTask{S, out, top.level + 1, top.k + 1, ub, nctr++});
-      wr.add_chunks(it, ub, slot.chunk, S->max_nb[top.level], S->max_ndup[top.level]);
-      if (top.cursor >= ngroups) S->stack.pop_back();
+      wr.add(it, S->max_nb[top.level], S->max_ndup[top.level]);
+      if (top.cursor >= nblocks) S->stack.pop_back();
     }
     if (nctr == 0) continue;
     if (nctr > slot.cap || wr.items.size() > slot.cap || fin.items.size() > slot.cap || fin1.items.size() > slot.cap)

@@ -147,10 +147,6 @@ class GemContext:
     def set_persistent_blocks(self, blocks):
         self._check(self._lib.gem_hip_set_persistent_blocks(self._h, int(blocks)), "gem_hip_set_persistent_blocks")
 
-    def set_join_chunk(self, entries):
-        """List entries walked per extend task of the batched path."""
-        self._check(self._lib.gem_hip_set_join_chunk(self._h, int(entries)), "gem_hip_set_join_chunk")
-
     def set_graph(self, on=True):
         """Replay the batched path's extend step as a captured hipGraph."""
         self._check(self._lib.gem_hip_set_graph(self._h, 1 if on else 0), "gem_hip_set_graph")

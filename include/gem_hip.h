@@ -266,11 +266,6 @@ int gem_hip_match_batch(gem_ctx *ctx, const gem_query_view *queries, uint32_t nq
  * kernels; they pull tasks from a device counter, so this sets occupancy,
  * not the amount of work (default 2048).  Set by the autotuner. */
 int gem_hip_set_persistent_blocks(gem_ctx *ctx, uint32_t blocks);
-/* Intersection tile of the batched path's extend kernels: each wave task
- * walks this many list entries (a multiple of 64, 64..2^24; default 4096), so
- * hub rows are spread over many waves and light rows are packed together.
- * Set by the autotuner. */
-int gem_hip_set_join_chunk(gem_ctx *ctx, uint32_t entries);
 /* on != 0: the batched path's per-iteration extend step (table uploads, three
  * persistent kernels, result download) is captured once per run as a hipGraph
  * and replayed; off (default): launched eagerly.  Kernel profiling forces the

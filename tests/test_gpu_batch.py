@@ -115,14 +115,6 @@ def test_batch_invariance(gpu_ctx, rmat_graph):
     base = [(r["count"], r["saturated"]) for r in gpu_ctx.match_batch(qs, cap=cap)]
     small = gpu_ctx.match_batch(qs, cap=cap, level_budget=1 << 20, group=3)
     assert [(r["count"], r["saturated"]) for r in small] == base
-    # intersection tile: one 64-entry step per task, and very large tasks
-    try:
-        for chunk in (64, 1 << 20):
-            gpu_ctx.set_join_chunk(chunk)
-            got = gpu_ctx.match_batch(qs, cap=cap)
-            assert [(r["count"], r["saturated"]) for r in got] == base, chunk
-    finally:
-        gpu_ctx.set_join_chunk(4096)
     rng = np.random.default_rng(5)
     orders = [_random_vertex_order(q, rng) for q in qs]
     other = gpu_ctx.match_batch(qs, vertex_orders=orders, cap=cap)
