@@ -78,8 +78,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=64,
                     help="run the plain path on a systematic sample of this many queries (every k-th; "
                          "0 = all queries)")
-    ap.add_argument("--model", default=os.path.join(ROOT, "gem_amd", "models", "order_mlp_rmat20.npz"),
-                    help="trained order model (.npz); absent = GEM-derived orders")
+    ap.add_argument("--model", default=None,
+                    help="trained order model (.npz); default: the one trained for the workload's graph "
+                         "(gem_amd/models/order_mlp_<graph>.npz); absent = GEM-derived orders")
     ap.add_argument("--orders", type=int, default=16, help="candidate orders scored per query")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-oracle work (0 = skip)")
     ap.add_argument("--cpu-work-limit", type=float, default=2e9,
@@ -121,6 +122,8 @@ def main():
     # ---- batched path with the learned order (the measured value)
     from gem_amd.learned import OrderModel, match_batch_learned
     model = None
+    if args.model is None:
+        args.model = os.path.join(ROOT, "gem_amd", "models", f"order_mlp_{args.workload}.npz")
     if args.model and os.path.exists(args.model):
         model = OrderModel.load(ctx, args.model)
         log(f"[bench] order model {args.model}: trained on {model.meta.get('workload')} "
