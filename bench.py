@@ -1,15 +1,18 @@
 #!/usr/bin/env python
 """Benchmark: embeddings/s and queries/s of GEM subgraph matching on one MI355X.
 
-A "step" is one pass of the batched matcher over the workload's whole query
-set (filter -> position index -> edge tables -> plan -> vertex-at-a-time join
-for every query, many queries per kernel launch), with the data graph already
-resident in HBM.  `value` is that path's queries/s.  The plain path (the
-reference's per-query pipeline with GEM's edge-at-a-time join) is timed over
-one pass in the same run, and the counts of the two are compared query by
-query.  Counts are exact; on the R-MAT workloads each query's count is capped
-at 1e9 (saturated queries report count = cap), and a query that processes more
-than --work-limit rows is reported unfinished.
+Default workload: the 512-query R-MAT batch (2^22 vertices, 32 Zipf labels,
+queries of 4-32 vertices).  A "step" is one pass of the batched matcher with
+model-chosen join orders over the whole query set (filter -> position index ->
+edge tables -> order candidates + MLP -> vertex-at-a-time join for every query,
+many queries per kernel launch), with the data graph already resident in HBM.
+`value` is that path's queries/s.  The plain path (the reference's per-query
+pipeline with GEM's edge-at-a-time join) is timed in the same run over a
+systematic sample of the queries, and the counts of the two are compared query
+by query.  Counts are exact; on the R-MAT workloads each query's count is
+capped at 1e9 (saturated queries report count = cap), and a query whose join
+exceeds --work-limit is reported unfinished (plain path: rows read by its
+steps; batched path: rows read plus list entries walked).
 
 Prints ONE JSON line on stdout (progress goes to stderr).
 """
@@ -68,11 +71,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat20", choices=sorted(DESCR))
+    ap.add_argument("--workload", default="rmat22", choices=sorted(DESCR))
     ap.add_argument("--queries", type=int, default=0, help="use only the first N queries")
     ap.add_argument("--cap", type=float, default=1e9)
-    ap.add_argument("--work-limit", type=float, default=2e10,
-                    help="rows a query's join may process before it is reported unfinished")
+    ap.add_argument("--work-limit", type=float, default=None,
+                    help="work a query's join may do before it is reported unfinished (default: 5e9 on rmat22 "
+                         "and ooc24, 2e10 on rmat20)")
     ap.add_argument("--group", type=int, default=0, help="queries resident at once in the batched path (0 = all)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path pass")
     ap.add_argument("--plain-sample", type=int, default=64,
@@ -108,6 +112,8 @@ def main():
     if args.queries:
         qs = qs[:args.queries]
     cap = int(args.cap) if args.workload != "er10k" else 0
+    if args.work_limit is None:
+        args.work_limit = {"rmat20": 2e10, "er10k": 0}.get(args.workload, 5e9)
     wl = int(args.work_limit)
     log(f"[bench] {args.workload}: |V|={g.num_vertices} |E|={g.num_edges} queries={len(qs)} "
         f"dense={sum(q.dense for q in qs)} built in {time.perf_counter() - t0:.1f}s")
