@@ -60,6 +60,23 @@ def cpu_model():
     return platform.processor()
 
 
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summaries (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    copy of known size), or None when no summary covers it."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for fn in sorted(os.listdir(pdir)):
+        if not fn.endswith(".md"):
+            continue
+        for line in open(os.path.join(pdir, fn)):
+            f = line.split()
+            if len(f) == 4 and f[0] == "traffic_per_launch" and f[1] == kernel and f[2] == workload:
+                return int(f[3])
+    return None
+
+
 def kernel_table(stats):
     return {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                 "GB/s": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else 0.0}
@@ -206,6 +223,7 @@ def main():
     # roofline: the kernel with the most time in the timed region
     dom_name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else 0.0
+    traffic = profiled_traffic(dom_name, args.workload)
 
     # ---- plain path, one pass over a systematic sample (every k-th query)
     plain = None
@@ -301,7 +319,9 @@ def main():
         "batched_gem_order": gem_batched,
         "speedup_vs_plain": round(qps / plain["queries_per_s"], 3) if plain else None,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": round(dom["bytes"] / max(dom["launches"], 1)),
+                     "launches": dom["launches"], "avg_launch_ms": round(dom["ms"] / max(dom["launches"], 1), 3)},
         "cpu_baseline": cpu,
         "streamed_tables": stream,
     }
