@@ -107,12 +107,12 @@ def main():
     ap.add_argument("--cpu-work-limit", type=float, default=2e9,
                     help="intermediate rows after which one oracle query is abandoned")
     ap.add_argument("--stream-cache", default=None,
-                    help="stream edge tables through a device cache of this size (e.g. 2G); default: 2G for ooc24, "
+                    help="stream edge tables through a device cache of this size (e.g. 8G); default: 8G for ooc24, "
                          "tables in HBM otherwise")
     ap.add_argument("--stream-slot", default="1M", help="cache slot size for streamed tables")
     args = ap.parse_args()
     if args.stream_cache is None:
-        args.stream_cache = "2G" if args.workload == "ooc24" else "0"
+        args.stream_cache = "8G" if args.workload == "ooc24" else "0"
     stream_cache = parse_bytes(args.stream_cache)
 
     if args.gpus != 1:

@@ -284,7 +284,9 @@ int gem_hip_set_graph(gem_ctx *ctx, int on);
  * are copied host -> device on a second stream into a device cache of
  * cache_bytes.  Counts are unchanged.  cache_bytes == 0: tables stay in HBM
  * (default).  A list longer than a slot, or a 256-row task whose blocks do
- * not fit the cache, fails with GEM_E_LIMIT.  The single-query entry points
+ * not fit the cache, fails with GEM_E_LIMIT; cache_bytes / slot_bytes >= 256 x
+ * (the most backward edges of any query vertex) always suffices.  The
+ * single-query entry points
  * (gem_hip_join, gem_hip_vjoin, gem_hip_match) always keep tables in HBM. */
 int gem_hip_set_table_stream(gem_ctx *ctx, uint64_t cache_bytes, uint64_t slot_bytes);
 
