@@ -7,7 +7,10 @@
 //   1. marks, per block, the first and last 256-row task that reads it
 //      (k_vmark: atomicMin / atomicMax of the task index),
 //   2. cuts its tasks into groups such that the blocks live in two
-//      consecutive groups fit the cache together,
+//      consecutive groups fit the cache together (a block is live from its
+//      first to its last use; when the blocks live at a single task do not
+//      fit, the iteration is marked and planned again in smaller windows of
+//      tasks, down to one task),
 //   3. for each group, copies the missing blocks host -> device on a second
 //      stream while the previous group's kernels run, rewrites their entries
 //      of the block map (vmap: block -> device address of its values, minus
@@ -55,8 +58,10 @@ class TableCache {
   // On the context stream: first = ~0, last = 0 for every block.
   void reset_marks();
   // Read the marks back (synchronises the context stream), then cut
-  // total_tasks into groups and decide every copy and eviction.
-  void plan(uint64_t total_tasks);
+  // total_tasks into groups and decide every copy and eviction.  False (and
+  // no plan) when the blocks live at one task cannot all be resident: the
+  // caller marks a smaller window of tasks and plans again.
+  bool plan(uint64_t total_tasks);
   const std::vector<Group> &groups() const { return groups_; }
   // Device queue counters for the group launches (3 per group), zeroed.
   unsigned int *queues();
@@ -90,6 +95,7 @@ class TableCache {
   hipEvent_t ev_step0_ = nullptr;
   size_t npatch_ = 0;
   uint64_t hits_ = 0;
+  void collect();
   uint64_t guard_addr() const;
   uint64_t slot_addr(uint32_t s) const;
 };

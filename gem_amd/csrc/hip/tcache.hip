@@ -164,17 +164,17 @@ void TableCache::reset_marks() {
   GEM_LAUNCH_CHECK("k_fill_u32");
 }
 
-void TableCache::plan(uint64_t total) {
+bool TableCache::plan(uint64_t total) {
   hipStream_t s = c_->stream;
-  groups_.clear();
-  npatch_ = 0;
-  hits_ = 0;
   if (nblk_) {
     GEM_HIP_CHECK(hipMemcpyAsync(h_first_.p, first_.p, 4 * nblk_, hipMemcpyDeviceToHost, s));
     GEM_HIP_CHECK(hipMemcpyAsync(h_last_.p, last_.p, 4 * nblk_, hipMemcpyDeviceToHost, s));
   }
   GEM_HIP_CHECK(hipStreamSynchronize(s));
-  if (total == 0) return;
+  collect();  // the previous window's groups have finished
+  npatch_ = 0;
+  hits_ = 0;
+  if (total == 0) return true;
   const uint32_t *fi = h_first_.as<uint32_t>(), *la = h_last_.as<uint32_t>();
   // needed blocks (empty blocks are never copied: their lists read nothing)
   std::vector<uint64_t> need;
@@ -205,10 +205,12 @@ void TableCache::plan(uint64_t total) {
     if (S(a + 1) - E(lo) > nslots_) {
       G.serial = true;
       lo = a;
-      if (S(a + 1) - E(a) > nslots_)
-        throw Error{GEM_E_LIMIT, "the table blocks read by one 256-row task do not fit the device cache (" +
-                                     std::to_string(S(a + 1) - E(a)) + " blocks, " + std::to_string(nslots_) +
-                                     " slots)"};
+      // the blocks live at task a (used at or before it and at or after it)
+      // cannot all be resident: the caller plans a smaller window
+      if (S(a + 1) - E(a) > nslots_) {
+        groups_.clear();
+        return false;
+      }
     }
     uint64_t b = a + 1;
     while (b < total && b - a < target && S(b + 1) - E(lo) <= nslots_) ++b;
@@ -295,6 +297,7 @@ void TableCache::plan(uint64_t total) {
     ev_.push_back(e);
   }
   GEM_HIP_CHECK(hipEventRecord(ev_step0_, s));
+  return true;
 }
 
 unsigned int *TableCache::queues() {
@@ -335,16 +338,23 @@ void TableCache::issue(size_t g) {
 
 void TableCache::done(size_t g) { GEM_HIP_CHECK(hipEventRecord(ev_[3 * g + 2], c_->stream)); }
 
-void TableCache::finish() {
+void TableCache::finish() { collect(); }
+
+// Statistics of the planned window's groups once both streams are past them;
+// then the plan is dropped.
+void TableCache::collect() {
   c_->sstats.blocks_hit += hits_;
+  hits_ = 0;
   if (groups_.empty()) return;
   GEM_HIP_CHECK(hipStreamSynchronize(copy_));
+  GEM_HIP_CHECK(hipStreamSynchronize(c_->stream));
   for (size_t g = 0; g < groups_.size(); ++g) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, ev_[3 * g], ev_[3 * g + 1]) == hipSuccess) c_->sstats.copy_ms += ms;
   }
   float ms = 0;
   if (hipEventElapsedTime(&ms, ev_step0_, ev_[3 * (groups_.size() - 1) + 2]) == hipSuccess) c_->sstats.step_ms += ms;
+  groups_.clear();
 }
 
 }  // namespace gemd

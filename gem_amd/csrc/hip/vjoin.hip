@@ -266,12 +266,16 @@ __global__ void __launch_bounds__(VB) k_vcount1(const VItem *__restrict__ items,
 // Streamed tables: for every table block a row of the launch will read, the
 // first and last task (launch task + base) that reads it.  Blocks (256-row
 // tasks) are strided over the grid.
+// Only the launch's tasks g with base + g in [wa, wb) are marked, as
+// base + g - wa (the iteration is planned one window of tasks at a time).
 __global__ void __launch_bounds__(VB) k_vmark(const VItem *__restrict__ items, uint32_t nitems,
-                                             const uint64_t *__restrict__ blk_prefix, uint64_t base,
-                                             bool count_only, uint32_t *__restrict__ first,
+                                             const uint64_t *__restrict__ blk_prefix, uint64_t base, uint64_t wa,
+                                             uint64_t wb, bool count_only, uint32_t *__restrict__ first,
                                              uint32_t *__restrict__ last) {
   const uint64_t total = blk_prefix[nitems];
-  for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
+  const uint64_t g0 = wa > base ? wa - base : 0;
+  const uint64_t g1 = wb - base < total ? wb - base : total;
+  for (uint64_t g = g0 + blockIdx.x; g < g1; g += gridDim.x) {
     const uint32_t t = find_item(blk_prefix, nitems, g);
     const VItem it = items[t];
     const VLevel &L = *it.lv;
@@ -279,7 +283,7 @@ __global__ void __launch_bounds__(VB) k_vmark(const VItem *__restrict__ items, u
     // k_vcount1 reads list values only to remove equal-label columns
     if (r >= it.row_end || (count_only && L.ndup == 0)) continue;
     const uint32_t *row = it.rows + r * it.k;
-    const uint32_t gt = (uint32_t)(base + g);
+    const uint32_t gt = (uint32_t)(base + g - wa);
     for (uint32_t j = 0; j < L.nb; ++j) {
       const uint32_t b = L.gblk[j] + (row[L.key_col[j]] >> L.kshift[j]);
       if (first[b] > gt) atomicMin(&first[b], gt);
@@ -486,35 +490,51 @@ struct ExtSlot {
       total += T[i];
     }
     if (total >= 0xFFFFFFFFull) throw Error{GEM_E_LIMIT, "too many tasks in one streamed iteration"};
-    tc.reset_marks();
-    for (int i = 0; i < 3; ++i) {
-      if (!T[i]) continue;
-      k_vmark<<<(uint32_t)std::min<uint64_t>(T[i], 4096), VB, 0, s>>>(
-          items[i].as<VItem>(), *nitems(i), blk[i].as<uint64_t>(), base[i], i == 2, tc.d_first(), tc.d_last());
-      GEM_LAUNCH_CHECK("k_vmark");
-    }
-    tc.plan(total);
-    const auto &G = tc.groups();
-    unsigned int *q = tc.queues();
+    // The iteration is planned in windows of tasks: a block's marks span its
+    // first to last use inside the window, so a window whose blocks cannot
+    // all be placed is halved (down to one task) and planned again.
     const uint32_t *dn = reinterpret_cast<const uint32_t *>(d + HDR_NITEMS);
     const bool prof = c->prof.on;
-    for (size_t g = 0; g < G.size(); ++g) {
-      tc.issue(g);
-      uint64_t lo[3], hi[3];
+    uint64_t wa = 0, W = total;
+    while (wa < total) {
+      const uint64_t wb = std::min(total, wa + W);
+      tc.reset_marks();
       for (int i = 0; i < 3; ++i) {
-        lo[i] = std::max(G[g].a, base[i]);
-        hi[i] = std::min(G[g].b, base[i] + T[i]);
-        if (lo[i] < hi[i]) {
-          lo[i] -= base[i];
-          hi[i] -= base[i];
-        } else {
-          lo[i] = hi[i] = 0;
-        }
+        if (!T[i] || base[i] + T[i] <= wa || base[i] >= wb) continue;
+        const uint64_t lo = std::max(wa, base[i]), hi = std::min(wb, base[i] + T[i]);
+        k_vmark<<<(uint32_t)std::min<uint64_t>(hi - lo, 4096), VB, 0, s>>>(
+            items[i].as<VItem>(), *nitems(i), blk[i].as<uint64_t>(), base[i], wa, wb, i == 2, tc.d_first(),
+            tc.d_last());
+        GEM_LAUNCH_CHECK("k_vmark");
       }
-      const uint64_t most = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
-      const uint32_t grd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(grid, most));
-      launch_kernels<true>(c, grd, dn, q + 3 * g, prof, prof, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
-      tc.done(g);
+      if (!tc.plan(wb - wa)) {
+        if (W == 1)
+          throw Error{GEM_E_LIMIT, "the table blocks read by one 256-row task do not fit the device cache"};
+        W = std::max<uint64_t>(1, W / 2);
+        continue;
+      }
+      const auto &G = tc.groups();
+      unsigned int *q = tc.queues();
+      for (size_t g = 0; g < G.size(); ++g) {
+        tc.issue(g);
+        uint64_t lo[3], hi[3];
+        for (int i = 0; i < 3; ++i) {
+          lo[i] = std::max(wa + G[g].a, base[i]);
+          hi[i] = std::min(wa + G[g].b, base[i] + T[i]);
+          if (lo[i] < hi[i]) {
+            lo[i] -= base[i];
+            hi[i] -= base[i];
+          } else {
+            lo[i] = hi[i] = 0;
+          }
+        }
+        const uint64_t most = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        const uint32_t grd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(grid, most));
+        launch_kernels<true>(c, grd, dn, q + 3 * g, prof, prof, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+        tc.done(g);
+      }
+      wa = wb;
+      W = std::min(total, 2 * W);
     }
     GEM_HIP_CHECK(hipMemcpyAsync(h_hdr, d, hdr_bytes, hipMemcpyDeviceToHost, s));
   }
