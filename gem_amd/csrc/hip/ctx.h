@@ -128,10 +128,10 @@ struct PinnedBuf {
     return *this;
   }
   ~PinnedBuf() { reset(); }
-  void alloc(size_t n) {
+  void alloc(size_t n, unsigned flags = hipHostMallocDefault) {
     reset();
     if (n == 0) return;
-    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+    hipError_t e = hipHostMalloc(&p, n, flags);
     if (e != hipSuccess) {
       p = nullptr;
       throw Error{GEM_E_NOMEM, "hipHostMalloc(" + std::to_string(n) + " bytes): " + hipGetErrorString(e)};

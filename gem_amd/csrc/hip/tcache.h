@@ -11,8 +11,9 @@
 //      first to its last use; when the blocks live at a single task do not
 //      fit, the iteration is marked and planned again in smaller windows of
 //      tasks, down to one task),
-//   3. for each group, copies the missing blocks host -> device on a second
-//      stream while the previous group's kernels run, rewrites their entries
+//   3. for each group, loads the missing blocks on a second stream while the
+//      previous group's kernels run (one kernel reads them from pinned host
+//      memory over the host link into their slots), rewrites their entries
 //      of the block map (vmap: block -> device address of its values, minus
 //      the block's first offset, so that address = vmap[block] + 4 * offset),
 //      and lets the group's kernels start once its copies are done.
@@ -20,6 +21,8 @@
 // resident stays inside device memory (and shows up as a wrong count, never
 // as a fault).  The cache keeps blocks across iterations (least recently used
 // ones are replaced first; blocks needed later in the same iteration last).
+// Slots come in two size classes, so that small blocks (most tables of a
+// sparse query) do not each pin a large slot.
 #pragma once
 
 #include <vector>
@@ -37,19 +40,20 @@ void spill_tables(gem_ctx *c, QueryDev &Qd, uint64_t slot_bytes);
 class TableCache {
  public:
   struct Group {
-    uint64_t a = 0, b = 0;  // task range [a, b) of the iteration
+    uint64_t a = 0, b = 0;  // task range [a, b) of the window
     bool serial = false;    // its copies wait for the previous group's kernels
     size_t p0 = 0, p1 = 0;  // its entries in the patch list
-    std::vector<std::pair<uint64_t, uint32_t>> loads;  // (block, slot)
+    size_t l0 = 0, l1 = 0;  // its entries in the load list
   };
+  static constexpr uint64_t SMALL_SLOT = 64 << 10;  // size class of small blocks
 
   TableCache(gem_ctx *c, uint64_t cache_bytes, uint64_t slot_bytes);
   ~TableCache();
   TableCache(const TableCache &) = delete;
   TableCache &operator=(const TableCache &) = delete;
 
-  // Assign global block ids to the queries' tables, then patch and upload
-  // their level descriptors (vmap, kshift, gblk).
+  // Assign global block ids to the queries' tables, size the two slot pools,
+  // then patch and upload the level descriptors (vmap, kshift, gblk).
   void attach(std::vector<VQueryState *> &qs);
 
   uint32_t *d_first() const { return first_.as<uint32_t>(); }
@@ -65,7 +69,7 @@ class TableCache {
   const std::vector<Group> &groups() const { return groups_; }
   // Device queue counters for the group launches (3 per group), zeroed.
   unsigned int *queues();
-  // Copy stream: group g's block copies and map updates; the context stream
+  // Copy stream: group g's block loads and map updates; the context stream
   // waits for them.
   void issue(size_t g);
   // Context stream: mark the end of group g's kernels.
@@ -75,29 +79,31 @@ class TableCache {
 
  private:
   gem_ctx *c_;
-  uint64_t slot_bytes_;
-  uint32_t nslots_;
+  uint64_t cache_bytes_;
+  uint64_t sbytes_[2];      // slot bytes of the small and the large class
+  uint32_t nslots_[2] = {0, 0};
+  uint64_t cbase_[2] = {0, 0}, pool_bytes_ = 0;
   hipStream_t copy_ = nullptr;
-  DevBuf pool_;  // (nslots + 1) slots; the last is the zeroed guard
-  DevBuf vmap_, first_, last_, dpatch_, queues_;
-  PinnedBuf h_first_, h_last_, h_patch_;
+  DevBuf pool_;  // small slots | large slots | zeroed guard (one large slot)
+  DevBuf vmap_, first_, last_, dpatch_, dload_, queues_;
+  PinnedBuf h_first_, h_last_, h_patch_, h_load_;
   uint64_t nblk_ = 0;
-  std::vector<const uint32_t *> bsrc_;  // host address of block values
+  std::vector<uint64_t> bsrc_;          // device-visible address of block values
   std::vector<uint64_t> boff_, blen_;   // first offset and length (entries)
-  std::vector<uint32_t> blk_slot_;      // block -> slot or NONE
-  std::vector<uint64_t> slot_blk_;      // slot -> block or NONE
-  std::vector<uint64_t> stamp_;         // slot -> last use
+  std::vector<uint8_t> cls_;            // block -> size class
+  std::vector<uint32_t> blk_slot_;      // block -> slot (in its class) or NONE
+  std::vector<uint64_t> slot_blk_[2];   // slot -> block or NONE
+  std::vector<uint64_t> stamp_[2];      // slot -> last use
   std::vector<int64_t> live_in_;        // block -> id of the last group it was live in
-  int64_t gbase_ = 0;                   // id of this iteration's first group
+  int64_t gbase_ = 0;                   // id of the window's first group
   uint64_t clock_ = 0;
   std::vector<Group> groups_;
   std::vector<hipEvent_t> ev_;          // 3 per group: copy begin, copy end, kernels end
   hipEvent_t ev_step0_ = nullptr;
-  size_t npatch_ = 0;
   uint64_t hits_ = 0;
   void collect();
   uint64_t guard_addr() const;
-  uint64_t slot_addr(uint32_t s) const;
+  uint64_t slot_addr(int cls, uint32_t s) const;
 };
 
 }  // namespace gemd

@@ -17,6 +17,23 @@ struct Patch {
   uint64_t idx, val;
 };
 
+// One block to bring in: `words` u32 values from src (device-visible address
+// of pinned host memory) to dst (a cache slot).
+struct Load {
+  uint64_t src, dst, words;
+};
+
+// A workgroup per block: consecutive lanes read consecutive words, so each
+// wave instruction reads 256 contiguous bytes of host memory over the link.
+__global__ void __launch_bounds__(256) k_load_blocks(const Load *__restrict__ ld, uint64_t n) {
+  for (uint64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const Load L = ld[i];
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(L.src);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(L.dst);
+    for (uint64_t k = threadIdx.x; k < L.words; k += blockDim.x) dst[k] = src[k];
+  }
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = v;
@@ -38,7 +55,8 @@ void spill_tables(gem_ctx *c, QueryDev &Qd, uint64_t slot_bytes) {
   uint64_t total = 0;
   for (const TableDev &T : Qd.tab)
     if (T.present) total += T.size * 4;
-  Qd.host_vals.alloc(std::max<uint64_t>(total, 4));
+  // mapped: the cache's load kernel reads blocks straight from it
+  Qd.host_vals.alloc(std::max<uint64_t>(total, 4), hipHostMallocMapped);
   unsigned char *dst = Qd.host_vals.as<unsigned char>();
   for (TableDev &T : Qd.tab) {
     if (!T.present) continue;
@@ -74,15 +92,12 @@ void spill_tables(gem_ctx *c, QueryDev &Qd, uint64_t slot_bytes) {
 }
 
 // ------------------------------------------------------------------- cache
-TableCache::TableCache(gem_ctx *c, uint64_t cache_bytes, uint64_t slot_bytes) : c_(c), slot_bytes_(slot_bytes) {
-  nslots_ = (uint32_t)std::max<uint64_t>(cache_bytes / slot_bytes, 2);
-  hipStream_t s = c->stream;
-  pool_.alloc((uint64_t)(nslots_ + 1) * slot_bytes_, s);
-  GEM_HIP_CHECK(hipMemsetAsync(pool_.as<unsigned char>() + (uint64_t)nslots_ * slot_bytes_, 0, slot_bytes_, s));
+TableCache::TableCache(gem_ctx *c, uint64_t cache_bytes, uint64_t slot_bytes)
+    : c_(c), cache_bytes_(cache_bytes) {
+  sbytes_[0] = std::min<uint64_t>(SMALL_SLOT, slot_bytes);
+  sbytes_[1] = slot_bytes;
   GEM_HIP_CHECK(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking));
   GEM_HIP_CHECK(hipEventCreate(&ev_step0_));
-  slot_blk_.assign(nslots_, NONE64);
-  stamp_.assign(nslots_, 0);
 }
 
 TableCache::~TableCache() {
@@ -96,28 +111,33 @@ TableCache::~TableCache() {
 }
 
 uint64_t TableCache::guard_addr() const {
-  return (uint64_t)(uintptr_t)(pool_.as<unsigned char>() + (uint64_t)nslots_ * slot_bytes_);
+  return (uint64_t)(uintptr_t)(pool_.as<unsigned char>() + pool_bytes_ - sbytes_[1]);
 }
-uint64_t TableCache::slot_addr(uint32_t s) const {
-  return (uint64_t)(uintptr_t)(pool_.as<unsigned char>() + (uint64_t)s * slot_bytes_);
+uint64_t TableCache::slot_addr(int cls, uint32_t s) const {
+  return (uint64_t)(uintptr_t)(pool_.as<unsigned char>() + cbase_[cls] + (uint64_t)s * sbytes_[cls]);
 }
 
 void TableCache::attach(std::vector<VQueryState *> &qs) {
   hipStream_t s = c_->stream;
-  // global block ids, table by table
+  // global block ids, table by table; blocks keep the device-visible address
+  // of their values in pinned host memory
   std::vector<std::vector<uint64_t>> base(qs.size());
   nblk_ = 0;
   for (size_t i = 0; i < qs.size(); ++i) {
     const QueryDev &Qd = *qs[i]->qd;
     if (!Qd.streamed) throw Error{GEM_E_STATE, "streamed join over tables that were not spilled"};
+    void *dev_vals = nullptr;
+    GEM_HIP_CHECK(hipHostGetDevicePointer(&dev_vals, Qd.host_vals.p, 0));
     base[i].assign(Qd.tab.size(), 0);
     for (size_t d = 0; d < Qd.tab.size(); ++d) {
       const TableDev &T = Qd.tab[d];
       if (!T.present) continue;
       base[i][d] = nblk_;
       const uint64_t nb = T.blk_off.size() - 1;
+      const uint64_t toff = (uint64_t)(reinterpret_cast<const unsigned char *>(T.hval) -
+                                       Qd.host_vals.as<const unsigned char>());
       for (uint64_t b = 0; b < nb; ++b) {
-        bsrc_.push_back(T.hval + T.blk_off[b]);
+        bsrc_.push_back((uint64_t)(uintptr_t)dev_vals + toff + 4 * T.blk_off[b]);
         boff_.push_back(T.blk_off[b]);
         blen_.push_back(T.blk_off[b + 1] - T.blk_off[b]);
       }
@@ -125,6 +145,37 @@ void TableCache::attach(std::vector<VQueryState *> &qs) {
     }
   }
   if (nblk_ >= NONE32) throw Error{GEM_E_LIMIT, "too many table blocks for one batch"};
+  // size classes: blocks of at most SMALL_SLOT bytes take small slots; the
+  // cache is split between the classes in proportion to their table bytes,
+  // after each class has a few slots of its own
+  cls_.assign(nblk_, 0);
+  uint64_t cbytes[2] = {0, 0}, cblocks[2] = {0, 0};
+  for (uint64_t b = 0; b < nblk_; ++b) {
+    cls_[b] = 4 * blen_[b] <= sbytes_[0] ? 0 : 1;
+    cbytes[cls_[b]] += 4 * blen_[b];
+    cblocks[cls_[b]] += 1;
+  }
+  uint64_t need_slots[2], rest = cache_bytes_;
+  for (int k = 0; k < 2; ++k) {
+    need_slots[k] = std::min<uint64_t>(cblocks[k], 64);
+    if (need_slots[k] * sbytes_[k] > rest) throw Error{GEM_E_LIMIT, "the device table cache is too small"};
+    rest -= need_slots[k] * sbytes_[k];
+  }
+  const double f0 = cbytes[0] + cbytes[1] ? (double)cbytes[0] / (double)(cbytes[0] + cbytes[1]) : 0.5;
+  for (int k = 0; k < 2; ++k) {
+    const double share = k == 0 ? f0 : 1.0 - f0;
+    nslots_[k] = (uint32_t)std::min<uint64_t>(need_slots[k] + (uint64_t)(share * (double)rest) / sbytes_[k],
+                                               std::max<uint64_t>(cblocks[k], 1));
+  }
+  cbase_[0] = 0;
+  cbase_[1] = (uint64_t)nslots_[0] * sbytes_[0];
+  pool_bytes_ = cbase_[1] + (uint64_t)nslots_[1] * sbytes_[1] + sbytes_[1];  // + the guard slot
+  pool_.alloc(pool_bytes_, s);
+  GEM_HIP_CHECK(hipMemsetAsync(pool_.as<unsigned char>() + pool_bytes_ - sbytes_[1], 0, sbytes_[1], s));
+  for (int k = 0; k < 2; ++k) {
+    slot_blk_[k].assign(nslots_[k], NONE64);
+    stamp_[k].assign(nslots_[k], 0);
+  }
   blk_slot_.assign(nblk_, NONE32);
   live_in_.assign(nblk_, -1);
   const uint64_t n = std::max<uint64_t>(nblk_, 1);
@@ -172,57 +223,68 @@ bool TableCache::plan(uint64_t total) {
   }
   GEM_HIP_CHECK(hipStreamSynchronize(s));
   collect();  // the previous window's groups have finished
-  npatch_ = 0;
   hits_ = 0;
   if (total == 0) return true;
   const uint32_t *fi = h_first_.as<uint32_t>(), *la = h_last_.as<uint32_t>();
-  // needed blocks (empty blocks are never copied: their lists read nothing)
+  // needed blocks (empty blocks are never copied: their lists read nothing),
+  // counted per size class
   std::vector<uint64_t> need;
-  std::vector<uint32_t> scount(total + 1, 0), ecount(total + 1, 0);
+  std::vector<uint32_t> scount[2], ecount[2];
+  for (int k = 0; k < 2; ++k) {
+    scount[k].assign(total + 1, 0);
+    ecount[k].assign(total + 1, 0);
+  }
   for (uint64_t b = 0; b < nblk_; ++b) {
     if (fi[b] == NONE32 || blen_[b] == 0) continue;
     if (fi[b] >= total || la[b] >= total || la[b] < fi[b]) throw Error{GEM_E_STATE, "block marks out of range"};
     need.push_back(b);
-    scount[fi[b] + 1]++;
-    ecount[la[b] + 1]++;
+    scount[cls_[b]][fi[b] + 1]++;
+    ecount[cls_[b]][la[b] + 1]++;
   }
   // S[t] = #needed blocks first read before task t, E[t] = #last read before t
-  for (uint64_t t = 1; t <= total; ++t) {
-    scount[t] += scount[t - 1];
-    ecount[t] += ecount[t - 1];
-  }
-  auto S = [&](uint64_t t) { return (uint64_t)scount[t]; };
-  auto E = [&](uint64_t t) { return (uint64_t)ecount[t]; };
+  for (int k = 0; k < 2; ++k)
+    for (uint64_t t = 1; t <= total; ++t) {
+      scount[k][t] += scount[k][t - 1];
+      ecount[k][t] += ecount[k][t - 1];
+    }
+  // blocks first read before b and last read at or after lo fit every class
+  auto fits = [&](uint64_t b, uint64_t lo) {
+    for (int k = 0; k < 2; ++k)
+      if ((uint64_t)scount[k][b] - ecount[k][lo] > nslots_[k]) return false;
+    return true;
+  };
   // groups: the blocks live in the previous and the current group fit the
   // cache together (copies overlap the previous group's kernels); at least
   // four groups when everything fits, so copies still overlap
+  groups_.clear();
   const uint64_t target = std::max<uint64_t>(1, (total + 3) / 4);
   uint64_t a_prev = 0, a = 0;
   while (a < total) {
     Group G;
     G.a = a;
     uint64_t lo = a_prev;
-    if (S(a + 1) - E(lo) > nslots_) {
+    if (!fits(a + 1, lo)) {
       G.serial = true;
       lo = a;
       // the blocks live at task a (used at or before it and at or after it)
       // cannot all be resident: the caller plans a smaller window
-      if (S(a + 1) - E(a) > nslots_) {
+      if (!fits(a + 1, a)) {
         groups_.clear();
         return false;
       }
     }
     uint64_t b = a + 1;
-    while (b < total && b - a < target && S(b + 1) - E(lo) <= nslots_) ++b;
+    while (b < total && b - a < target && fits(b + 1, lo)) ++b;
     G.b = b;
     groups_.push_back(std::move(G));
     a_prev = a;
     a = b;
   }
   // simulate the cache over the groups: which blocks each group copies and
-  // which it evicts (every change is one map patch)
+  // which it evicts (every change is one map patch, every copy one load)
   std::sort(need.begin(), need.end(), [&](uint64_t x, uint64_t y) { return fi[x] < fi[y]; });
   std::vector<Patch> patches;
+  std::vector<Load> loads;
   std::vector<uint64_t> live;  // needed blocks first read before the group's end
   size_t nx = 0;
   for (size_t g = 0; g < groups_.size(); ++g) {
@@ -232,64 +294,75 @@ bool TableCache::plan(uint64_t total) {
     const int64_t gid = gbase_ + (int64_t)g;  // unique over the run's iterations
     for (uint64_t b : live) live_in_[b] = gid;
     ++clock_;
-    std::vector<uint64_t> miss;
+    std::vector<uint64_t> miss[2];
     for (uint64_t b : live) {
       if (blk_slot_[b] != NONE32) {
-        stamp_[blk_slot_[b]] = clock_;
+        stamp_[cls_[b]][blk_slot_[b]] = clock_;
         ++hits_;
       } else {
-        miss.push_back(b);
+        miss[cls_[b]].push_back(b);
       }
     }
     G.p0 = patches.size();
-    if (!miss.empty()) {
+    G.l0 = loads.size();
+    for (int k = 0; k < 2; ++k) {
+      if (miss[k].empty()) continue;
       // victims: free slots, then slots whose block is not live in this or
       // (unless serial) the previous group: blocks not read again in this
-      // iteration first (least recently used first), then those read latest
+      // window first (least recently used first), then those read latest
       const int64_t keep_from = G.serial ? gid : gid - 1;
       std::vector<std::pair<uint64_t, uint32_t>> cand;  // (priority, slot)
-      for (uint32_t sl = 0; sl < nslots_; ++sl) {
-        const uint64_t b = slot_blk_[sl];
+      for (uint32_t sl = 0; sl < nslots_[k]; ++sl) {
+        const uint64_t b = slot_blk_[k][sl];
         if (b == NONE64) {
           cand.push_back({0, sl});
           continue;
         }
         if (live_in_[b] >= keep_from) continue;
         const bool later = fi[b] != NONE32 && la[b] >= G.b;
-        cand.push_back({later ? (2ull << 62) - fi[b] : (1ull << 62) + stamp_[sl], sl});
+        cand.push_back({later ? (2ull << 62) - fi[b] : (1ull << 62) + stamp_[k][sl], sl});
       }
-      if (cand.size() < miss.size()) throw Error{GEM_E_STATE, "table cache plan ran out of slots"};
-      std::partial_sort(cand.begin(), cand.begin() + miss.size(), cand.end());
-      for (size_t k = 0; k < miss.size(); ++k) {
-        const uint32_t sl = cand[k].second;
-        const uint64_t old = slot_blk_[sl];
+      if (cand.size() < miss[k].size()) throw Error{GEM_E_STATE, "table cache plan ran out of slots"};
+      std::partial_sort(cand.begin(), cand.begin() + miss[k].size(), cand.end());
+      for (size_t m = 0; m < miss[k].size(); ++m) {
+        const uint32_t sl = cand[m].second;
+        const uint64_t old = slot_blk_[k][sl];
         if (old != NONE64) {
           blk_slot_[old] = NONE32;
           patches.push_back({old, guard_addr() - 4 * boff_[old]});
         }
-        const uint64_t b = miss[k];
-        slot_blk_[sl] = b;
+        const uint64_t b = miss[k][m];
+        slot_blk_[k][sl] = b;
         blk_slot_[b] = sl;
-        stamp_[sl] = clock_;
-        patches.push_back({b, slot_addr(sl) - 4 * boff_[b]});
-        G.loads.push_back({b, sl});
+        stamp_[k][sl] = clock_;
+        patches.push_back({b, slot_addr(k, sl) - 4 * boff_[b]});
+        loads.push_back({bsrc_[b], slot_addr(k, sl), blen_[b]});
       }
     }
     G.p1 = patches.size();
+    G.l1 = loads.size();
   }
-  // one unused id between iterations: the previous iteration's last group is
-  // never "the previous group" of this one's first
+  // one unused id between windows: the previous window's last group is never
+  // "the previous group" of this one's first
   gbase_ += (int64_t)groups_.size() + 1;
-  npatch_ = patches.size();
-  // patch lists: pinned (async copy source) and device copies; growing them
-  // is safe here because the previous iteration has been synchronised
-  const size_t need_bytes = std::max<size_t>(npatch_, 1) * sizeof(Patch);
-  if (h_patch_.bytes < need_bytes) h_patch_.alloc(need_bytes * 2);
-  if (dpatch_.bytes < need_bytes) {
-    dpatch_.alloc(need_bytes * 2, s);
-    GEM_HIP_CHECK(hipStreamSynchronize(s));
+  // patch and load lists: pinned (async copy source) and device copies;
+  // growing them is safe here because the streams are past the last window
+  const size_t pbytes = std::max<size_t>(patches.size(), 1) * sizeof(Patch);
+  const size_t lbytes = std::max<size_t>(loads.size(), 1) * sizeof(Load);
+  if (h_patch_.bytes < pbytes) h_patch_.alloc(pbytes * 2);
+  if (h_load_.bytes < lbytes) h_load_.alloc(lbytes * 2);
+  bool grown = false;
+  if (dpatch_.bytes < pbytes) {
+    dpatch_.alloc(pbytes * 2, s);
+    grown = true;
   }
-  if (npatch_) std::memcpy(h_patch_.p, patches.data(), npatch_ * sizeof(Patch));
+  if (dload_.bytes < lbytes) {
+    dload_.alloc(lbytes * 2, s);
+    grown = true;
+  }
+  if (grown) GEM_HIP_CHECK(hipStreamSynchronize(s));
+  if (!patches.empty()) std::memcpy(h_patch_.p, patches.data(), patches.size() * sizeof(Patch));
+  if (!loads.empty()) std::memcpy(h_load_.p, loads.data(), loads.size() * sizeof(Load));
   // events: 3 per group
   while (ev_.size() < 3 * groups_.size()) {
     hipEvent_t e;
@@ -316,13 +389,16 @@ void TableCache::issue(size_t g) {
   if (g == 0) GEM_HIP_CHECK(hipStreamWaitEvent(copy_, ev_step0_, 0));
   if (wait_for >= 0) GEM_HIP_CHECK(hipStreamWaitEvent(copy_, ev_[3 * wait_for + 2], 0));
   GEM_HIP_CHECK(hipEventRecord(ev_[3 * g], copy_));
-  for (const auto &ld : G.loads) {
-    const uint64_t b = ld.first;
-    GEM_HIP_CHECK(hipMemcpyAsync(pool_.as<unsigned char>() + (uint64_t)ld.second * slot_bytes_, bsrc_[b],
-                                 4 * blen_[b], hipMemcpyHostToDevice, copy_));
-    c_->sstats.h2d_bytes += 4 * blen_[b];
+  const size_t nl = G.l1 - G.l0;
+  if (nl) {
+    // the group's blocks, read from pinned host memory by one kernel
+    Load *dl = dload_.as<Load>() + G.l0;
+    GEM_HIP_CHECK(hipMemcpyAsync(dl, h_load_.as<Load>() + G.l0, nl * sizeof(Load), hipMemcpyHostToDevice, copy_));
+    k_load_blocks<<<(uint32_t)std::min<size_t>(nl, 2048), 256, 0, copy_>>>(dl, nl);
+    GEM_LAUNCH_CHECK("k_load_blocks");
+    for (size_t i = G.l0; i < G.l1; ++i) c_->sstats.h2d_bytes += 4 * h_load_.as<Load>()[i].words;
   }
-  c_->sstats.blocks_loaded += G.loads.size();
+  c_->sstats.blocks_loaded += nl;
   const size_t np = G.p1 - G.p0;
   if (np) {
     Patch *dp = dpatch_.as<Patch>() + G.p0;
