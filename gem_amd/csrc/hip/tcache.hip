@@ -147,7 +147,8 @@ void TableCache::attach(std::vector<VQueryState *> &qs) {
   if (nblk_ >= NONE32) throw Error{GEM_E_LIMIT, "too many table blocks for one batch"};
   // size classes: blocks of at most SMALL_SLOT bytes take small slots; the
   // cache is split between the classes in proportion to their table bytes,
-  // after each class has a few slots of its own
+  // after each class has two slots of its own (with slots of at most
+  // SMALL_SLOT bytes every block is in the small class)
   cls_.assign(nblk_, 0);
   uint64_t cbytes[2] = {0, 0}, cblocks[2] = {0, 0};
   for (uint64_t b = 0; b < nblk_; ++b) {
@@ -157,7 +158,7 @@ void TableCache::attach(std::vector<VQueryState *> &qs) {
   }
   uint64_t need_slots[2], rest = cache_bytes_;
   for (int k = 0; k < 2; ++k) {
-    need_slots[k] = std::min<uint64_t>(cblocks[k], 64);
+    need_slots[k] = std::min<uint64_t>(cblocks[k], 2);
     if (need_slots[k] * sbytes_[k] > rest) throw Error{GEM_E_LIMIT, "the device table cache is too small"};
     rest -= need_slots[k] * sbytes_[k];
   }
