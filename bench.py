@@ -94,7 +94,9 @@ def main():
     ap.add_argument("--work-limit", type=float, default=None,
                     help="work a query's join may do before it is reported unfinished (default: 5e9 on rmat22 "
                          "and ooc24, 2e10 on rmat20)")
-    ap.add_argument("--group", type=int, default=0, help="queries resident at once in the batched path (0 = all)")
+    ap.add_argument("--group", type=int, default=None,
+                    help="queries resident at once in the batched path (0 = all; default: autotuned, else 0)")
+    ap.add_argument("--no-tuning", action="store_true", help="ignore gem_amd/tuning/<workload>.json")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path pass")
     ap.add_argument("--plain-sample", type=int, default=64,
                     help="run the plain path on a systematic sample of this many queries (every k-th; "
@@ -102,7 +104,8 @@ def main():
     ap.add_argument("--model", default=None,
                     help="trained order model (.npz); default: the one trained for the workload's graph "
                          "(gem_amd/models/order_mlp_<graph>.npz); absent = GEM-derived orders")
-    ap.add_argument("--orders", type=int, default=16, help="candidate orders scored per query")
+    ap.add_argument("--orders", type=int, default=None,
+                    help="candidate orders scored per query (default: autotuned, else 16)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-oracle work (0 = skip)")
     ap.add_argument("--cpu-work-limit", type=float, default=2e9,
                     help="intermediate rows after which one oracle query is abandoned")
@@ -138,6 +141,17 @@ def main():
     ctx = GemContext(0)
     ctx.load_data(g)
     ctx.set_work_limit(wl)
+    # the autotuner's choices for this workload, unless given on the command line
+    from gem_amd.autotune import apply, load_tuning
+    tuning = {} if args.no_tuning else load_tuning(os.path.join(ROOT, "gem_amd", "tuning", f"{args.workload}.json"))
+    apply(ctx, tuning)
+    if args.group is None:
+        args.group = int(tuning.get("group", 0))
+    if args.orders is None:
+        args.orders = int(tuning.get("max_orders", 16))
+    level_budget = int(tuning.get("level_budget", 0))
+    if tuning:
+        log(f"[bench] autotuned: {tuning}")
     if stream_cache:
         ctx.set_table_stream(stream_cache, parse_bytes(args.stream_slot))
         log(f"[bench] edge tables streamed from host memory through a {args.stream_cache} device cache")
@@ -156,6 +170,7 @@ def main():
 
     def batched_pass():
         return match_batch_learned(ctx, qs, model, max_orders=args.orders, cap=cap, work_limit=wl,
+                                   level_budget=level_budget,
                                    group=args.group)
 
     for w in range(args.warmup):
@@ -210,7 +225,8 @@ def main():
     gem_batched = None
     if model is not None:
         t = time.perf_counter()
-        res_g = match_batch_learned(ctx, qs, None, cap=cap, work_limit=wl, group=args.group)
+        res_g = match_batch_learned(ctx, qs, None, cap=cap, work_limit=wl, level_budget=level_budget,
+                                    group=args.group)
         tg = time.perf_counter() - t
         gem_batched = {"queries_per_s": round(len(qs) / tg, 4), "seconds": round(tg, 3),
                        "rows_processed": int(sum(r["rows_processed"] for r in res_g)),
@@ -243,6 +259,7 @@ def main():
         # the batched learned path over exactly the same queries, for a matched ratio
         t = time.perf_counter()
         matched = match_batch_learned(ctx, sample, model, max_orders=args.orders, cap=cap, work_limit=wl,
+                                      level_budget=level_budget,
                                       group=args.group)
         mt = time.perf_counter() - t
         both = [(batched[i], b) for i, b in zip(idx, plain_res)
@@ -312,7 +329,8 @@ def main():
                    "path": "batched vertex-at-a-time join, " + (
                        "learned order (MLP over candidate orders)" if model is not None else "GEM-derived order"),
                    "cap": cap,
-                   "work_limit_rows": wl},
+                   "work_limit": wl,
+                   "tuning": tuning or None},
         "embeddings_per_s": round(eps, 1),
         "saturated_queries": sum(r["saturated"] for r in batched),
         "unfinished_queries": sum(r["work_limited"] for r in batched),

@@ -1,6 +1,6 @@
 """Autotuner for the batched path.
 
-    python -m gem_amd.autotune --workload rmat20 --out gem_amd/tuning/rmat20.json
+    python -m gem_amd.autotune --workload rmat22 --out gem_amd/tuning/rmat22.json
 
 Times the batched matcher on a sample of the workload's queries for every
 combination of the tunables and keeps the fastest:
@@ -25,8 +25,8 @@ from .workloads import CAP, make_graph, make_queries
 
 SPACE = {
     "persistent_blocks": [1024, 2048, 4096],
-    "level_budget": [4 << 30, 16 << 30, 48 << 30],
-    "group": [0, 32],
+    "level_budget": [4 << 30, 16 << 30],
+    "group": [0, 24],
     "max_orders": [8, 16],
 }
 
@@ -43,23 +43,29 @@ def load_tuning(path):
 
 
 def apply(ctx, cfg):
+    """Set the context-level knobs of a tuning (the others are per-call
+    arguments: level_budget, group, max_orders)."""
     if "persistent_blocks" in cfg:
         ctx.set_persistent_blocks(cfg["persistent_blocks"])
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="rmat20")
-    ap.add_argument("--sample", type=int, default=40, help="queries timed per trial")
+    ap.add_argument("--workload", default="rmat22")
+    ap.add_argument("--sample", type=int, default=48, help="queries timed per trial (every k-th of the workload)")
     ap.add_argument("--cap", type=float, default=CAP)
-    ap.add_argument("--work-limit", type=float, default=2e10)
-    ap.add_argument("--model", default=os.path.join(os.path.dirname(__file__), "models", "order_mlp_rmat20.npz"))
+    ap.add_argument("--work-limit", type=float, default=5e9)
+    ap.add_argument("--model", default=None, help="order model (default: models/order_mlp_<workload>.npz)")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     out = args.out or os.path.join(os.path.dirname(__file__), "tuning", f"{args.workload}.json")
 
     g = make_graph(args.workload)
-    qs = make_queries(args.workload, g)[:args.sample]
+    allq = make_queries(args.workload, g)
+    k = max(1, len(allq) // args.sample)
+    qs = allq[::k][:args.sample]
+    if args.model is None:
+        args.model = os.path.join(os.path.dirname(__file__), "models", f"order_mlp_{args.workload}.npz")
     ctx = GemContext(0)
     ctx.load_data(g)
     model = OrderModel.load(ctx, args.model) if os.path.exists(args.model) else None
@@ -82,6 +88,7 @@ def main():
         same = all(a == b for a, b in zip(sig, ref) if not a[2] and not b[2])
         trials.append({**cfg, "seconds": dt, "counts_equal": same})
         log(f"[tune] {cfg} -> {dt:.3f}s counts_equal={same}")
+        apply(ctx, {"persistent_blocks": 2048})
     ok = [t for t in trials if t["counts_equal"]]
     best = min(ok, key=lambda t: t["seconds"])
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
