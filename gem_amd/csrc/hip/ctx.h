@@ -197,13 +197,16 @@ enum KernelId {
   K_VEXT_WRITE,
   K_VEXT_FINAL,
   K_VCOUNT1,
+  K_VEXT_HEAVY_WRITE,
+  K_VEXT_HEAVY_FINAL,
   K_NUM
 };
 inline const char *kernel_name(int id) {
   static const char *n[K_NUM] = {"k_filter", "k_filter_heavy", "k_compact", "k_tables<count>",
                                  "k_tables<write>",    "k_step_count<expand>", "k_step_count<select>",
                                  "k_step_count<expand,final>", "k_step_count<select,final>", "k_expand_write",
-                                 "k_select_write", "k_vlen", "k_vext<write>", "k_vext<final>", "k_vcount1"};
+                                 "k_select_write", "k_vlen", "k_vext<write>", "k_vext<final>", "k_vcount1",
+                                 "k_vext_heavy<write>", "k_vext_heavy<final>"};
   return n[id];
 }
 struct KStat {

@@ -11,24 +11,6 @@ constexpr uint32_t SCAN_THREADS = 256;
 constexpr uint32_t SCAN_ITEMS = 16;
 constexpr uint32_t SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
-// block-wide exclusive scan of one u64 per thread; returns block total
-__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t x, uint64_t *excl, uint64_t *smem /*[8]*/) {
-  const uint32_t t = threadIdx.x, w = t / 64, l = t % 64;
-  const uint32_t nw = blockDim.x / 64;
-  uint64_t inc = wave_incl_scan_u64(x);
-  if (l == 63) smem[w] = inc;
-  __syncthreads();
-  uint64_t wbase = 0, tot = 0;
-  for (uint32_t i = 0; i < nw; ++i) {
-    uint64_t s = smem[i];
-    if (i < w) wbase += s;
-    tot += s;
-  }
-  __syncthreads();
-  *excl = wbase + inc - x;
-  return tot;
-}
-
 __global__ void __launch_bounds__(SCAN_THREADS) k_tile_sums_u64(const uint64_t *in, uint64_t n, uint64_t *sums) {
   __shared__ uint64_t sm[8];
   const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;

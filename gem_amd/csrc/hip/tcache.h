@@ -67,7 +67,8 @@ class TableCache {
   // caller marks a smaller window of tasks and plans again.
   bool plan(uint64_t total_tasks);
   const std::vector<Group> &groups() const { return groups_; }
-  // Device queue counters for the group launches (3 per group), zeroed.
+  // Device counters for the group launches, zeroed: per group 3 work queues,
+  // 2 heavy-row queues and 2 heavy-row counts.
   unsigned int *queues();
   // Copy stream: group g's block loads and map updates; the context stream
   // waits for them.

@@ -376,7 +376,7 @@ bool TableCache::plan(uint64_t total) {
 
 unsigned int *TableCache::queues() {
   hipStream_t s = c_->stream;
-  const size_t need = 4 * 3 * std::max<size_t>(groups_.size(), 1);
+  const size_t need = 4 * 7 * std::max<size_t>(groups_.size(), 1);
   if (queues_.bytes < need) queues_.alloc(need * 2, s);
   GEM_HIP_CHECK(hipMemsetAsync(queues_.p, 0, need, s));
   return queues_.as<unsigned int>();

@@ -37,6 +37,13 @@ struct VItem {
   unsigned long long *counter;
 };
 
+// A row whose driving list is too long for one wave: walked in chunks by
+// k_vext_heavy (t = item of the launch, r = row of the item's level).
+struct HeavyRow {
+  uint32_t t, len;
+  uint64_t r;
+};
+
 struct VStackItem {
   uint32_t level;                  // the level these rows extend to next
   std::shared_ptr<DevBuf> rows;

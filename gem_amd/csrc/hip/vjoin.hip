@@ -105,21 +105,34 @@ __device__ __forceinline__ uint64_t list_addr(const VLevel &L, uint32_t j, uint3
 }
 
 // ------------------------------------------------------ extend (walk) pass
+// Each wave pulls 64-row tasks from the launch's queue on its own (no block
+// barrier).  For each of its rows the shortest backward list drives; a row
+// whose driving list is longer than heavy_t is not walked here but appended to
+// the heavy-row list (when it has room), which k_vext_heavy then walks in
+// fixed-size chunks spread over all waves: a hub row no longer holds one wave
+// for the whole launch.
 // Dynamic LDS per wave: other lists (nb-1) x 64 x (u64 address + u32 len),
 // dup positions ndup x 64 u32.
+// Survivors are written as whole rows: their (parent row, new value) pairs are
+// staged in LDS, then each half-wave copies one row (k + 1 <= 32 consecutive
+// words).
 // The item count is read from device memory so that one captured graph can
 // replay the launch for any number of items.  Tasks [g_begin, g_end) of the
-// launch are processed (streamed tables run a launch in task groups).
+// launch (256-row units) are processed (streamed tables run a launch in task
+// groups).
 template <bool FINAL, bool OOC>
 __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p,
                                             const uint64_t *__restrict__ blk_prefix, unsigned int *__restrict__ queue,
                                             uint32_t max_nb, uint32_t max_ndup,
-                                            unsigned long long *__restrict__ prof_items, uint64_t g_begin,
-                                            uint64_t g_end) {
+                                            unsigned long long *__restrict__ prof_items,
+                                            unsigned long long *__restrict__ prof_out, uint64_t g_begin,
+                                            uint64_t g_end, HeavyRow *__restrict__ heavy,
+                                            unsigned int *__restrict__ heavy_n, uint32_t heavy_cap,
+                                            uint32_t heavy_t) {
   const uint32_t nitems = *nitems_p;
-  __shared__ uint32_t task;
   __shared__ uint32_t st[VW][65];
   __shared__ uint64_t drv[VW][64];  // address of each row's driving list
+  __shared__ uint32_t sv_row[VW][64], sv_val[VW][64];  // this step's survivors
   extern __shared__ unsigned char vlds[];
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const uint32_t per_wave = 64 * ((max_nb > 0 ? max_nb - 1 : 0) * 12 + max_ndup * 4);
@@ -128,17 +141,19 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
   uint32_t *olen = reinterpret_cast<uint32_t *>(mine + (size_t)64 * 8 * (max_nb - 1));   // [nb-1][64]
   uint32_t *dpos = reinterpret_cast<uint32_t *>(mine + (size_t)64 * 12 * (max_nb - 1));  // [ndup][64]
   const uint64_t total = blk_prefix[nitems] < g_end ? blk_prefix[nitems] : g_end;
+  uint64_t walked = 0, written = 0;
   for (;;) {
-    if (threadIdx.x == 0) task = atomicAdd(queue, 1u);
-    __syncthreads();
-    const uint64_t g = g_begin + task;
-    __syncthreads();
-    if (g >= total) break;
+    uint32_t wt = 0;
+    if (l == 0) wt = atomicAdd(queue, 1u);
+    wt = readlane_u32(wt, 0);
+    const uint64_t q = g_begin * VW + wt;  // 64-row task
+    if (q >= total * VW) break;
+    const uint64_t g = q / VW;
     const uint32_t t = find_item(blk_prefix, nitems, g);
     const VItem it = items[t];
     const VLevel &L = *it.lv;
     const uint32_t nb = L.nb, ndup = L.ndup, k = it.k;
-    const uint64_t rbase = it.row_begin + (g - blk_prefix[t]) * VB + w * 64;
+    const uint64_t rbase = it.row_begin + (g - blk_prefix[t]) * VB + (q % VW) * 64;
     const uint64_t r = rbase + l;
     const bool valid = r < it.row_end;
     const uint32_t *row = it.rows + r * k;
@@ -156,6 +171,13 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
           dj = j;
           dbase = b;
           dp = p;
+        }
+      }
+      if (heavy && dlen > heavy_t) {
+        const uint32_t h = atomicAdd(heavy_n, 1u);
+        if (h < heavy_cap) {
+          heavy[h] = HeavyRow{t, dlen, r};
+          dlen = 0;  // walked by k_vext_heavy
         }
       }
       daddr = list_addr<OOC>(L, dj, dp, dbase);
@@ -180,6 +202,7 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
     drv[w][l] = daddr;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    walked += wtotal;
     uint64_t survivors = 0;
     for (uint32_t s = 0; s < wtotal; s += 64) {
       const uint32_t item = s + l;
@@ -203,21 +226,154 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
         uint64_t base = 0;
         if (l == 0) base = atomicAdd(it.counter, (unsigned long long)nk);
         base = readlane_u64(base, 0);
+        written += (uint64_t)nk * 4 * (k + 1);
+        if (keep) {
+          const uint32_t i = rank_below(m);
+          sv_row[w][i] = rw;
+          sv_val[w][i] = val;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // half-wave h copies survivor i + h: k parent columns, then the value
+        const uint32_t h = l >> 5, c = l & 31;
+        for (uint32_t i = 0; i < nk; i += 2) {
+          const uint32_t sidx = i + h;
+          const uint64_t o = base + sidx;
+          if (sidx < nk && c <= k && o < it.out_cap) {
+            const uint32_t v = c < k ? it.rows[(rbase + sv_row[w][sidx]) * k + c] : sv_val[w][sidx];
+            it.out[o * (k + 1) + c] = v;
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+    }
+    if (FINAL && l == 0 && survivors) atomicAdd(it.counter, (unsigned long long)survivors);
+    // st / drv / LDS lists of this wave are rewritten by its next task
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  if (prof_items && l == 0 && walked) atomicAdd(prof_items, (unsigned long long)walked);
+  if (prof_out && l == 0 && written) atomicAdd(prof_out, (unsigned long long)written);
+}
+
+// One block: chunk prefix of the heavy rows, pre[i] = sum over rows before i
+// of ceil(len / chunk), pre[n] = total, n = min(heavy_n, heavy_cap).
+__global__ void __launch_bounds__(1024) k_heavy_scan(const HeavyRow *__restrict__ heavy,
+                                                    const unsigned int *__restrict__ heavy_n, uint32_t heavy_cap,
+                                                    uint32_t chunk, uint64_t *__restrict__ pre) {
+  __shared__ uint64_t sm[16];
+  const uint32_t n = *heavy_n < heavy_cap ? *heavy_n : heavy_cap;
+  uint64_t carry = 0;
+  for (uint32_t b = 0; b < n; b += blockDim.x) {
+    const uint32_t i = b + threadIdx.x;
+    const uint64_t x = i < n ? (heavy[i].len + chunk - 1) / chunk : 0;
+    uint64_t ex;
+    const uint64_t tot = block_excl_scan_u64(x, &ex, sm);
+    if (i < n) pre[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) pre[n] = carry;
+}
+
+// Heavy rows: each wave task walks `chunk` entries of one row's driving list
+// (the same list the main pass picked: shortest, first on ties), probing the
+// row's other lists and equal-label columns; survivors are counted or appended
+// to the row's item output like the main pass's.
+template <bool FINAL, bool OOC>
+__global__ void __launch_bounds__(VB) k_vext_heavy(const VItem *__restrict__ items,
+                                                  const HeavyRow *__restrict__ heavy,
+                                                  const unsigned int *__restrict__ heavy_n, uint32_t heavy_cap,
+                                                  const uint64_t *__restrict__ pre, unsigned int *__restrict__ queue,
+                                                  uint32_t chunk, unsigned long long *__restrict__ prof_items,
+                                                  unsigned long long *__restrict__ prof_out) {
+  __shared__ uint64_t oaddr[VW][GEM_MAX_QV];
+  __shared__ uint32_t olen[VW][GEM_MAX_QV], dpos[VW][GEM_MAX_QV];
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const uint32_t n = *heavy_n < heavy_cap ? *heavy_n : heavy_cap;
+  const uint64_t total = pre[n];
+  uint64_t walked = 0, written = 0;
+  for (;;) {
+    uint32_t wt = 0;
+    if (l == 0) wt = atomicAdd(queue, 1u);
+    wt = readlane_u32(wt, 0);
+    if (wt >= total) break;
+    const uint32_t h = find_item(pre, n, wt);
+    const HeavyRow e = heavy[h];
+    const VItem it = items[e.t];
+    const VLevel &L = *it.lv;
+    const uint32_t nb = L.nb, ndup = L.ndup, k = it.k;
+    const uint32_t *row = it.rows + e.r * k;
+    // the row's lists (every lane computes the same; lane j keeps list j)
+    uint32_t dlen = 0xFFFFFFFFu, dj = 0, dp = 0;
+    uint64_t dbase = 0;
+    for (uint32_t j = 0; j < nb; ++j) {
+      const uint32_t p = row[L.key_col[j]];
+      const uint64_t b = L.toff[j][p];
+      const uint32_t len = (uint32_t)(L.toff[j][p + 1] - b);
+      if (len < dlen) {
+        dlen = len;
+        dj = j;
+        dbase = b;
+        dp = p;
+      }
+    }
+    const uint32_t *dlist = reinterpret_cast<const uint32_t *>(list_addr<OOC>(L, dj, dp, dbase));
+    if (l < nb) {
+      const uint32_t j = l;
+      const uint32_t p = row[L.key_col[j]];
+      const uint64_t b = L.toff[j][p];
+      const uint32_t o = j < dj ? j : j - 1;  // other lists, in order, without the driver
+      if (j != dj) {
+        oaddr[w][o] = list_addr<OOC>(L, j, p, b);
+        olen[w][o] = (uint32_t)(L.toff[j][p + 1] - b);
+      }
+    }
+    if (l < ndup) dpos[w][l] = vposition_in(L.bm_new, L.rank_new, L.dup_can[l][row[L.dup_col[l]]]);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint64_t s0 = (uint64_t)(wt - pre[h]) * chunk;
+    const uint64_t s1 = s0 + chunk < dlen ? s0 + chunk : dlen;
+    walked += s1 - s0;
+    uint64_t survivors = 0;
+    for (uint64_t s = s0; s < s1; s += 64) {
+      const uint64_t item = s + l;
+      const bool iv = item < s1;
+      bool keep = false;
+      uint32_t val = 0;
+      if (iv) {
+        val = dlist[item];
+        keep = true;
+        for (uint32_t c = 0; c < ndup && keep; ++c)
+          if (dpos[w][c] == val) keep = false;
+        for (uint32_t o = 0; o + 1 < nb && keep; ++o)
+          keep = sorted_contains(reinterpret_cast<const uint32_t *>(oaddr[w][o]), olen[w][o], val);
+      }
+      const uint64_t m = __ballot(keep);
+      const uint32_t nk = (uint32_t)__popcll(m);
+      if (FINAL) {
+        survivors += nk;
+      } else if (nk) {
+        uint64_t base = 0;
+        if (l == 0) base = atomicAdd(it.counter, (unsigned long long)nk);
+        base = readlane_u64(base, 0);
+        written += (uint64_t)nk * 4 * (k + 1);
         if (keep) {
           const uint64_t o = base + rank_below(m);
           if (o < it.out_cap) {
-            const uint32_t *src = it.rows + (rbase + rw) * k;
             uint32_t *dst = it.out + o * (k + 1);
-            for (uint32_t c = 0; c < k; ++c) dst[c] = src[c];
+            for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
             dst[k] = val;
           }
         }
       }
     }
     if (FINAL && l == 0 && survivors) atomicAdd(it.counter, (unsigned long long)survivors);
-    if (prof_items && l == 0 && wtotal) atomicAdd(prof_items, (unsigned long long)wtotal);
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
+  if (prof_items && l == 0 && walked) atomicAdd(prof_items, (unsigned long long)walked);
+  if (prof_out && l == 0 && written) atomicAdd(prof_out, (unsigned long long)written);
 }
 
 // Last level, one backward list: |list| - equal-label columns present.
@@ -374,18 +530,30 @@ uint32_t grid_for(uint64_t blocks) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks, g_persistent_blocks));
 }
 
+// Rows whose driving list has more entries than HEAVY_T are walked by
+// k_vext_heavy, HEAVY_CHUNK entries per wave task; at most HEAVY_CAP such rows
+// per launch (a row that finds the list full is walked by its own wave).
+constexpr uint32_t HEAVY_T = 1024, HEAVY_CHUNK = 1024, HEAVY_CAP = 1u << 20;
+
 // Fixed buffers for the extend step of every iteration of one run: item
 // tables and block prefixes of the three launches (write, final walk, final
-// single-list count), and a header with the item counts, work counters and
-// per-task result counters.  Host side: pinned mirrors.  Because every
-// pointer, grid and LDS size is fixed for the run, the step
-//   H2D header, items, prefixes -> 3 kernels -> D2H header
+// single-list count), the heavy-row lists of the two walking launches, and a
+// header with the item counts, work queues, heavy-row counts, walked-entry
+// counts and per-task result counters.  Host side: pinned mirrors.  Because
+// every pointer, grid and LDS size is fixed for the run, the step
+//   H2D header, items, prefixes -> 3 kernels -> 2 x (heavy scan, heavy walk)
+//   -> D2H header
 // is captured once as a hipGraph and replayed each iteration (no memset node:
 // the zeroed counters travel in the header copy).
 struct ExtSlot {
-  static constexpr size_t HDR_NITEMS = 0, HDR_QUEUE = 16, HDR_PITEMS = 32, HDR_CTR = 48;
+  // header: item counts [3], queues [3 main + 2 heavy], heavy-row counts [2],
+  // profile counters [6] (u64: entries walked by the write / final / heavy
+  // write / heavy final kernels, bytes written by the write / heavy write
+  // kernels), result counters [cap] (u64)
+  static constexpr size_t HDR_NITEMS = 0, HDR_QUEUE = 16, HDR_HEAVYN = 40, HDR_PITEMS = 48, HDR_CTR = 96;
   uint32_t cap = 0;  // items per launch = counters
   DevBuf items[3], blk[3], hdr;
+  DevBuf heavy[2], hpre[2];  // heavy rows and their chunk prefix, per walking launch
   VItem *h_items[3] = {};
   uint64_t *h_blk[3] = {};
   unsigned char *h_hdr = nullptr;
@@ -406,6 +574,10 @@ struct ExtSlot {
     grid = g_persistent_blocks;
     hdr_bytes = HDR_CTR + 8ull * cap;
     hdr.alloc(hdr_bytes, s);
+    for (int i = 0; i < 2; ++i) {
+      heavy[i].alloc(sizeof(HeavyRow) * HEAVY_CAP, s);
+      hpre[i].alloc(8ull * (HEAVY_CAP + 1), s);
+    }
     GEM_HIP_CHECK(hipHostMalloc((void **)&h_hdr, hdr_bytes, hipHostMallocDefault));
     for (int i = 0; i < 3; ++i) {
       items[i].alloc(sizeof(VItem) * cap, s);
@@ -444,33 +616,56 @@ struct ExtSlot {
     }
     const uint32_t *dn = reinterpret_cast<const uint32_t *>(d + HDR_NITEMS);
     unsigned int *dq = reinterpret_cast<unsigned int *>(d + HDR_QUEUE);
-    launch_kernels<false>(c, grid, dn, dq, prof_kernels, count_items, 0, ~0ull, 0, ~0ull, 0, ~0ull);
+    unsigned int *hn = reinterpret_cast<unsigned int *>(d + HDR_HEAVYN);
+    launch_kernels<false>(c, grid, dn, dq, dq + 3, hn, prof_kernels, count_items, 0, ~0ull, 0, ~0ull, 0, ~0ull);
     GEM_HIP_CHECK(hipMemcpyAsync(h_hdr, d, hdr_bytes, hipMemcpyDeviceToHost, s));
   }
 
-  // The three kernels over task ranges [b_i, e_i) of launch i (queues dq[0..2]).
+  // The three kernels over task ranges [b_i, e_i) of launch i (queues
+  // dq[0..2]), then the heavy rows the two walking launches set aside (queues
+  // hq[0..1], row counts hn[0..1], all starting at zero).
   template <bool OOC>
-  void launch_kernels(gem_ctx *c, uint32_t grd, const uint32_t *dn, unsigned int *dq, bool prof_kernels,
-                      bool count_items, uint64_t b0, uint64_t e0, uint64_t b1, uint64_t e1, uint64_t b2,
-                      uint64_t e2) {
+  void launch_kernels(gem_ctx *c, uint32_t grd, const uint32_t *dn, unsigned int *dq, unsigned int *hq,
+                      unsigned int *hn, bool prof_kernels, bool count_items, uint64_t b0, uint64_t e0, uint64_t b1,
+                      uint64_t e1, uint64_t b2, uint64_t e2) {
     hipStream_t s = c->stream;
     unsigned long long *dp = reinterpret_cast<unsigned long long *>(hdr.as<unsigned char>() + HDR_PITEMS);
     if (b0 < e0) {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_WRITE : -1);
       k_vext<false, OOC><<<grd, VB, lds, s>>>(items[0].as<VItem>(), dn + 0, blk[0].as<uint64_t>(), dq + 0, max_nb,
-                                               max_ndup, count_items ? dp : nullptr, b0, e0);
+                                               max_ndup, count_items ? dp : nullptr, count_items ? dp + 4 : nullptr,
+                                               b0, e0, heavy[0].as<HeavyRow>(), hn + 0, HEAVY_CAP, HEAVY_T);
       GEM_LAUNCH_CHECK("k_vext<write>");
     }
     if (b1 < e1) {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_FINAL : -1);
       k_vext<true, OOC><<<grd, VB, lds, s>>>(items[1].as<VItem>(), dn + 1, blk[1].as<uint64_t>(), dq + 1, max_nb,
-                                              max_ndup, count_items ? dp + 1 : nullptr, b1, e1);
+                                              max_ndup, count_items ? dp + 1 : nullptr, nullptr, b1, e1,
+                                              heavy[1].as<HeavyRow>(), hn + 1, HEAVY_CAP, HEAVY_T);
       GEM_LAUNCH_CHECK("k_vext<final>");
     }
     if (b2 < e2) {
       ProfLaunch pl(c, prof_kernels ? K_VCOUNT1 : -1);
       k_vcount1<OOC><<<grd, VB, 0, s>>>(items[2].as<VItem>(), dn + 2, blk[2].as<uint64_t>(), dq + 2, b2, e2);
       GEM_LAUNCH_CHECK("k_vcount1");
+    }
+    if (b0 < e0) {
+      ProfLaunch pl(c, prof_kernels ? K_VEXT_HEAVY_WRITE : -1);
+      k_heavy_scan<<<1, 1024, 0, s>>>(heavy[0].as<HeavyRow>(), hn + 0, HEAVY_CAP, HEAVY_CHUNK, hpre[0].as<uint64_t>());
+      GEM_LAUNCH_CHECK("k_heavy_scan");
+      k_vext_heavy<false, OOC><<<grd, VB, 0, s>>>(items[0].as<VItem>(), heavy[0].as<HeavyRow>(), hn + 0, HEAVY_CAP,
+                                                   hpre[0].as<uint64_t>(), hq + 0, HEAVY_CHUNK,
+                                                   count_items ? dp + 2 : nullptr, count_items ? dp + 5 : nullptr);
+      GEM_LAUNCH_CHECK("k_vext_heavy<write>");
+    }
+    if (b1 < e1) {
+      ProfLaunch pl(c, prof_kernels ? K_VEXT_HEAVY_FINAL : -1);
+      k_heavy_scan<<<1, 1024, 0, s>>>(heavy[1].as<HeavyRow>(), hn + 1, HEAVY_CAP, HEAVY_CHUNK, hpre[1].as<uint64_t>());
+      GEM_LAUNCH_CHECK("k_heavy_scan");
+      k_vext_heavy<true, OOC><<<grd, VB, 0, s>>>(items[1].as<VItem>(), heavy[1].as<HeavyRow>(), hn + 1, HEAVY_CAP,
+                                                  hpre[1].as<uint64_t>(), hq + 1, HEAVY_CHUNK,
+                                                  count_items ? dp + 3 : nullptr, nullptr);
+      GEM_LAUNCH_CHECK("k_vext_heavy<final>");
     }
   }
 
@@ -530,7 +725,9 @@ struct ExtSlot {
         }
         const uint64_t most = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
         const uint32_t grd = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(grid, most));
-        launch_kernels<true>(c, grd, dn, q + 3 * g, prof, prof, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+        // per group: 3 main queues, 2 heavy queues, 2 heavy-row counts
+        unsigned int *gq = q + 7 * g;
+        launch_kernels<true>(c, grd, dn, gq, gq + 3, gq + 5, prof, prof, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
         tc.done(g);
       }
       wa = wb;
@@ -776,10 +973,9 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
       slot.launch(c_, c_->use_graph && !prof);
     GEM_HIP_CHECK(hipStreamSynchronize(s));
     if (tc) tc->finish();
-    std::vector<uint64_t> hc(nctr + 2, 0);
+    std::vector<uint64_t> hc(nctr, 0);
     for (size_t i = 0; i < nctr; ++i) hc[i] = slot.counters()[i];
-    hc[nctr] = slot.pitems()[0];
-    hc[nctr + 1] = slot.pitems()[1];
+    const uint64_t *pc = slot.pitems();
     if (prof) {
       // algorithmic bytes: rows read (4 B per column + an offset pair per
       // backward list), 4 B per list entry walked, output rows written
@@ -789,11 +985,12 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
           b += (double)(it.row_end - it.row_begin) * (4.0 * it.k + 16.0 * (double)L.max_nb);
         return b;
       };
-      double outb = 0;
-      for (auto &w : wt) outb += 4.0 * w.k * (double)hc[w.ctr];
-      prof_bytes(c_, K_VEXT_WRITE, rows_bytes(wr) + 4.0 * (double)hc[nctr] + outb);
-      prof_bytes(c_, K_VEXT_FINAL, rows_bytes(fin) + 4.0 * (double)hc[nctr + 1]);
+      // (the heavy-row kernels: the entries they walk and the rows they write)
+      prof_bytes(c_, K_VEXT_WRITE, rows_bytes(wr) + 4.0 * (double)pc[0] + (double)pc[4]);
+      prof_bytes(c_, K_VEXT_FINAL, rows_bytes(fin) + 4.0 * (double)pc[1]);
       prof_bytes(c_, K_VCOUNT1, rows_bytes(fin1));
+      prof_bytes(c_, K_VEXT_HEAVY_WRITE, 4.0 * (double)pc[2] + (double)pc[5]);
+      prof_bytes(c_, K_VEXT_HEAVY_FINAL, 4.0 * (double)pc[3]);
     }
     for (auto &w : wt) {
       const uint64_t got = hc[w.ctr];

@@ -47,6 +47,25 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
   return x;
 }
 
+// Block-wide exclusive scan of one u64 per thread (blockDim.x a multiple of
+// 64, at most 8 waves per smem slot count); returns the block total.
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t x, uint64_t *excl, uint64_t *smem /*[nwaves]*/) {
+  const uint32_t t = threadIdx.x, w = t / 64, l = t % 64;
+  const uint32_t nw = blockDim.x / 64;
+  uint64_t inc = wave_incl_scan_u64(x);
+  if (l == 63) smem[w] = inc;
+  __syncthreads();
+  uint64_t wbase = 0, tot = 0;
+  for (uint32_t i = 0; i < nw; ++i) {
+    uint64_t s = smem[i];
+    if (i < w) wbase += s;
+    tot += s;
+  }
+  __syncthreads();
+  *excl = wbase + inc - x;
+  return tot;
+}
+
 // Largest r in [0, 64) with starts[r] <= item (starts ascending, starts[0] == 0).
 __device__ __forceinline__ uint32_t seg_of(const uint32_t *starts, uint32_t item) {
   uint32_t lo = 0;
