@@ -97,7 +97,14 @@ def main():
     ap.add_argument("--group", type=int, default=None,
                     help="queries resident at once in the batched path (0 = all; default: autotuned, else 0)")
     ap.add_argument("--no-tuning", action="store_true", help="ignore gem_amd/tuning/<workload>.json")
+    ap.add_argument("--level-budget", default=None,
+                    help="bytes of pending level rows shared by the running queries (e.g. 64G; default: "
+                         "autotuned, else the context's)")
+    ap.add_argument("--heavy-split", default=None,
+                    help="THRESHOLD,CHUNK: rows with longer driving lists are walked in CHUNK-entry pieces "
+                         "(default: autotuned, else 1024,1024)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path pass")
+    ap.add_argument("--no-ablation", action="store_true", help="skip the pass with GEM-derived orders")
     ap.add_argument("--plain-sample", type=int, default=64,
                     help="run the plain path on a systematic sample of this many queries (every k-th; "
                          "0 = all queries)")
@@ -144,12 +151,15 @@ def main():
     # the autotuner's choices for this workload, unless given on the command line
     from gem_amd.autotune import apply, load_tuning
     tuning = {} if args.no_tuning else load_tuning(os.path.join(ROOT, "gem_amd", "tuning", f"{args.workload}.json"))
+    if args.heavy_split:
+        ht, hc = (int(x) for x in args.heavy_split.split(","))
+        tuning = {**tuning, "heavy_t": ht, "heavy_chunk": hc}
     apply(ctx, tuning)
     if args.group is None:
         args.group = int(tuning.get("group", 0))
     if args.orders is None:
         args.orders = int(tuning.get("max_orders", 16))
-    level_budget = int(tuning.get("level_budget", 0))
+    level_budget = parse_bytes(args.level_budget) if args.level_budget else int(tuning.get("level_budget", 0))
     if tuning:
         log(f"[bench] autotuned: {tuning}")
     if stream_cache:
@@ -223,7 +233,7 @@ def main():
 
     # ablation: the same batched path with GEM-derived orders, one pass
     gem_batched = None
-    if model is not None:
+    if model is not None and not args.no_ablation:
         t = time.perf_counter()
         res_g = match_batch_learned(ctx, qs, None, cap=cap, work_limit=wl, level_budget=level_budget,
                                     group=args.group)

@@ -5,12 +5,15 @@
 Times the batched matcher on a sample of the workload's queries for every
 combination of the tunables and keeps the fastest:
   * persistent grid size of the work-queue kernels (occupancy),
-  * level budget (bytes of pending level chunks shared by running queries;
-    sets the chunk size of every level),
+  * heavy-row split of the walk kernels (driving-list length from which a row
+    is walked in pieces by many waves, and the piece length): load balance,
   * group size (queries resident on the GPU at once),
   * candidate orders scored per query by the order model.
-Counts do not depend on any of them (checked on every trial).  The chosen
-configuration is stored as JSON and read by bench.py.
+The level budget (bytes of pending level chunks) is left automatic: a
+quarter of the device memory free when a join starts; a sample of the
+workload would not exercise it.  Counts do not depend on any of them
+(checked on every trial).  The chosen configuration is stored as JSON and
+read by bench.py.
 """
 import argparse
 import itertools
@@ -24,9 +27,10 @@ from .learned import OrderModel, match_batch_learned
 from .workloads import CAP, make_graph, make_queries
 
 SPACE = {
-    "persistent_blocks": [1024, 2048, 4096],
-    "level_budget": [4 << 30, 16 << 30],
-    "group": [0, 24],
+    "persistent_blocks": [2048, 4096],
+    "heavy_t": [128, 256, 1024],
+    "heavy_chunk": [256, 1024],
+    "group": [0],
     "max_orders": [8, 16],
 }
 
@@ -47,12 +51,14 @@ def apply(ctx, cfg):
     arguments: level_budget, group, max_orders)."""
     if "persistent_blocks" in cfg:
         ctx.set_persistent_blocks(cfg["persistent_blocks"])
+    if "heavy_t" in cfg:
+        ctx.set_heavy_split(cfg["heavy_t"], cfg.get("heavy_chunk", 1024))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="rmat22")
-    ap.add_argument("--sample", type=int, default=48, help="queries timed per trial (every k-th of the workload)")
+    ap.add_argument("--sample", type=int, default=96, help="queries timed per trial (every k-th of the workload)")
     ap.add_argument("--cap", type=float, default=CAP)
     ap.add_argument("--work-limit", type=float, default=5e9)
     ap.add_argument("--model", default=None, help="order model (default: models/order_mlp_<workload>.npz)")
@@ -80,7 +86,7 @@ def main():
         apply(ctx, cfg)
         t = time.perf_counter()
         res = match_batch_learned(ctx, qs, model, max_orders=cfg["max_orders"], cap=cap, work_limit=wl,
-                                  level_budget=cfg["level_budget"], group=cfg["group"])
+                                  group=cfg["group"])
         dt = time.perf_counter() - t
         sig = [(r["count"], r["saturated"], r["work_limited"]) for r in res]
         if ref is None:
@@ -88,12 +94,12 @@ def main():
         same = all(a == b for a, b in zip(sig, ref) if not a[2] and not b[2])
         trials.append({**cfg, "seconds": dt, "counts_equal": same})
         log(f"[tune] {cfg} -> {dt:.3f}s counts_equal={same}")
-        apply(ctx, {"persistent_blocks": 2048})
+        apply(ctx, {"persistent_blocks": 2048, "heavy_t": 1024, "heavy_chunk": 1024})
     ok = [t for t in trials if t["counts_equal"]]
     best = min(ok, key=lambda t: t["seconds"])
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
     with open(out, "w") as f:
-        json.dump({"workload": args.workload, "sample": len(qs), "best": {k: best[k] for k in keys},
+        json.dump({"workload": args.workload, "sample": len(qs), "best": {**{k: best[k] for k in keys}, "level_budget": 0},
                    "best_seconds": best["seconds"], "trials": trials}, f, indent=1)
     print(json.dumps({"best": {k: best[k] for k in keys}, "seconds": best["seconds"], "out": out}))
     ctx.close()

@@ -43,11 +43,6 @@ int gem_hip_create(int device, gem_ctx **out) {
   int st = guard(c, [&]() {
     GEM_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) GEM_HIP_CHECK(hipEventCreate(&e));
-    // keep freed pool memory for reuse instead of returning it to the driver
-    hipMemPool_t pool;
-    GEM_HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, device));
-    uint64_t thr = UINT64_MAX;
-    GEM_HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
   });
   if (st) {
     gem_hip_destroy(c);
@@ -69,8 +64,13 @@ void gem_hip_destroy(gem_ctx *c) {
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->pinned) (void)hipHostFree(c->pinned);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
-  delete c;
+  hipStream_t s = c->stream;
+  delete c;  // its remaining device buffers go back to the block cache
+  if (s) {
+    (void)hipStreamSynchronize(s);
+    gemd::pool_release(s);
+    (void)hipStreamDestroy(s);
+  }
 }
 
 const char *gem_hip_last_error(const gem_ctx *c) { return c ? c->err.c_str() : "null context"; }
@@ -84,6 +84,13 @@ int gem_hip_set_level_budget(gem_ctx *c, uint64_t bytes) {
 int gem_hip_set_persistent_blocks(gem_ctx *c, uint32_t blocks) {
   if (!c || blocks == 0 || blocks > (1u << 20)) return GEM_E_INVALID;
   c->persistent_blocks = blocks;
+  return GEM_OK;
+}
+
+int gem_hip_set_heavy_split(gem_ctx *c, uint32_t threshold, uint32_t chunk) {
+  if (!c || threshold < 32 || threshold > (1u << 24) || chunk < 64 || chunk > (1u << 20)) return GEM_E_INVALID;
+  c->heavy_t = threshold;
+  c->heavy_chunk = chunk;
   return GEM_OK;
 }
 

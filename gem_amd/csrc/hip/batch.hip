@@ -2,6 +2,9 @@
 // edge tables for each query, a vertex plan per query (given, or derived from
 // GEM's edge order), then one VBatchRunner over the whole group.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 
 #include "ctx.h"
 #include "tcache.h"
@@ -78,7 +81,7 @@ void run_match_batch(gem_ctx *c, const gem_query_view *qs, uint32_t nq, const ui
   GEM_HIP_CHECK(hipEventRecord(c->ev[6], s));
   for (uint32_t g0 = 0; g0 < nq; g0 += group) {
     const uint32_t gn = std::min(group, nq - g0);
-    const uint64_t budget = o.level_budget ? o.level_budget : (16ull << 30);  // shared by the running queries
+    const uint64_t budget = o.level_budget;  // shared by the running queries (0 = from free memory)
     std::vector<QueryDev> qd(gn);
     std::vector<VQueryState> st(gn);
     std::vector<VQueryState *> ptrs;
@@ -129,17 +132,26 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
   GEM_HIP_CHECK(hipEventRecord(c->ev[6], s));
   for (uint32_t g0 = 0; g0 < nq; g0 += group) {
     const uint32_t gn = std::min(group, nq - g0);
-    const uint64_t budget = o.level_budget ? o.level_budget : (16ull << 30);  // shared by the running queries
+    const uint64_t budget = o.level_budget;  // shared by the running queries (0 = from free memory)
     std::vector<QueryDev> qd(gn);
     std::vector<VQueryState> st(gn);
     std::vector<VQueryState *> ptrs;
     std::vector<std::vector<std::vector<uint32_t>>> cand(gn);
     std::vector<int32_t> feats;
     std::vector<uint32_t> owner;
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+    double t_filter = 0, t_tables = 0, t_orders = 0;
+    const auto t_prep = clk::now();
     for (uint32_t i = 0; i < gn; ++i) {
+      const auto ta = clk::now();
       run_filter(c, qd[i], &qs[g0 + i], 0);
+      const auto tb = clk::now();
       run_tables(c, qd[i]);
       if (streamed) spill_tables(c, qd[i], c->stream_slot_bytes);
+      const auto tc = clk::now();
+      t_filter += secs(ta, tb);
+      t_tables += secs(tb, tc);
       const gemh::QueryStats qsx = query_stats(c, qd[i]);
       gemh::candidate_orders(qd[i].Q, qsx, gem_vertex_order(qd[i]), mlp ? max_orders : 1, 0x9e3779b9ull + g0 + i,
                              &cand[i]);
@@ -151,6 +163,7 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
           owner.push_back(i);
         }
       }
+      t_orders += secs(tc, clk::now());
     }
     std::vector<uint32_t> pick(gn, 0);
     if (mlp && !owner.empty()) {
@@ -176,6 +189,15 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
       plan_query(c, st[i], qd[i], cand[i][pick[i]].data());
       ptrs.push_back(&st[i]);
       if (chosen) chosen[g0 + i] = pick[i];
+    }
+    const char *verbose_env = std::getenv("GEM_VERBOSE");
+    if (verbose_env && verbose_env[0] == '1') {
+      std::fprintf(stderr,
+                   "[batch] %u queries prepared in %.2f s: filter %.2f s, edge tables %.2f s, order candidates and "
+                   "features %.2f s, model and plans %.2f s\n",
+                   gn, secs(t_prep, clk::now()), t_filter, t_tables, t_orders,
+                   secs(t_prep, clk::now()) - t_filter - t_tables - t_orders);
+      std::fflush(stderr);
     }
     VBatchRunner runner(c, o.cap, o.work_limit, budget, streamed);
     runner.run(ptrs);

@@ -40,21 +40,34 @@ struct Error {
       throw ::gemd::Error{GEM_E_HIP, std::string("launch ") + (name) + ": " + hipGetErrorString(e_)}; \
   } while (0)
 
-// Stream-ordered device allocation (hipMallocAsync on the context stream; the
-// default pool keeps freed memory for reuse).
+// Device memory of DevBuf comes from a caching allocator over hipMalloc:
+// freed blocks are kept in size classes (four per power of two, 256 B at
+// least) and handed out again to allocations on the stream that freed them.
+// All device work of a context runs on its one stream, so a block the host
+// frees after enqueueing its last use can serve later work on that stream
+// without a wait.  A failed hipMalloc releases every cached block and retries.
+void *pool_alloc(size_t n, hipStream_t s, size_t *cls);
+void pool_free(void *p, size_t cls, hipStream_t s);
+// hipFree every cached block of stream s (its work must have finished).
+void pool_release(hipStream_t s);
+// Bytes held by the allocator for reuse (not in use by any buffer).
+size_t pool_cached_bytes();
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  size_t cls = 0;  // size of the pool block behind p
   hipStream_t s = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
-  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), s(o.s) { o.p = nullptr; }
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), cls(o.cls), s(o.s) { o.p = nullptr; }
   DevBuf &operator=(DevBuf &&o) noexcept {
     if (this != &o) {
       reset();
       p = o.p;
       bytes = o.bytes;
+      cls = o.cls;
       s = o.s;
       o.p = nullptr;
     }
@@ -64,16 +77,12 @@ struct DevBuf {
   void alloc(size_t n, hipStream_t stream) {
     reset();
     s = stream;
-    bytes = n;
     if (n == 0) return;
-    hipError_t e = hipMallocAsync(&p, n, stream);
-    if (e != hipSuccess) {
-      p = nullptr;
-      throw Error{GEM_E_NOMEM, "hipMallocAsync(" + std::to_string(n) + " bytes): " + hipGetErrorString(e)};
-    }
+    p = pool_alloc(n, stream, &cls);
+    bytes = n;
   }
   void reset() {
-    if (p) (void)hipFreeAsync(p, s);
+    if (p) pool_free(p, cls, s);
     p = nullptr;
     bytes = 0;
   }
@@ -237,6 +246,7 @@ struct gem_ctx {
   uint64_t level_budget_bytes = 1ull << 31;  // per join level buffer
   uint64_t work_limit_rows = 0;             // 0 = none
   uint32_t persistent_blocks = 2048;        // grid of the batched path's persistent kernels
+  uint32_t heavy_t = 1024, heavy_chunk = 1024;  // batched walks: long driving lists split over waves
   bool use_graph = false;                   // replay the batched extend step as a hipGraph
   // batched path with streamed edge tables: values in pinned host memory,
   // a device cache of stream_cache_bytes in slots of stream_slot_bytes

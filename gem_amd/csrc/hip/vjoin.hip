@@ -530,10 +530,11 @@ uint32_t grid_for(uint64_t blocks) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks, g_persistent_blocks));
 }
 
-// Rows whose driving list has more entries than HEAVY_T are walked by
-// k_vext_heavy, HEAVY_CHUNK entries per wave task; at most HEAVY_CAP such rows
-// per launch (a row that finds the list full is walked by its own wave).
-constexpr uint32_t HEAVY_T = 1024, HEAVY_CHUNK = 1024, HEAVY_CAP = 1u << 20;
+// Rows whose driving list has more entries than the context's heavy_t are
+// walked by k_vext_heavy, heavy_chunk entries per wave task; at most HEAVY_CAP
+// such rows per launch (a row that finds the list full is walked by its own
+// wave).
+constexpr uint32_t HEAVY_CAP = 1u << 20;
 
 // Fixed buffers for the extend step of every iteration of one run: item
 // tables and block prefixes of the three launches (write, final walk, final
@@ -561,6 +562,7 @@ struct ExtSlot {
   uint32_t max_nb = 1, max_ndup = 0;
   size_t lds = 0;
   uint32_t grid = 0;
+  uint32_t heavy_t = 1024, heavy_chunk = 1024;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
@@ -572,6 +574,8 @@ struct ExtSlot {
     lds = (size_t)VW * 64 * ((max_nb - 1) * 12 + max_ndup * 4);
     if (lds > 60 * 1024) throw Error{GEM_E_LIMIT, "a level has too many backward edges for the vjoin LDS layout"};
     grid = g_persistent_blocks;
+    heavy_t = c->heavy_t;
+    heavy_chunk = c->heavy_chunk;
     hdr_bytes = HDR_CTR + 8ull * cap;
     hdr.alloc(hdr_bytes, s);
     for (int i = 0; i < 2; ++i) {
@@ -634,14 +638,14 @@ struct ExtSlot {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_WRITE : -1);
       k_vext<false, OOC><<<grd, VB, lds, s>>>(items[0].as<VItem>(), dn + 0, blk[0].as<uint64_t>(), dq + 0, max_nb,
                                                max_ndup, count_items ? dp : nullptr, count_items ? dp + 4 : nullptr,
-                                               b0, e0, heavy[0].as<HeavyRow>(), hn + 0, HEAVY_CAP, HEAVY_T);
+                                               b0, e0, heavy[0].as<HeavyRow>(), hn + 0, HEAVY_CAP, heavy_t);
       GEM_LAUNCH_CHECK("k_vext<write>");
     }
     if (b1 < e1) {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_FINAL : -1);
       k_vext<true, OOC><<<grd, VB, lds, s>>>(items[1].as<VItem>(), dn + 1, blk[1].as<uint64_t>(), dq + 1, max_nb,
                                               max_ndup, count_items ? dp + 1 : nullptr, nullptr, b1, e1,
-                                              heavy[1].as<HeavyRow>(), hn + 1, HEAVY_CAP, HEAVY_T);
+                                              heavy[1].as<HeavyRow>(), hn + 1, HEAVY_CAP, heavy_t);
       GEM_LAUNCH_CHECK("k_vext<final>");
     }
     if (b2 < e2) {
@@ -651,19 +655,19 @@ struct ExtSlot {
     }
     if (b0 < e0) {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_HEAVY_WRITE : -1);
-      k_heavy_scan<<<1, 1024, 0, s>>>(heavy[0].as<HeavyRow>(), hn + 0, HEAVY_CAP, HEAVY_CHUNK, hpre[0].as<uint64_t>());
+      k_heavy_scan<<<1, 1024, 0, s>>>(heavy[0].as<HeavyRow>(), hn + 0, HEAVY_CAP, heavy_chunk, hpre[0].as<uint64_t>());
       GEM_LAUNCH_CHECK("k_heavy_scan");
       k_vext_heavy<false, OOC><<<grd, VB, 0, s>>>(items[0].as<VItem>(), heavy[0].as<HeavyRow>(), hn + 0, HEAVY_CAP,
-                                                   hpre[0].as<uint64_t>(), hq + 0, HEAVY_CHUNK,
+                                                   hpre[0].as<uint64_t>(), hq + 0, heavy_chunk,
                                                    count_items ? dp + 2 : nullptr, count_items ? dp + 5 : nullptr);
       GEM_LAUNCH_CHECK("k_vext_heavy<write>");
     }
     if (b1 < e1) {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_HEAVY_FINAL : -1);
-      k_heavy_scan<<<1, 1024, 0, s>>>(heavy[1].as<HeavyRow>(), hn + 1, HEAVY_CAP, HEAVY_CHUNK, hpre[1].as<uint64_t>());
+      k_heavy_scan<<<1, 1024, 0, s>>>(heavy[1].as<HeavyRow>(), hn + 1, HEAVY_CAP, heavy_chunk, hpre[1].as<uint64_t>());
       GEM_LAUNCH_CHECK("k_heavy_scan");
       k_vext_heavy<true, OOC><<<grd, VB, 0, s>>>(items[1].as<VItem>(), heavy[1].as<HeavyRow>(), hn + 1, HEAVY_CAP,
-                                                  hpre[1].as<uint64_t>(), hq + 1, HEAVY_CHUNK,
+                                                  hpre[1].as<uint64_t>(), hq + 1, heavy_chunk,
                                                   count_items ? dp + 3 : nullptr, nullptr);
       GEM_LAUNCH_CHECK("k_vext_heavy<final>");
     }
@@ -767,7 +771,14 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   // budget_bytes_ is shared by the queries still running: a query's chunk may
   // use total / active (at most 4 GiB, at least 1 MiB), so a long-running last
   // query is not held to a small slice.
-  const uint64_t total_budget = std::max<uint64_t>(budget_bytes_, 1ull << 20);
+  uint64_t total_budget = budget_bytes_;
+  if (!total_budget) {
+    // a quarter of what the device could still give the run's level buffers
+    size_t free_b = 0, total_b = 0;
+    GEM_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    total_budget = std::max<uint64_t>((free_b + pool_cached_bytes()) / 4, 1ull << 30);
+  }
+  total_budget = std::max<uint64_t>(total_budget, 1ull << 20);
   // fixed launch buffers for the run (one item per query per launch at most)
   ExtSlot slot;
   {
@@ -805,8 +816,17 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   // GEM_VERBOSE=1: a progress line on stderr every ~2 s of a long batch
   const char *verbose_env = std::getenv("GEM_VERBOSE");
   const bool verbose = verbose_env && verbose_env[0] == '1';
-  auto last_print = std::chrono::steady_clock::now();
+  using clk = std::chrono::steady_clock;
+  auto last_print = clk::now();
+  auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+  // where the run's time goes (reported with GEM_VERBOSE=1): host planning,
+  // length passes (launch to sync), extend steps (launch to sync), and how
+  // many iterations ran on the 1 MiB minimum chunk budget
+  double t_plan = 0, t_len = 0, t_ext = 0, t_alloc = 0;
+  uint64_t n_len = 0, n_floor = 0, ext_tasks = 0;
+  const auto run_start = clk::now();
   for (;;) {
+    const auto it_start = clk::now();
     std::vector<VQueryState *> act;
     for (VQueryState *S : qs)
       if (!S->done && !S->stack.empty()) act.push_back(S);
@@ -820,10 +840,13 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
       for (const VStackItem &it : S->stack) live += it.rows ? it.rows->bytes : 0;
     const uint64_t left = total_budget > live ? total_budget - live : 0;
     const uint64_t budget = std::max<uint64_t>(std::min<uint64_t>(left / act.size(), 4ull << 30), 1ull << 20);
-    if (verbose && std::chrono::steady_clock::now() - last_print > std::chrono::seconds(2)) {
-      last_print = std::chrono::steady_clock::now();
-      std::fprintf(stderr, "[vjoin] iteration %llu: %zu of %zu queries active\n", (unsigned long long)iterations_,
-                   act.size(), qs.size());
+    if (left / act.size() < (1ull << 20)) ++n_floor;
+    if (verbose && clk::now() - last_print > std::chrono::seconds(2)) {
+      last_print = clk::now();
+      std::fprintf(stderr,
+                   "[vjoin] iteration %llu: %zu of %zu queries active, pending levels %.2f GiB, chunk budget %.1f "
+                   "MiB\n",
+                   (unsigned long long)iterations_, act.size(), qs.size(), live / 1073741824.0, budget / 1048576.0);
       std::fflush(stderr);
     }
     // ---- length pass for unmeasured, non-final top items
@@ -845,6 +868,9 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
       len_owner.push_back(S);
     }
     if (!lenL.empty()) {
+      const auto t0 = clk::now();
+      t_plan += secs(it_start, t0);
+      ++n_len;
       DevLaunch D;
       upload(c_, lenL, D);
       const uint64_t nblk = lenL.blk.back();
@@ -871,7 +897,9 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
         // rows + one offset pair per backward list
         prof_bytes(c_, K_VLEN, (double)top.nrows * (4.0 * top.k + 16.0 * len_owner[t]->max_nb[top.level]));
       }
+      t_len += secs(t0, clk::now());
     }
+    const auto t_ext_plan = clk::now();
     // ---- extend pass: write chunks and final counts
     Launch wr, fin, fin1;
     struct WTask {
@@ -940,7 +968,9 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
         continue;
       }
       auto out = std::make_shared<DevBuf>();
+      const auto ta = clk::now();
       out->alloc(ub * row_bytes, s);
+      t_alloc += secs(ta, clk::now());
       it.out = out->as<uint32_t>();
       it.out_cap = ub;
       it.counter = reinterpret_cast<unsigned long long *>((uintptr_t)nctr);
@@ -967,11 +997,17 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
     }
     std::memset(slot.h_hdr + ExtSlot::HDR_QUEUE, 0, slot.hdr_bytes - ExtSlot::HDR_QUEUE);
     const bool prof = c_->prof.on;
+    const auto t1 = clk::now();
+    t_plan += secs(t_ext_plan, t1);
+    ext_tasks += wr.blk.empty() ? 0 : wr.blk.back();
+    ext_tasks += fin.blk.empty() ? 0 : fin.blk.back();
+    ext_tasks += fin1.blk.empty() ? 0 : fin1.blk.back();
     if (tc)
       slot.launch_streamed(c_, *tc);
     else
       slot.launch(c_, c_->use_graph && !prof);
     GEM_HIP_CHECK(hipStreamSynchronize(s));
+    t_ext += secs(t1, clk::now());
     if (tc) tc->finish();
     std::vector<uint64_t> hc(nctr, 0);
     for (size_t i = 0; i < nctr; ++i) hc[i] = slot.counters()[i];
@@ -996,7 +1032,20 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
       const uint64_t got = hc[w.ctr];
       if (got > w.cap_rows) throw Error{GEM_E_STATE, "vjoin output exceeded its bound"};
       w.S->level_rows[w.level - 1] += got;
-      if (got) w.S->stack.push_back(VStackItem{w.level, w.out, got, w.k, {}, 0, false});
+      if (!got) continue;
+      // the output was allocated for the length pass's bound; when fewer than
+      // half of those rows survived, the rows move to a buffer of their size
+      // (pending rows are what the level budget is charged with)
+      std::shared_ptr<DevBuf> rows = w.out;
+      const uint64_t rb = 4ull * w.k;
+      if (2 * got <= w.cap_rows && w.cap_rows * rb >= (4ull << 20)) {
+        const auto ta = clk::now();
+        rows = std::make_shared<DevBuf>();
+        rows->alloc(got * rb, s);
+        GEM_HIP_CHECK(hipMemcpyAsync(rows->p, w.out->p, got * rb, hipMemcpyDeviceToDevice, s));
+        t_alloc += secs(ta, clk::now());
+      }
+      w.S->stack.push_back(VStackItem{w.level, std::move(rows), got, w.k, {}, 0, false});
     }
     for (auto &f : ft) {
       const uint64_t got = hc[f.ctr];
@@ -1017,6 +1066,15 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
       }
     }
     prof_flush(c_);
+  }
+  if (verbose) {
+    std::fprintf(stderr,
+                 "[vjoin] run of %zu queries: %llu iterations in %.2f s: host planning %.2f s (level allocations "
+                 "%.2f s), %llu length passes %.2f s, extend steps %.2f s (%llu 256-row tasks); %llu iterations on "
+                 "the minimum chunk budget\n",
+                 qs.size(), (unsigned long long)iterations_, secs(run_start, clk::now()), t_plan, t_alloc,
+                 (unsigned long long)n_len, t_len, t_ext, (unsigned long long)ext_tasks, (unsigned long long)n_floor);
+    std::fflush(stderr);
   }
 }
 

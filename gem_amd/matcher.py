@@ -147,6 +147,11 @@ class GemContext:
     def set_persistent_blocks(self, blocks):
         self._check(self._lib.gem_hip_set_persistent_blocks(self._h, int(blocks)), "gem_hip_set_persistent_blocks")
 
+    def set_heavy_split(self, threshold, chunk):
+        """Rows whose driving list is longer than `threshold` are walked in
+        `chunk`-entry pieces spread over waves (load balance only)."""
+        self._check(self._lib.gem_hip_set_heavy_split(self._h, int(threshold), int(chunk)), "gem_hip_set_heavy_split")
+
     def set_graph(self, on=True):
         """Replay the batched path's extend step as a captured hipGraph."""
         self._check(self._lib.gem_hip_set_graph(self._h, 1 if on else 0), "gem_hip_set_graph")

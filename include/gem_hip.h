@@ -248,7 +248,8 @@ typedef struct gem_batch_opts {
   uint64_t work_limit;   /* work a query may do before it stops unfinished:
                             rows read plus list entries walked (0 = none)  */
   uint64_t level_budget; /* bytes for pending level chunks, shared by the
-                            group's running queries (0 = 16 GiB)           */
+                            group's running queries (0 = a quarter of the
+                            device memory free when the join starts)       */
   uint32_t group;        /* queries resident on the GPU at once (0 = all)   */
   uint32_t reserved;
 } gem_batch_opts;
@@ -266,6 +267,12 @@ int gem_hip_match_batch(gem_ctx *ctx, const gem_query_view *queries, uint32_t nq
  * kernels; they pull tasks from a device counter, so this sets occupancy,
  * not the amount of work (default 2048).  Set by the autotuner. */
 int gem_hip_set_persistent_blocks(gem_ctx *ctx, uint32_t blocks);
+/* Load balance of the batched path's walks: a row whose driving (shortest)
+ * list has more than `threshold` entries is set aside and walked `chunk`
+ * entries per wave task by a second kernel instead of by the wave holding the
+ * row (defaults 1024 and 1024; threshold in [32, 2^24], chunk in
+ * [64, 2^20]).  Counts do not depend on either.  Set by the autotuner. */
+int gem_hip_set_heavy_split(gem_ctx *ctx, uint32_t threshold, uint32_t chunk);
 /* on != 0: the batched path's per-iteration extend step (table uploads, three
  * persistent kernels, result download) is captured once per run as a hipGraph
  * and replayed; off (default): launched eagerly.  Kernel profiling forces the

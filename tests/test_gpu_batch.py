@@ -115,6 +115,13 @@ def test_batch_invariance(gpu_ctx, rmat_graph):
     base = [(r["count"], r["saturated"]) for r in gpu_ctx.match_batch(qs, cap=cap)]
     small = gpu_ctx.match_batch(qs, cap=cap, level_budget=1 << 20, group=3)
     assert [(r["count"], r["saturated"]) for r in small] == base
+    # most rows walked in 64-entry pieces by the heavy-row kernel
+    try:
+        gpu_ctx.set_heavy_split(32, 64)
+        split = gpu_ctx.match_batch(qs, cap=cap)
+    finally:
+        gpu_ctx.set_heavy_split(1024, 1024)
+    assert [(r["count"], r["saturated"]) for r in split] == base
     rng = np.random.default_rng(5)
     orders = [_random_vertex_order(q, rng) for q in qs]
     other = gpu_ctx.match_batch(qs, vertex_orders=orders, cap=cap)
