@@ -121,7 +121,7 @@ __device__ __forceinline__ uint64_t list_addr(const VLevel &L, uint32_t j, uint3
 // launch (256-row units) are processed (streamed tables run a launch in task
 // groups).
 template <bool FINAL, bool OOC>
-__global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p,
+__global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8))) k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p,
                                             const uint64_t *__restrict__ blk_prefix, unsigned int *__restrict__ queue,
                                             uint32_t max_nb, uint32_t max_ndup,
                                             unsigned long long *__restrict__ prof_items,
@@ -132,7 +132,8 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
   const uint32_t nitems = *nitems_p;
   __shared__ uint32_t st[VW][65];
   __shared__ uint64_t drv[VW][64];  // address of each row's driving list
-  __shared__ uint32_t sv_row[VW][64], sv_val[VW][64];  // this step's survivors
+  __shared__ uint8_t sv_row[VW][128];    // this step's survivors: row in the task
+  __shared__ uint32_t sv_val[VW][128];  // and new value
   extern __shared__ unsigned char vlds[];
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const uint32_t per_wave = 64 * ((max_nb > 0 ? max_nb - 1 : 0) * 12 + max_ndup * 4);
@@ -204,22 +205,24 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     walked += wtotal;
     uint64_t survivors = 0;
-    for (uint32_t s = 0; s < wtotal; s += 64) {
-      const uint32_t item = s + l;
-      const bool iv = item < wtotal;
-      bool keep = false;
-      uint32_t val = 0, rw = 0;
-      if (iv) {
-        rw = seg_of(st[w], item);
-        val = reinterpret_cast<const uint32_t *>(drv[w][rw])[item - st[w][rw]];
-        keep = true;
-        for (uint32_t c = 0; c < ndup && keep; ++c)
-          if (dpos[c * 64 + rw] == val) keep = false;
-        for (uint32_t o = 0; o + 1 < nb && keep; ++o)
-          keep = sorted_contains(reinterpret_cast<const uint32_t *>(obase[o * 64 + rw]), olen[o * 64 + rw], val);
+    // 128 items per step, two per lane (items s + l and s + 64 + l), so that
+    // each lane has two loads in flight at every stage of the probes
+    for (uint32_t s = 0; s < wtotal; s += 128) {
+      uint32_t val0 = 0, val1 = 0, rw0 = 0, rw1 = 0;
+      bool keep0 = s + l < wtotal, keep1 = s + 64 + l < wtotal;
+      if (keep0) rw0 = seg_of(st[w], s + l);
+      if (keep1) rw1 = seg_of(st[w], s + 64 + l);
+      if (keep0) val0 = reinterpret_cast<const uint32_t *>(drv[w][rw0])[s + l - st[w][rw0]];
+      if (keep1) val1 = reinterpret_cast<const uint32_t *>(drv[w][rw1])[s + 64 + l - st[w][rw1]];
+      for (uint32_t c = 0; c < ndup; ++c) {
+        if (dpos[c * 64 + rw0] == val0) keep0 = false;
+        if (dpos[c * 64 + rw1] == val1) keep1 = false;
       }
-      const uint64_t m = __ballot(keep);
-      const uint32_t nk = (uint32_t)__popcll(m);
+      for (uint32_t o = 0; o + 1 < nb; ++o)
+        contains_pair(reinterpret_cast<const uint32_t *>(obase[o * 64 + rw0]), olen[o * 64 + rw0], val0, keep0,
+                      reinterpret_cast<const uint32_t *>(obase[o * 64 + rw1]), olen[o * 64 + rw1], val1, keep1);
+      const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
+      const uint32_t nk0 = (uint32_t)__popcll(m0), nk = nk0 + (uint32_t)__popcll(m1);
       if (FINAL) {
         survivors += nk;
       } else if (nk) {
@@ -227,10 +230,15 @@ __global__ void __launch_bounds__(VB) k_vext(const VItem *__restrict__ items, co
         if (l == 0) base = atomicAdd(it.counter, (unsigned long long)nk);
         base = readlane_u64(base, 0);
         written += (uint64_t)nk * 4 * (k + 1);
-        if (keep) {
-          const uint32_t i = rank_below(m);
-          sv_row[w][i] = rw;
-          sv_val[w][i] = val;
+        if (keep0) {
+          const uint32_t i = rank_below(m0);
+          sv_row[w][i] = rw0;
+          sv_val[w][i] = val0;
+        }
+        if (keep1) {
+          const uint32_t i = nk0 + rank_below(m1);
+          sv_row[w][i] = rw1;
+          sv_val[w][i] = val1;
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -336,21 +344,22 @@ __global__ void __launch_bounds__(VB) k_vext_heavy(const VItem *__restrict__ ite
     const uint64_t s1 = s0 + chunk < dlen ? s0 + chunk : dlen;
     walked += s1 - s0;
     uint64_t survivors = 0;
-    for (uint64_t s = s0; s < s1; s += 64) {
-      const uint64_t item = s + l;
-      const bool iv = item < s1;
-      bool keep = false;
-      uint32_t val = 0;
-      if (iv) {
-        val = dlist[item];
-        keep = true;
-        for (uint32_t c = 0; c < ndup && keep; ++c)
-          if (dpos[w][c] == val) keep = false;
-        for (uint32_t o = 0; o + 1 < nb && keep; ++o)
-          keep = sorted_contains(reinterpret_cast<const uint32_t *>(oaddr[w][o]), olen[w][o], val);
+    // two items per lane per step, as in k_vext
+    for (uint64_t s = s0; s < s1; s += 128) {
+      bool keep0 = s + l < s1, keep1 = s + 64 + l < s1;
+      uint32_t val0 = 0, val1 = 0;
+      if (keep0) val0 = dlist[s + l];
+      if (keep1) val1 = dlist[s + 64 + l];
+      for (uint32_t c = 0; c < ndup; ++c) {
+        if (dpos[w][c] == val0) keep0 = false;
+        if (dpos[w][c] == val1) keep1 = false;
       }
-      const uint64_t m = __ballot(keep);
-      const uint32_t nk = (uint32_t)__popcll(m);
+      for (uint32_t o = 0; o + 1 < nb; ++o) {
+        const uint32_t *a = reinterpret_cast<const uint32_t *>(oaddr[w][o]);
+        contains_pair(a, olen[w][o], val0, keep0, a, olen[w][o], val1, keep1);
+      }
+      const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
+      const uint32_t nk0 = (uint32_t)__popcll(m0), nk = nk0 + (uint32_t)__popcll(m1);
       if (FINAL) {
         survivors += nk;
       } else if (nk) {
@@ -358,13 +367,16 @@ __global__ void __launch_bounds__(VB) k_vext_heavy(const VItem *__restrict__ ite
         if (l == 0) base = atomicAdd(it.counter, (unsigned long long)nk);
         base = readlane_u64(base, 0);
         written += (uint64_t)nk * 4 * (k + 1);
-        if (keep) {
-          const uint64_t o = base + rank_below(m);
-          if (o < it.out_cap) {
-            uint32_t *dst = it.out + o * (k + 1);
-            for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
-            dst[k] = val;
-          }
+        const uint64_t o0 = base + rank_below(m0), o1 = base + nk0 + rank_below(m1);
+        if (keep0 && o0 < it.out_cap) {
+          uint32_t *dst = it.out + o0 * (k + 1);
+          for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
+          dst[k] = val0;
+        }
+        if (keep1 && o1 < it.out_cap) {
+          uint32_t *dst = it.out + o1 * (k + 1);
+          for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
+          dst[k] = val1;
         }
       }
     }

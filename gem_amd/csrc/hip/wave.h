@@ -90,6 +90,46 @@ __device__ __forceinline__ bool sorted_contains(const T *a, uint64_t n, T v) {
   return lo < n && a[lo] == v;
 }
 
+// Two membership tests at once: is v0 in sorted a0[0..n0) and v1 in
+// a1[0..n1)?  k0 / k1 enter as "still wanted" (a false one is not searched)
+// and leave as the result.  The two lower-bound searches advance in lockstep
+// so their loads are in flight together, and a hit is seen when it is probed
+// (a lower bound i with a[i] == v is always probed: the range's right end
+// only moves onto i by probing it), so no load follows the search.
+__device__ __forceinline__ void contains_pair(const uint32_t *a0, uint32_t n0, uint32_t v0, bool &k0,
+                                              const uint32_t *a1, uint32_t n1, uint32_t v1, bool &k1) {
+  uint32_t lo0 = 0, lo1 = 0;
+  bool f0 = false, f1 = false;
+  if (!k0) n0 = 0;
+  if (!k1) n1 = 0;
+  while (n0 | n1) {
+    const uint32_t h0 = n0 >> 1, h1 = n1 >> 1;
+    uint32_t x0 = 0, x1 = 0;
+    if (n0) x0 = a0[lo0 + h0];
+    if (n1) x1 = a1[lo1 + h1];
+    if (n0) {
+      if (x0 < v0) {
+        lo0 += h0 + 1;
+        n0 -= h0 + 1;
+      } else {
+        f0 |= x0 == v0;
+        n0 = h0;
+      }
+    }
+    if (n1) {
+      if (x1 < v1) {
+        lo1 += h1 + 1;
+        n1 -= h1 + 1;
+      } else {
+        f1 |= x1 == v1;
+        n1 = h1;
+      }
+    }
+  }
+  k0 = k0 && f0;
+  k1 = k1 && f1;
+}
+
 // Map a hardware workgroup id to a logical one so that consecutive logical
 // blocks run on the same XCD (workgroups are dispatched round-robin over the
 // 8 XCDs); neighbouring row blocks then share one L2.
