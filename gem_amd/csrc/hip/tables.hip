@@ -44,8 +44,9 @@ __device__ __forceinline__ uint64_t lane_range(uint32_t a, uint32_t b) {  // bit
   return hi & ~((1ull << a) - 1);
 }
 
-// Rows longer than HEAVY_ROW are walked by k_tables_heavy, one wave per row,
-// so that a wave of 64 hub rows cannot become a straggler.
+// Rows longer than HEAVY_ROW are walked by k_tables_heavy in fixed-size
+// chunks spread over waves, so that neither a wave of 64 hub rows nor one
+// hub's row can become a straggler.
 constexpr uint32_t HEAVY_ROW = 256;
 
 template <bool WRITE>
@@ -137,29 +138,77 @@ k_tables(const uint64_t *__restrict__ goff, const uint32_t *__restrict__ gnbr, c
   }
 }
 
-// One wave per heavy row (persistent grid over the list built by the light
-// pass); lane j carries table j's running hit count across 64-item steps.
+// Heavy rows are walked in chunks of TCH adjacency entries, one wave task per
+// chunk, so that a hub's row is spread over many waves.  The count pass stores
+// each chunk's hit count per table (ccnt[chunk * ndmax + j]); k_theavy_bases
+// turns those into each chunk's offset inside its row (exclusive over the
+// row's chunks) and writes the rows' totals; the write pass starts each
+// chunk's hits at that offset, so values stay in adjacency order.
+constexpr uint32_t TCH = 2048;
+
+// Chunk prefix of the heavy rows: cpre[h] = chunks of rows before h,
+// cpre[n] = *total = all chunks (one block).
+__global__ void __launch_bounds__(1024) k_theavy_prefix(const uint64_t *__restrict__ goff,
+                                                       const uint32_t *__restrict__ can, const TableJob job,
+                                                       const uint64_t *__restrict__ heavy_list,
+                                                       const uint32_t *__restrict__ heavy_n, uint64_t *__restrict__ cpre,
+                                                       uint64_t *__restrict__ total) {
+  __shared__ uint64_t sm[16];
+  const uint32_t n = *heavy_n;
+  uint64_t carry = 0;
+  for (uint32_t b = 0; b < n; b += blockDim.x) {
+    const uint32_t h = b + threadIdx.x;
+    uint64_t x = 0;
+    if (h < n) {
+      const uint64_t e = heavy_list[h];
+      const uint32_t v = can[job.can_off[(uint32_t)(e >> 32)] + (uint32_t)e];
+      x = (goff[v + 1] - goff[v] + TCH - 1) / TCH;
+    }
+    uint64_t ex;
+    const uint64_t tot = block_excl_scan_u64(x, &ex, sm);
+    if (h < n) cpre[h] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    cpre[n] = carry;
+    *total = carry;
+  }
+}
+
+// Persistent grid over the chunks; lane j carries table j's running count.
 template <bool WRITE>
 __global__ void __launch_bounds__(TAB_WAVES * 64)
 k_tables_heavy(const uint64_t *__restrict__ goff, const uint32_t *__restrict__ gnbr, const uint32_t *__restrict__ can,
                const uint64_t *__restrict__ bitmap, const uint32_t *__restrict__ rank, const TableJob job,
                const TableDesc *__restrict__ desc, unsigned long long *__restrict__ adj_len,
-               const uint64_t *__restrict__ heavy_list, const uint32_t *__restrict__ heavy_n) {
+               const uint64_t *__restrict__ heavy_list, const uint32_t *__restrict__ heavy_n,
+               const uint64_t *__restrict__ cpre, uint32_t *__restrict__ ccnt, uint32_t ndmax) {
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const uint32_t nh = *heavy_n, W = job.W;
-  for (uint32_t h = blockIdx.x * TAB_WAVES + w; h < nh; h += gridDim.x * TAB_WAVES) {
-    const uint64_t e = heavy_list[h];
+  const uint64_t nchunks = cpre[nh];
+  for (uint64_t c = (uint64_t)blockIdx.x * TAB_WAVES + w; c < nchunks; c += (uint64_t)gridDim.x * TAB_WAVES) {
+    uint32_t lo = 0, hi = nh;  // the chunk's row: largest h with cpre[h] <= c
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (cpre[mid] <= c)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint64_t e = heavy_list[lo];
     const uint32_t ju = (uint32_t)(e >> 32), i = (uint32_t)e;
     const uint32_t v = can[job.can_off[ju] + i];
     const uint64_t o0 = goff[v], len = goff[v + 1] - o0;
-    if (!WRITE && l == 0) atomicAdd(adj_len + ju, (unsigned long long)len);
-    const uint32_t nd = job.nd[ju], t0 = job.tab0[ju];
+    const uint64_t t0 = (c - cpre[lo]) * TCH, t1 = t0 + TCH < len ? t0 + TCH : len;
+    if (!WRITE && l == 0) atomicAdd(adj_len + ju, (unsigned long long)(t1 - t0));
+    const uint32_t nd = job.nd[ju], tb = job.tab0[ju];
     uint32_t run_reg = 0;
-    for (uint64_t t = 0; t < len; t += 64) {
-      const bool iv = t + l < len;
+    if (WRITE && l < nd) run_reg = ccnt[c * ndmax + l];
+    for (uint64_t t = t0; t < t1; t += 64) {
+      const bool iv = t + l < t1;
       const uint32_t wv = iv ? gnbr[o0 + t + l] : 0;
       for (uint32_t j = 0; j < nd; ++j) {
-        const TableDesc d = desc[t0 + j];
+        const TableDesc d = desc[tb + j];
         bool hit = false;
         uint64_t word = 0;
         if (iv) {
@@ -176,8 +225,28 @@ k_tables_heavy(const uint64_t *__restrict__ goff, const uint32_t *__restrict__ g
         if (l == j) run_reg = run + (uint32_t)__popcll(m);
       }
     }
-    if (!WRITE && l < nd) desc[t0 + l].off[i] = run_reg;
+    if (!WRITE && l < nd) ccnt[c * ndmax + l] = run_reg;
   }
+}
+
+// One thread per (heavy row, table slot): chunk counts -> chunk offsets inside
+// the row, and the row's total as its count.
+__global__ void k_theavy_bases(const TableJob job, const TableDesc *__restrict__ desc,
+                               const uint64_t *__restrict__ heavy_list, const uint32_t *__restrict__ heavy_n,
+                               const uint64_t *__restrict__ cpre, uint32_t *__restrict__ ccnt, uint32_t ndmax) {
+  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= (uint64_t)(*heavy_n) * ndmax) return;
+  const uint32_t h = (uint32_t)(x / ndmax), j = (uint32_t)(x % ndmax);
+  const uint64_t e = heavy_list[h];
+  const uint32_t ju = (uint32_t)(e >> 32), i = (uint32_t)e;
+  if (j >= job.nd[ju]) return;
+  uint32_t run = 0;
+  for (uint64_t c = cpre[h]; c < cpre[h + 1]; ++c) {
+    const uint32_t n = ccnt[c * ndmax + j];
+    ccnt[c * ndmax + j] = run;
+    run += n;
+  }
+  desc[job.tab0[ju] + j].off[i] = run;
 }
 
 void run_tables(gem_ctx *c) { run_tables(c, c->q); }
@@ -234,32 +303,69 @@ void run_tables(gem_ctx *c, QueryDev &Qd) {
   heavy_list.alloc(std::max<uint64_t>(rows_total, 1) * 8, s);
   heavy_n.alloc(4, s);
   GEM_HIP_CHECK(hipMemsetAsync(heavy_n.p, 0, 4, s));
-  const uint32_t heavy_blocks = 512;  // persistent: 2048 waves walk the heavy rows
-  auto launch = [&](bool write) {
+  uint32_t ndmax = 1;
+  for (uint32_t ju = 0; ju < job.nu; ++ju) ndmax = std::max(ndmax, job.nd[ju]);
+  DevBuf cpre, ccnt;
+  cpre.alloc((std::max<uint64_t>(rows_total, 1) + 2) * 8, s);
+  uint64_t *d_chunks = cpre.as<uint64_t>() + std::max<uint64_t>(rows_total, 1) + 1;
+  uint32_t nh = 0;
+  uint64_t nchunks = 0;
+  auto launch_light = [&](bool write) {
     if (!blocks) return;
-    ProfLaunch pl(c, write ? K_TABLES_WRITE : K_TABLES_COUNT);
-    if (write) {
+    if (write)
       k_tables<true><<<blocks, TAB_WAVES * 64, 0, s>>>(
           G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
           Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
           heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>());
-      k_tables_heavy<true><<<heavy_blocks, TAB_WAVES * 64, 0, s>>>(
-          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
-          Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
-          heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>());
-    } else {
+    else
       k_tables<false><<<blocks, TAB_WAVES * 64, 0, s>>>(
           G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
           Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
           heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>());
-      k_tables_heavy<false><<<heavy_blocks, TAB_WAVES * 64, 0, s>>>(
-          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
-          Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
-          heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>());
-    }
     GEM_LAUNCH_CHECK(write ? "k_tables<write>" : "k_tables<count>");
   };
-  launch(false);
+  auto launch_heavy = [&](bool write) {
+    if (!nchunks) return;
+    const uint32_t hb = (uint32_t)std::min<uint64_t>((nchunks + TAB_WAVES - 1) / TAB_WAVES, 2048);
+    if (write)
+      k_tables_heavy<true><<<hb, TAB_WAVES * 64, 0, s>>>(
+          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
+          Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
+          heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>(), cpre.as<uint64_t>(), ccnt.as<uint32_t>(), ndmax);
+    else
+      k_tables_heavy<false><<<hb, TAB_WAVES * 64, 0, s>>>(
+          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
+          Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
+          heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>(), cpre.as<uint64_t>(), ccnt.as<uint32_t>(), ndmax);
+    GEM_LAUNCH_CHECK(write ? "k_tables_heavy<write>" : "k_tables_heavy<count>");
+  };
+  {
+    ProfLaunch pl(c, K_TABLES_COUNT);
+    launch_light(false);
+    // rows longer than HEAVY_ROW: chunk prefix, then (sizes known) the chunked count
+    k_theavy_prefix<<<1, 1024, 0, s>>>(G.off.as<uint64_t>(), Qd.can.as<uint32_t>(), job, heavy_list.as<uint64_t>(),
+                                       heavy_n.as<uint32_t>(), cpre.as<uint64_t>(), d_chunks);
+    GEM_LAUNCH_CHECK("k_theavy_prefix");
+  }
+  {
+    uint64_t *hh = pinned<uint64_t>(c, 2);
+    hh[0] = 0;
+    GEM_HIP_CHECK(hipMemcpyAsync(hh, heavy_n.p, 4, hipMemcpyDeviceToHost, s));
+    GEM_HIP_CHECK(hipMemcpyAsync(hh + 1, d_chunks, 8, hipMemcpyDeviceToHost, s));
+    GEM_HIP_CHECK(hipStreamSynchronize(s));
+    nh = (uint32_t)hh[0];
+    nchunks = hh[1];
+  }
+  if (nchunks) {
+    ccnt.alloc(nchunks * ndmax * 4, s);
+    ProfLaunch pl(c, K_TABLES_COUNT);
+    launch_heavy(false);
+    const uint64_t nthreads = (uint64_t)nh * ndmax;
+    k_theavy_bases<<<(uint32_t)((nthreads + 255) / 256), 256, 0, s>>>(
+        job, ddesc.as<TableDesc>(), heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>(), cpre.as<uint64_t>(),
+        ccnt.as<uint32_t>(), ndmax);
+    GEM_LAUNCH_CHECK("k_theavy_bases");
+  }
   const size_t nt = desc_tab.size();
   uint64_t *h = pinned<uint64_t>(c, nt + GEM_MAX_QV);
   for (size_t k = 0; k < nt; ++k) {
@@ -286,7 +392,11 @@ void run_tables(gem_ctx *c, QueryDev &Qd) {
   prof_bytes(c, K_TABLES_WRITE, shared_bytes + write_bytes);
   fill_desc();
   GEM_HIP_CHECK(hipMemcpyAsync(ddesc.p, desc.data(), desc.size() * sizeof(TableDesc), hipMemcpyHostToDevice, s));
-  launch(true);
+  {
+    ProfLaunch pl(c, K_TABLES_WRITE);
+    launch_light(true);
+    launch_heavy(true);
+  }
   GEM_HIP_CHECK(hipEventRecord(c->ev[3], s));
   GEM_HIP_CHECK(hipStreamSynchronize(s));
   prof_flush(c);

@@ -150,3 +150,65 @@ def test_query_errors(gpu_ctx, er_graph):
     # a label absent from the data graph: no candidates, count 0
     c, s, _ = gpu_ctx.match(Query(np.array([0, 1, 999], np.uint32), np.array([[0, 1], [1, 2]], np.uint32)))
     assert c == 0 and not s
+
+
+@pytest.fixture(scope="module")
+def hub_graph():
+    """30,000 vertices, 3 labels, ~60,000 random edges plus 8 hubs of degree
+    3,000-9,000: candidate rows longer than one 2,048-entry table chunk and
+    driving lists longer than the walk kernels' 1,024-entry heavy threshold."""
+    rng = np.random.default_rng(77)
+    n = 30000
+    pairs = set()
+    while len(pairs) < 60000:
+        a, b = (int(x) for x in rng.integers(0, n, 2))
+        if a != b:
+            pairs.add((min(a, b), max(a, b)))
+    for h in range(8):
+        for b in rng.choice(np.arange(8, n), size=int(rng.integers(3000, 9000)), replace=False):
+            pairs.add((h, int(b)))
+    labels = rng.integers(0, 3, n).astype(np.uint32)
+    return Graph.from_edges(n, sorted(pairs), labels)
+
+
+def _hub_queries(g):
+    lab = g.labels
+    l0, l1, l2 = int(lab[0]), int(lab[1]), int(lab[2])
+    return [Query([l0, l1, l2], [(0, 1), (1, 2)]),
+            Query([l0, l1, l2], [(0, 1), (0, 2), (1, 2)]),
+            Query([l1, l0, l2, l1], [(0, 1), (1, 2), (2, 3), (0, 3)])]
+
+
+def test_edge_tables_hub_rows(gpu_ctx, hub_graph):
+    """Tables whose rows span several chunks of the chunked heavy-row pass."""
+    g = hub_graph
+    assert int(np.diff(g.offsets).max()) > 2 * 2048
+    gpu_ctx.load_data(g)
+    for q in _hub_queries(g):
+        print("hub tables query", q.labels.tolist(), q.edges.tolist(), flush=True)
+        num, lists = gpu_ctx.filter(q)
+        sizes = gpu_ctx.build_edge_tables()
+        assert np.array_equal(sizes, oracle.table_sizes(g, q))
+        deg = np.bincount(q.edges.reshape(-1), minlength=q.num_v)
+        for d in range(2 * q.num_e):
+            key = q.edges[d // 2][d % 2]
+            if deg[key] == 1:
+                continue
+            off, val = gpu_ctx.read_edge_table(d, int(num[key]), sizes[d])
+            ref = oracle.edge_table(g, q, d, int(num[key]), sizes[d])
+            assert np.array_equal(off, ref[0])
+            assert np.array_equal(val, ref[2])
+
+
+def test_counts_hub_rows(gpu_ctx, hub_graph):
+    """Plain and batched counts on hub rows (heavy walk kernels) equal the oracle's."""
+    g = hub_graph
+    gpu_ctx.load_data(g)
+    qs = _hub_queries(g)
+    res = gpu_ctx.match_batch(qs)
+    for q, r in zip(qs, res):
+        ref = oracle.count(g, q)
+        print("hub count query", q.labels.tolist(), q.edges.tolist(), ref["count"], r["count"], flush=True)
+        assert r["count"] == ref["count"] and not r["saturated"] and not r["work_limited"]
+        c, s, _ = gpu_ctx.match(q)
+        assert (c, s) == (ref["count"], False)
