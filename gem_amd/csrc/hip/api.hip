@@ -140,6 +140,7 @@ int gem_hip_load_data_csr(gem_ctx *c, const gem_csr_view *g) {
     for (uint32_t v = 0; v < g->num_v; ++v) {
       if (g->offsets[v + 1] < g->offsets[v]) throw Error{GEM_E_INVALID, "offsets not monotone"};
       G.max_deg = std::max<uint64_t>(G.max_deg, g->offsets[v + 1] - g->offsets[v]);
+      if (g->offsets[v + 1] - g->offsets[v] > gemd::FILTER_HUB_DEG) ++G.n_hub;
     }
     // dense label ids in first-seen order
     std::vector<uint32_t> dense(g->num_v);

@@ -100,6 +100,7 @@ struct GraphDev {
   uint32_t n = 0;
   uint64_t slots = 0;
   uint64_t max_deg = 0;
+  uint32_t n_hub = 0;  // vertices of degree > FILTER_HUB_DEG
   DevBuf off;     // u64[n + 1]
   DevBuf nbr;     // u32[slots]
   DevBuf lab;     // u8[n] or u32[n] dense label ids
@@ -108,6 +109,10 @@ struct GraphDev {
   std::vector<uint64_t> label_freq;                 // dense id -> #vertices
   std::vector<uint64_t> label_deg;                  // dense id -> sum of degrees
 };
+
+// The filter walks the adjacency of vertices of larger degree than this in
+// chunks of this many entries spread over waves.
+constexpr uint32_t FILTER_HUB_DEG = 4096;
 
 // Query description consumed by the filter kernel (device copy).
 struct FilterQuery {

@@ -103,16 +103,36 @@ k_filter(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr, con
   }
 }
 
-// Vertices of degree > LIGHT_DEG: one wave per vertex (persistent grid over
-// the list k_filter built).  Neighbour labels are counted 64 at a time, one
-// ballot + popcount per query label (lane s keeps label s's count); lane u then
-// decides query vertex u and sets its bit with an atomic OR.
+// Query vertex u's decision for data vertex v of degree deg whose neighbour
+// label counts (per query label slot) are cnt: lane u sets v's bit.
+__device__ __forceinline__ void filter_decide(const FilterQuery &fq, uint32_t l, uint32_t v, uint64_t deg, uint32_t g,
+                                              const uint32_t *cnt, unsigned long long *bitmap,
+                                              uint32_t *tile_count) {
+  if (l < fq.nv && ((g >> l) & 1u) && (uint64_t)fq.qdeg[l] <= deg + fq.T) {
+    uint32_t deficit = 0;
+    for (uint32_t r = 0; r < fq.nreq[l]; ++r) {
+      const uint32_t need = fq.req_cnt[l][r], have = cnt[fq.req_slot[l][r]];
+      deficit += need > have ? need - have : 0;
+    }
+    if (deficit <= fq.T) {
+      atomicOr(&bitmap[(uint64_t)l * fq.W + (v >> 6)], 1ull << (v & 63));
+      atomicAdd(&tile_count[(uint64_t)l * fq.W + (v >> 6)], 1u);
+    }
+  }
+}
+
+// Vertices of degree in (LIGHT_DEG, FILTER_HUB_DEG]: one wave per vertex
+// (persistent grid over the list k_filter built); larger ones are passed on to
+// the hub list for k_filter_hub.  Neighbour labels are counted 64 at a time,
+// one ballot + popcount per query label (lane s keeps label s's count); lane u
+// then decides query vertex u and sets its bit with an atomic OR.
 template <class LabT>
 __global__ void __launch_bounds__(FILTER_WAVES * 64)
 k_filter_heavy(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr, const LabT *__restrict__ lab,
                const FilterQuery *__restrict__ fqp, const uint8_t *__restrict__ slot_of,
                unsigned long long *__restrict__ bitmap, uint32_t *__restrict__ tile_count,
-               const uint32_t *__restrict__ heavy_list, const uint32_t *__restrict__ heavy_n) {
+               const uint32_t *__restrict__ heavy_list, const uint32_t *__restrict__ heavy_n,
+               uint32_t *__restrict__ hub_list, uint32_t *__restrict__ hub_n, uint32_t hub_cap) {
   __shared__ FilterQuery fq;
   __shared__ uint32_t cnt[FILTER_WAVES][GEM_MAX_QV];
   {
@@ -123,10 +143,17 @@ k_filter_heavy(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nb
   __syncthreads();
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const uint32_t nh = *heavy_n;
-  const uint32_t nslots = fq.nslots, T = fq.T;
+  const uint32_t nslots = fq.nslots;
   for (uint32_t i = blockIdx.x * FILTER_WAVES + w; i < nh; i += gridDim.x * FILTER_WAVES) {
     const uint32_t v = heavy_list[i];
     const uint64_t o0 = off[v], deg = off[v + 1] - o0;
+    if (deg > FILTER_HUB_DEG) {
+      if (l == 0) {
+        const uint32_t h = atomicAdd(hub_n, 1u);
+        if (h < hub_cap) hub_list[h] = v;  // hub_cap = all such vertices of the graph
+      }
+      continue;
+    }
     const uint32_t g = fq.group[slot_of[lab[v]]];
     uint32_t acc = 0;
     for (uint64_t j = 0; j < deg; j += 64) {
@@ -140,19 +167,66 @@ k_filter_heavy(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nb
     if (l < nslots) cnt[w][l] = acc;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (l < fq.nv && ((g >> l) & 1u) && (uint64_t)fq.qdeg[l] <= deg + T) {
-      uint32_t deficit = 0;
-      for (uint32_t r = 0; r < fq.nreq[l]; ++r) {
-        const uint32_t need = fq.req_cnt[l][r], have = cnt[w][fq.req_slot[l][r]];
-        deficit += need > have ? need - have : 0;
-      }
-      if (deficit <= T) {
-        atomicOr(&bitmap[(uint64_t)l * fq.W + (v >> 6)], 1ull << (v & 63));
-        atomicAdd(&tile_count[(uint64_t)l * fq.W + (v >> 6)], 1u);
-      }
-    }
+    filter_decide(fq, l, v, deg, g, cnt[w], bitmap, tile_count);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+}
+
+// Hubs (degree > FILTER_HUB_DEG): task t = (hub t / maxc, chunk t % maxc) of
+// FILTER_HUB_DEG adjacency entries; label counts are added into
+// hub_cnt[hub][slot] (zeroed by the host).  maxc = chunks of the largest degree.
+template <class LabT>
+__global__ void __launch_bounds__(FILTER_WAVES * 64)
+k_filter_hub(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr, const LabT *__restrict__ lab,
+             const FilterQuery *__restrict__ fqp, const uint8_t *__restrict__ slot_of,
+             const uint32_t *__restrict__ hub_list, const uint32_t *__restrict__ hub_n, uint32_t hub_cap,
+             uint32_t maxc, uint32_t *__restrict__ hub_cnt) {
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const uint32_t nslots = fqp->nslots;
+  const uint32_t nh = *hub_n < hub_cap ? *hub_n : hub_cap;
+  const uint64_t ntask = (uint64_t)nh * maxc;
+  for (uint64_t t = (uint64_t)blockIdx.x * FILTER_WAVES + w; t < ntask; t += (uint64_t)gridDim.x * FILTER_WAVES) {
+    const uint32_t h = (uint32_t)(t / maxc), ch = (uint32_t)(t % maxc);
+    const uint32_t v = hub_list[h];
+    const uint64_t o0 = off[v], deg = off[v + 1] - o0;
+    const uint64_t j0 = (uint64_t)ch * FILTER_HUB_DEG;
+    if (j0 >= deg) continue;
+    const uint64_t j1 = j0 + FILTER_HUB_DEG < deg ? j0 + FILTER_HUB_DEG : deg;
+    uint32_t acc = 0;
+    for (uint64_t j = j0; j < j1; j += 64) {
+      uint32_t s = NO_SLOT;
+      if (j + l < j1) s = slot_of[lab[nbr[o0 + j + l]]];
+      for (uint32_t q = 0; q < nslots; ++q) {
+        const uint32_t c = (uint32_t)__popcll(__ballot(s == q));
+        if (l == q) acc += c;
+      }
+    }
+    if (l < nslots && acc) atomicAdd(&hub_cnt[(uint64_t)h * GEM_MAX_QV + l], acc);
+  }
+}
+
+// One wave per hub: the filter decision from the summed label counts.
+template <class LabT>
+__global__ void __launch_bounds__(FILTER_WAVES * 64)
+k_filter_hub_decide(const uint64_t *__restrict__ off, const LabT *__restrict__ lab,
+                    const FilterQuery *__restrict__ fqp, const uint8_t *__restrict__ slot_of,
+                    const uint32_t *__restrict__ hub_list, const uint32_t *__restrict__ hub_n, uint32_t hub_cap,
+                    const uint32_t *__restrict__ hub_cnt, unsigned long long *__restrict__ bitmap,
+                    uint32_t *__restrict__ tile_count) {
+  __shared__ FilterQuery fq;
+  {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(fqp);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&fq);
+    for (uint32_t i = threadIdx.x; i < sizeof(FilterQuery) / 4; i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const uint32_t nh = *hub_n < hub_cap ? *hub_n : hub_cap;
+  for (uint32_t h = blockIdx.x * FILTER_WAVES + w; h < nh; h += gridDim.x * FILTER_WAVES) {
+    const uint32_t v = hub_list[h];
+    const uint64_t deg = off[v + 1] - off[v];
+    filter_decide(fq, l, v, deg, fq.group[slot_of[lab[v]]], hub_cnt + (uint64_t)h * GEM_MAX_QV, bitmap, tile_count);
   }
 }
 
@@ -276,6 +350,14 @@ void run_filter(gem_ctx *c, QueryDev &Qd, const gem_query_view *qv, uint32_t T) 
           heavy_n.as<uint32_t>());
     GEM_LAUNCH_CHECK("k_filter");
   }
+  const uint32_t hub_cap = std::max<uint32_t>(G.n_hub, 1);
+  const uint32_t maxc = (uint32_t)((G.max_deg + FILTER_HUB_DEG - 1) / FILTER_HUB_DEG);
+  DevBuf hub_list, hub_n, hub_cnt;
+  hub_list.alloc((uint64_t)hub_cap * 4, s);
+  hub_n.alloc(4, s);
+  hub_cnt.alloc((uint64_t)hub_cap * GEM_MAX_QV * 4, s);
+  GEM_HIP_CHECK(hipMemsetAsync(hub_n.p, 0, 4, s));
+  if (G.n_hub) GEM_HIP_CHECK(hipMemsetAsync(hub_cnt.p, 0, (uint64_t)hub_cap * GEM_MAX_QV * 4, s));
   {
     ProfLaunch pl(c, K_FILTER_HEAVY);
     const uint32_t hb = 1024;  // persistent: 4096 waves walk the heavy list
@@ -283,13 +365,40 @@ void run_filter(gem_ctx *c, QueryDev &Qd, const gem_query_view *qv, uint32_t T) 
       k_filter_heavy<uint8_t><<<hb, FILTER_WAVES * 64, 0, s>>>(
           G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint8_t>(), Qd.fq.as<FilterQuery>(),
           Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<unsigned long long>(), tile_count.as<uint32_t>(),
-          heavy_list.as<uint32_t>(), heavy_n.as<uint32_t>());
+          heavy_list.as<uint32_t>(), heavy_n.as<uint32_t>(), hub_list.as<uint32_t>(), hub_n.as<uint32_t>(),
+          hub_cap);
     else
       k_filter_heavy<uint32_t><<<hb, FILTER_WAVES * 64, 0, s>>>(
           G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint32_t>(), Qd.fq.as<FilterQuery>(),
           Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<unsigned long long>(), tile_count.as<uint32_t>(),
-          heavy_list.as<uint32_t>(), heavy_n.as<uint32_t>());
+          heavy_list.as<uint32_t>(), heavy_n.as<uint32_t>(), hub_list.as<uint32_t>(), hub_n.as<uint32_t>(),
+          hub_cap);
     GEM_LAUNCH_CHECK("k_filter_heavy");
+    if (G.n_hub) {
+      const uint32_t tb = (uint32_t)std::min<uint64_t>(((uint64_t)G.n_hub * maxc + FILTER_WAVES - 1) / FILTER_WAVES,
+                                                       2048);
+      const uint32_t db = (G.n_hub + FILTER_WAVES - 1) / FILTER_WAVES;
+      if (G.lab8) {
+        k_filter_hub<uint8_t><<<tb, FILTER_WAVES * 64, 0, s>>>(
+            G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint8_t>(), Qd.fq.as<FilterQuery>(),
+            Qd.slot_of.as<uint8_t>(), hub_list.as<uint32_t>(), hub_n.as<uint32_t>(), hub_cap, maxc,
+            hub_cnt.as<uint32_t>());
+        k_filter_hub_decide<uint8_t><<<db, FILTER_WAVES * 64, 0, s>>>(
+            G.off.as<uint64_t>(), G.lab.as<uint8_t>(), Qd.fq.as<FilterQuery>(), Qd.slot_of.as<uint8_t>(),
+            hub_list.as<uint32_t>(), hub_n.as<uint32_t>(), hub_cap, hub_cnt.as<uint32_t>(),
+            Qd.bitmap.as<unsigned long long>(), tile_count.as<uint32_t>());
+      } else {
+        k_filter_hub<uint32_t><<<tb, FILTER_WAVES * 64, 0, s>>>(
+            G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint32_t>(), Qd.fq.as<FilterQuery>(),
+            Qd.slot_of.as<uint8_t>(), hub_list.as<uint32_t>(), hub_n.as<uint32_t>(), hub_cap, maxc,
+            hub_cnt.as<uint32_t>());
+        k_filter_hub_decide<uint32_t><<<db, FILTER_WAVES * 64, 0, s>>>(
+            G.off.as<uint64_t>(), G.lab.as<uint32_t>(), Qd.fq.as<FilterQuery>(), Qd.slot_of.as<uint8_t>(),
+            hub_list.as<uint32_t>(), hub_n.as<uint32_t>(), hub_cap, hub_cnt.as<uint32_t>(),
+            Qd.bitmap.as<unsigned long long>(), tile_count.as<uint32_t>());
+      }
+      GEM_LAUNCH_CHECK("k_filter_hub");
+    }
   }
   // algorithmic bytes: 16 B per vertex (label, degree, offset pair), 8 B per
   // adjacency slot of a label-matching vertex (neighbour id + its label), the
