@@ -284,12 +284,15 @@ __global__ void __launch_bounds__(1024) k_heavy_scan(const HeavyRow *__restrict_
   if (threadIdx.x == 0) pre[n] = carry;
 }
 
+// Entries of a staged window of the first other list (k_vext_heavy).
+constexpr uint32_t HWIN = 512;
+
 // Heavy rows: each wave task walks `chunk` entries of one row's driving list
 // (the same list the main pass picked: shortest, first on ties), probing the
 // row's other lists and equal-label columns; survivors are counted or appended
 // to the row's item output like the main pass's.
 template <bool FINAL, bool OOC>
-__global__ void __launch_bounds__(VB) k_vext_heavy(const VItem *__restrict__ items,
+__global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8))) k_vext_heavy(const VItem *__restrict__ items,
                                                   const HeavyRow *__restrict__ heavy,
                                                   const unsigned int *__restrict__ heavy_n, uint32_t heavy_cap,
                                                   const uint64_t *__restrict__ pre, unsigned int *__restrict__ queue,
@@ -297,6 +300,8 @@ __global__ void __launch_bounds__(VB) k_vext_heavy(const VItem *__restrict__ ite
                                                   unsigned long long *__restrict__ prof_out) {
   __shared__ uint64_t oaddr[VW][GEM_MAX_QV];
   __shared__ uint32_t olen[VW][GEM_MAX_QV], dpos[VW][GEM_MAX_QV];
+  __shared__ uint32_t wend[VW][64];   // window ends in the first other list
+  __shared__ uint32_t win[VW][HWIN];  // staged window
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const uint32_t n = *heavy_n < heavy_cap ? *heavy_n : heavy_cap;
   const uint64_t total = pre[n];
@@ -344,41 +349,81 @@ __global__ void __launch_bounds__(VB) k_vext_heavy(const VItem *__restrict__ ite
     const uint64_t s1 = s0 + chunk < dlen ? s0 + chunk : dlen;
     walked += s1 - s0;
     uint64_t survivors = 0;
-    // two items per lane per step, as in k_vext
-    for (uint64_t s = s0; s < s1; s += 128) {
-      bool keep0 = s + l < s1, keep1 = s + 64 + l < s1;
-      uint32_t val0 = 0, val1 = 0;
-      if (keep0) val0 = dlist[s + l];
-      if (keep1) val1 = dlist[s + 64 + l];
-      for (uint32_t c = 0; c < ndup; ++c) {
-        if (dpos[w][c] == val0) keep0 = false;
-        if (dpos[w][c] == val1) keep1 = false;
-      }
-      for (uint32_t o = 0; o + 1 < nb; ++o) {
-        const uint32_t *a = reinterpret_cast<const uint32_t *>(oaddr[w][o]);
-        contains_pair(a, olen[w][o], val0, keep0, a, olen[w][o], val1, keep1);
-      }
-      const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
-      const uint32_t nk0 = (uint32_t)__popcll(m0), nk = nk0 + (uint32_t)__popcll(m1);
-      if (FINAL) {
-        survivors += nk;
-      } else if (nk) {
-        uint64_t base = 0;
-        if (l == 0) base = atomicAdd(it.counter, (unsigned long long)nk);
-        base = readlane_u64(base, 0);
-        written += (uint64_t)nk * 4 * (k + 1);
-        const uint64_t o0 = base + rank_below(m0), o1 = base + nk0 + rank_below(m1);
-        if (keep0 && o0 < it.out_cap) {
-          uint32_t *dst = it.out + o0 * (k + 1);
-          for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
-          dst[k] = val0;
+    // Two items per lane per step, as in k_vext.  The first other list is
+    // probed through windows: for each 128-entry step of the chunk, the range
+    // of that list that can hold the step's values (both lists are sorted and
+    // duplicate-free) is found for up to 63 steps at once (lane j: the end of
+    // step j, lane 63: the start of the first step), and a window of at most
+    // HWIN entries is staged in LDS with coalesced loads and searched there.
+    const uint32_t *l0p = nb > 1 ? reinterpret_cast<const uint32_t *>(oaddr[w][0]) : nullptr;
+    const uint32_t l0n = nb > 1 ? olen[w][0] : 0;
+    for (uint64_t S = s0; S < s1; S += 63 * 128) {
+      const uint64_t S1 = S + 63 * 128 < s1 ? S + 63 * 128 : s1;
+      const uint32_t nsub = (uint32_t)((S1 - S + 127) / 128);
+      if (nb > 1) {
+        uint32_t pos = 0;
+        if (l < nsub) {
+          const uint64_t last = (S + 128ull * (l + 1) < S1 ? S + 128ull * (l + 1) : S1) - 1;
+          pos = upper_bound_u32(l0p, l0n, dlist[last]);
+        } else if (l == 63) {
+          pos = lower_bound_u32(l0p, l0n, dlist[S]);
         }
-        if (keep1 && o1 < it.out_cap) {
-          uint32_t *dst = it.out + o1 * (k + 1);
-          for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
-          dst[k] = val1;
+        wend[w][l] = pos;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+      for (uint32_t j = 0; j < nsub; ++j) {
+        const uint64_t s = S + 128ull * j;
+        bool keep0 = s + l < S1, keep1 = s + 64 + l < S1;
+        uint32_t val0 = 0, val1 = 0;
+        if (keep0) val0 = dlist[s + l];
+        if (keep1) val1 = dlist[s + 64 + l];
+        for (uint32_t c = 0; c < ndup; ++c) {
+          if (dpos[w][c] == val0) keep0 = false;
+          if (dpos[w][c] == val1) keep1 = false;
+        }
+        if (nb > 1) {
+          const uint32_t a = j ? wend[w][j - 1] : wend[w][63], wn = wend[w][j] - a;
+          if (wn <= HWIN) {
+            for (uint32_t i = l; i < wn; i += 64) win[w][i] = l0p[a + i];
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            contains_pair(win[w], wn, val0, keep0, win[w], wn, val1, keep1);
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          } else {
+            contains_pair(l0p + a, wn, val0, keep0, l0p + a, wn, val1, keep1);
+          }
+        }
+        for (uint32_t o = 1; o + 1 < nb; ++o) {
+          const uint32_t *a = reinterpret_cast<const uint32_t *>(oaddr[w][o]);
+          contains_pair(a, olen[w][o], val0, keep0, a, olen[w][o], val1, keep1);
+        }
+        const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
+        const uint32_t nk0 = (uint32_t)__popcll(m0), nk = nk0 + (uint32_t)__popcll(m1);
+        if (FINAL) {
+          survivors += nk;
+        } else if (nk) {
+          uint64_t base = 0;
+          if (l == 0) base = atomicAdd(it.counter, (unsigned long long)nk);
+          base = readlane_u64(base, 0);
+          written += (uint64_t)nk * 4 * (k + 1);
+          const uint64_t o0 = base + rank_below(m0), o1 = base + nk0 + rank_below(m1);
+          if (keep0 && o0 < it.out_cap) {
+            uint32_t *dst = it.out + o0 * (k + 1);
+            for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
+            dst[k] = val0;
+          }
+          if (keep1 && o1 < it.out_cap) {
+            uint32_t *dst = it.out + o1 * (k + 1);
+            for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
+            dst[k] = val1;
+          }
         }
       }
+      // wend of this wave is rewritten by the next group of steps
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
     if (FINAL && l == 0 && survivors) atomicAdd(it.counter, (unsigned long long)survivors);
     __builtin_amdgcn_wave_barrier();

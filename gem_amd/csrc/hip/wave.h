@@ -90,6 +90,34 @@ __device__ __forceinline__ bool sorted_contains(const T *a, uint64_t n, T v) {
   return lo < n && a[lo] == v;
 }
 
+// First index of sorted a[0..n) whose value is >= v (lower) or > v (upper).
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t n, uint32_t v) {
+  uint32_t lo = 0;
+  while (n) {
+    const uint32_t h = n >> 1;
+    if (a[lo + h] < v) {
+      lo += h + 1;
+      n -= h + 1;
+    } else {
+      n = h;
+    }
+  }
+  return lo;
+}
+__device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t n, uint32_t v) {
+  uint32_t lo = 0;
+  while (n) {
+    const uint32_t h = n >> 1;
+    if (a[lo + h] <= v) {
+      lo += h + 1;
+      n -= h + 1;
+    } else {
+      n = h;
+    }
+  }
+  return lo;
+}
+
 // Two membership tests at once: is v0 in sorted a0[0..n0) and v1 in
 // a1[0..n1)?  k0 / k1 enter as "still wanted" (a false one is not searched)
 // and leave as the result.  The two lower-bound searches advance in lockstep
