@@ -114,8 +114,8 @@ __device__ __forceinline__ uint64_t list_addr(const VLevel &L, uint32_t j, uint3
 // Dynamic LDS per wave: other lists (nb-1) x 64 x (u64 address + u32 len),
 // dup positions ndup x 64 u32.
 // Survivors are written as whole rows: their (parent row, new value) pairs are
-// staged in LDS, then each half-wave copies one row (k + 1 <= 32 consecutive
-// words).
+// staged in LDS, then the wave writes the step's new rows as one run of
+// consecutive words.
 // The item count is read from device memory so that one captured graph can
 // replay the launch for any number of items.  Tasks [g_begin, g_end) of the
 // launch (256-row units) are processed (streamed tables run a launch in task
@@ -242,15 +242,14 @@ __global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8))) k_
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        // half-wave h copies survivor i + h: k parent columns, then the value
-        const uint32_t h = l >> 5, c = l & 31;
-        for (uint32_t i = 0; i < nk; i += 2) {
-          const uint32_t sidx = i + h;
-          const uint64_t o = base + sidx;
-          if (sidx < nk && c <= k && o < it.out_cap) {
-            const uint32_t v = c < k ? it.rows[(rbase + sv_row[w][sidx]) * k + c] : sv_val[w][sidx];
-            it.out[o * (k + 1) + c] = v;
-          }
+        // the nk new rows are nk * (k + 1) consecutive output words, written
+        // by consecutive lanes: word q is column q % (k + 1) of survivor
+        // q / (k + 1) (k parent columns, then the value)
+        const uint32_t rl = k + 1;
+        for (uint32_t q = l; q < nk * rl; q += 64) {
+          const uint32_t i = q / rl, c = q - i * rl;
+          if (base + i < it.out_cap)
+            it.out[(base + i) * rl + c] = c < k ? it.rows[(rbase + sv_row[w][i]) * k + c] : sv_val[w][i];
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -302,6 +301,7 @@ __global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8))) k_
   __shared__ uint32_t olen[VW][GEM_MAX_QV], dpos[VW][GEM_MAX_QV];
   __shared__ uint32_t wend[VW][64];   // window ends in the first other list
   __shared__ uint32_t win[VW][HWIN];  // staged window
+  __shared__ uint32_t prow[VW][GEM_MAX_QV], hsv[VW][128];  // parent row, this step's surviving values
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const uint32_t n = *heavy_n < heavy_cap ? *heavy_n : heavy_cap;
   const uint64_t total = pre[n];
@@ -343,6 +343,7 @@ __global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8))) k_
       }
     }
     if (l < ndup) dpos[w][l] = vposition_in(L.bm_new, L.rank_new, L.dup_can[l][row[L.dup_col[l]]]);
+    if (!FINAL && l < k) prow[w][l] = row[l];
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const uint64_t s0 = (uint64_t)(wt - pre[h]) * chunk;
@@ -408,17 +409,19 @@ __global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8))) k_
           if (l == 0) base = atomicAdd(it.counter, (unsigned long long)nk);
           base = readlane_u64(base, 0);
           written += (uint64_t)nk * 4 * (k + 1);
-          const uint64_t o0 = base + rank_below(m0), o1 = base + nk0 + rank_below(m1);
-          if (keep0 && o0 < it.out_cap) {
-            uint32_t *dst = it.out + o0 * (k + 1);
-            for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
-            dst[k] = val0;
+          // the nk new rows share the parent row: they are nk * (k + 1)
+          // consecutive words of the output, written by consecutive lanes
+          if (keep0) hsv[w][rank_below(m0)] = val0;
+          if (keep1) hsv[w][nk0 + rank_below(m1)] = val1;
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          const uint32_t rl = k + 1;
+          for (uint32_t q = l; q < nk * rl; q += 64) {
+            const uint32_t r = q / rl, c = q - r * rl;
+            if (base + r < it.out_cap) it.out[(base + r) * rl + c] = c < k ? prow[w][c] : hsv[w][r];
           }
-          if (keep1 && o1 < it.out_cap) {
-            uint32_t *dst = it.out + o1 * (k + 1);
-            for (uint32_t c = 0; c < k; ++c) dst[c] = row[c];
-            dst[k] = val1;
-          }
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
       }
       // wend of this wave is rewritten by the next group of steps
