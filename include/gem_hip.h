@@ -120,6 +120,11 @@ int gem_plan_gem_order(const gem_query_view *q, const uint64_t *table_sizes,
                        uint32_t *order_out);
 
 /* -------------------------------------------------------- device context */
+/* A context owns one HIP stream on `device`; all of its device work runs on
+ * that stream.  Device memory the library frees is kept for reuse by the same
+ * context (large buffers are carved from 8 GiB segments) and is returned to
+ * the driver when the context is destroyed, or earlier when the driver refuses
+ * an allocation. */
 typedef struct gem_ctx gem_ctx;
 
 int gem_hip_create(int device, gem_ctx **out);
