@@ -621,6 +621,7 @@ struct ExtSlot {
   size_t hdr_bytes = 0;
   uint32_t max_nb = 1, max_ndup = 0;
   size_t lds = 0;
+  uint32_t vthreads = VB;  // k_vext block size: 64 when four waves' lists do not fit one block's LDS
   uint32_t grid = 0;
   uint32_t heavy_t = 1024, heavy_chunk = 1024;
   hipGraph_t graph = nullptr;
@@ -631,7 +632,11 @@ struct ExtSlot {
     cap = std::max<uint32_t>(capacity, 1);
     max_nb = std::max<uint32_t>(mnb, 1);
     max_ndup = mnd;
-    lds = (size_t)VW * 64 * ((max_nb - 1) * 12 + max_ndup * 4);
+    // per wave: (nb - 1) list addresses and lengths and ndup equal-label
+    // positions for each of its 64 rows (at most 31 KB with 32 query vertices)
+    const size_t lds_wave = (size_t)64 * ((max_nb - 1) * 12 + max_ndup * 4);
+    vthreads = lds_wave * VW <= 60 * 1024 ? VB : 64;
+    lds = lds_wave * (vthreads / 64);
     if (lds > 60 * 1024) throw Error{GEM_E_LIMIT, "a level has too many backward edges for the vjoin LDS layout"};
     grid = g_persistent_blocks;
     heavy_t = c->heavy_t;
@@ -696,14 +701,14 @@ struct ExtSlot {
     unsigned long long *dp = reinterpret_cast<unsigned long long *>(hdr.as<unsigned char>() + HDR_PITEMS);
     if (b0 < e0) {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_WRITE : -1);
-      k_vext<false, OOC><<<grd, VB, lds, s>>>(items[0].as<VItem>(), dn + 0, blk[0].as<uint64_t>(), dq + 0, max_nb,
+      k_vext<false, OOC><<<grd * (VB / vthreads), vthreads, lds, s>>>(items[0].as<VItem>(), dn + 0, blk[0].as<uint64_t>(), dq + 0, max_nb,
                                                max_ndup, count_items ? dp : nullptr, count_items ? dp + 4 : nullptr,
                                                b0, e0, heavy[0].as<HeavyRow>(), hn + 0, HEAVY_CAP, heavy_t);
       GEM_LAUNCH_CHECK("k_vext<write>");
     }
     if (b1 < e1) {
       ProfLaunch pl(c, prof_kernels ? K_VEXT_FINAL : -1);
-      k_vext<true, OOC><<<grd, VB, lds, s>>>(items[1].as<VItem>(), dn + 1, blk[1].as<uint64_t>(), dq + 1, max_nb,
+      k_vext<true, OOC><<<grd * (VB / vthreads), vthreads, lds, s>>>(items[1].as<VItem>(), dn + 1, blk[1].as<uint64_t>(), dq + 1, max_nb,
                                               max_ndup, count_items ? dp + 1 : nullptr, nullptr, b1, e1,
                                               heavy[1].as<HeavyRow>(), hn + 1, HEAVY_CAP, heavy_t);
       GEM_LAUNCH_CHECK("k_vext<final>");

@@ -131,3 +131,35 @@ def test_batch_invariance(gpu_ctx, rmat_graph):
         ref = oracle.count(rmat_graph, q, cap=cap, work_limit=5 * 10 ** 8)
         if ref is not None:
             assert b == (ref["count"], ref["saturated"])
+
+
+def _wide_level_case():
+    """A 32-vertex query whose last vertex has 17 backward edges and 14 earlier
+    same-label vertices it is not adjacent to (four waves' probe lists then
+    exceed one block's LDS, so the walk runs in one-wave blocks), embedded in a
+    data graph made of the query's own copy plus 2,000 noise vertices (whose
+    labels avoid the shared one, so the count stays small)."""
+    rng = np.random.default_rng(31)
+    labels = list(range(17)) + [99] * 15
+    edges = [(i, i + 1) for i in range(16)] + [(i + 1, 17 + i) for i in range(14)] + [(i, 31) for i in range(17)]
+    q = Query(labels, edges)
+    n = 32 + 2000
+    lab = np.concatenate([np.array(labels, np.uint32), rng.integers(0, 17, 2000).astype(np.uint32)])
+    pairs = {(min(a, b), max(a, b)) for a, b in edges}
+    while len(pairs) < len(edges) + 12000:
+        a, b = (int(x) for x in rng.integers(0, n, 2))
+        if a != b:
+            pairs.add((min(a, b), max(a, b)))
+    g = Graph.from_edges(n, sorted(pairs), lab)
+    return g, q, np.arange(32, dtype=np.uint32)
+
+
+def test_wide_level_one_wave_blocks(gpu_ctx):
+    g, q, vo = _wide_level_case()
+    gpu_ctx.load_data(g)
+    ref = oracle.vcount(g, q, vo)
+    print("wide level", ref["count"], list(ref["level_rows"]), flush=True)
+    assert ref["count"] >= 1
+    r = gpu_ctx.match_batch([q], vertex_orders=[vo])[0]
+    assert r["count"] == ref["count"] and not r["saturated"]
+    assert np.array_equal(r["level_rows"][:q.num_v], np.asarray(ref["level_rows"])[:q.num_v])
