@@ -28,8 +28,8 @@ from .workloads import CAP, make_graph, make_queries
 
 SPACE = {
     "persistent_blocks": [2048, 4096],
-    "heavy_t": [128, 256, 1024],
-    "heavy_chunk": [256, 1024],
+    "heavy_t": [256, 1024, 4096],
+    "heavy_chunk": [1024, 4096],
     "group": [0],
     "max_orders": [8, 16],
 }
