@@ -11,15 +11,18 @@
 //   k_vlen    per 256-row block: sum over rows of the shortest list length
 //             (an upper bound on the level's output, used to size and split)
 //   k_vext    waves walk the shortest ("driver") list of each of their 64
-//             rows, 64 items per step; each item is binary-searched in the
-//             row's other lists and compared with the equal-label columns;
-//             survivors are counted (last level) or appended to the output
-//             (one atomic reservation per wave step; row order is free)
+//             rows, 128 items per step (two per lane); each item is
+//             binary-searched in the row's other lists and compared with the
+//             equal-label columns; survivors are counted (last level) or
+//             appended to the output (one atomic reservation per wave step;
+//             row order is free); rows with long driving lists are left to
+//             k_vext_heavy
+//   k_vext_heavy  walks those rows in fixed-size pieces spread over waves
 //   k_vcount1 last level with a single backward list: |list| minus the
 //             equal-label columns found in it (no walk)
 //
 // Work distribution: each launch covers many (query, level, row range) items;
-// blocks pull 256-row tasks from a device counter (persistent grid), so queries
+// waves pull 64-row tasks from a device counter (persistent grid), so queries
 // of very different sizes share one launch.  The host keeps, per query, a
 // depth-first stack of pending levels: a level larger than the per-query budget
 // is processed in block ranges, the deepest pending range first, which bounds
@@ -112,7 +115,8 @@ __device__ __forceinline__ uint64_t list_addr(const VLevel &L, uint32_t j, uint3
 // fixed-size chunks spread over all waves: a hub row no longer holds one wave
 // for the whole launch.
 // Dynamic LDS per wave: other lists (nb-1) x 64 x (u64 address + u32 len),
-// dup positions ndup x 64 u32.
+// dup positions ndup x 64 u32 (blocks of one wave when four waves' share
+// would not fit a block's LDS).
 // Survivors are written as whole rows: their (parent row, new value) pairs are
 // staged in LDS, then the wave writes the step's new rows as one run of
 // consecutive words.
@@ -121,14 +125,12 @@ __device__ __forceinline__ uint64_t list_addr(const VLevel &L, uint32_t j, uint3
 // launch (256-row units) are processed (streamed tables run a launch in task
 // groups).
 template <bool FINAL, bool OOC>
-__global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8))) k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p,
-                                            const uint64_t *__restrict__ blk_prefix, unsigned int *__restrict__ queue,
-                                            uint32_t max_nb, uint32_t max_ndup,
-                                            unsigned long long *__restrict__ prof_items,
-                                            unsigned long long *__restrict__ prof_out, uint64_t g_begin,
-                                            uint64_t g_end, HeavyRow *__restrict__ heavy,
-                                            unsigned int *__restrict__ heavy_n, uint32_t heavy_cap,
-                                            uint32_t heavy_t) {
+__global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8)))
+k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p, const uint64_t *__restrict__ blk_prefix,
+       unsigned int *__restrict__ queue, uint32_t max_nb, uint32_t max_ndup,
+       unsigned long long *__restrict__ prof_items, unsigned long long *__restrict__ prof_out, uint64_t g_begin,
+       uint64_t g_end, HeavyRow *__restrict__ heavy, unsigned int *__restrict__ heavy_n, uint32_t heavy_cap,
+       uint32_t heavy_t) {
   const uint32_t nitems = *nitems_p;
   __shared__ uint32_t st[VW][65];
   __shared__ uint64_t drv[VW][64];  // address of each row's driving list
@@ -291,12 +293,11 @@ constexpr uint32_t HWIN = 512;
 // row's other lists and equal-label columns; survivors are counted or appended
 // to the row's item output like the main pass's.
 template <bool FINAL, bool OOC>
-__global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8))) k_vext_heavy(const VItem *__restrict__ items,
-                                                  const HeavyRow *__restrict__ heavy,
-                                                  const unsigned int *__restrict__ heavy_n, uint32_t heavy_cap,
-                                                  const uint64_t *__restrict__ pre, unsigned int *__restrict__ queue,
-                                                  uint32_t chunk, unsigned long long *__restrict__ prof_items,
-                                                  unsigned long long *__restrict__ prof_out) {
+__global__ void __launch_bounds__(VB) __attribute__((amdgpu_waves_per_eu(8)))
+k_vext_heavy(const VItem *__restrict__ items, const HeavyRow *__restrict__ heavy,
+             const unsigned int *__restrict__ heavy_n, uint32_t heavy_cap, const uint64_t *__restrict__ pre,
+             unsigned int *__restrict__ queue, uint32_t chunk, unsigned long long *__restrict__ prof_items,
+             unsigned long long *__restrict__ prof_out) {
   __shared__ uint64_t oaddr[VW][GEM_MAX_QV];
   __shared__ uint32_t olen[VW][GEM_MAX_QV], dpos[VW][GEM_MAX_QV];
   __shared__ uint32_t wend[VW][64];   // window ends in the first other list
