@@ -183,6 +183,11 @@ k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p, c
           dlen = 0;  // walked by k_vext_heavy
         }
       }
+    }
+    // a row with nothing to walk here (empty driving list, or set aside as a
+    // heavy row) owns no item of the steps below, so its probe lists and
+    // equal-label positions are never read: skip their loads
+    if (valid && dlen) {
       daddr = list_addr<OOC>(L, dj, dp, dbase);
       uint32_t o = 0;
       for (uint32_t j = 0; j < nb; ++j) {
