@@ -116,3 +116,51 @@ def test_library_has_no_cpu_fallback():
     from gem_amd import GemContext
     with pytest.raises(GemError):
         GemContext(0)
+
+
+def test_committed_tuning_is_within_the_api_limits():
+    """The autotuner's committed choice is accepted by the setters it feeds
+    (ranges of gem_hip_set_persistent_blocks / gem_hip_set_heavy_split) and
+    every trial it kept reported the same counts as the first."""
+    import json
+    path = os.path.join(ROOT, "gem_amd", "tuning", "rmat22.json")
+    d = json.load(open(path))
+    best = d["best"]
+    assert 1 <= best["persistent_blocks"] <= 1 << 20
+    assert 32 <= best["heavy_t"] <= 1 << 24 and 64 <= best["heavy_chunk"] <= 1 << 20
+    assert best["level_budget"] == 0 and best["max_orders"] >= 1
+    assert all(t["counts_equal"] for t in d["trials"])
+    assert min(t["seconds"] for t in d["trials"]) == d["best_seconds"]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run(["python3", os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_order_model_evaluation_rule():
+    """The trainer's evaluation: per query, the cheapest predicted candidate
+    replaces GEM's order (candidate 0) only when predicted cheaper by more than
+    the margin; work is 2^label."""
+    from gem_amd.train_order_model import evaluate
+
+    class Fixed:
+        def __init__(self, p):
+            self.p = np.asarray(p, np.float32)
+
+        def predict(self, F):
+            return self.p
+
+    # query 0: candidates (gem, a, b), query 1: (gem, c)
+    Q = np.array([0, 0, 0, 1, 1])
+    C = np.array([0, 1, 2, 0, 1])
+    Y = np.array([10.0, 8.0, 12.0, 6.0, 5.0], np.float32)
+    F = np.zeros((5, 1), np.int32)
+    pred = [10.0, 7.5, 11.0, 6.0, 5.8]  # model output is de-normalised as pred * sd + mu
+    ev = evaluate(Fixed(pred), F, Y, Q, C, 0.0, 1.0, [0, 1], margin=1.0)
+    # query 0: 7.5 < 10 - 1 -> candidate 1 (2^8); query 1: 5.8 is not < 6 - 1 -> GEM's (2^6)
+    assert ev["gem_work"] == 2 ** 10 + 2 ** 6 and ev["learned_work"] == 2 ** 8 + 2 ** 6
+    assert ev["best_work"] == 2 ** 8 + 2 ** 5
+    ev0 = evaluate(Fixed(pred), F, Y, Q, C, 0.0, 1.0, [0, 1], margin=0.0)
+    assert ev0["learned_work"] == 2 ** 8 + 2 ** 5
