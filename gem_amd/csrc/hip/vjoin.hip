@@ -45,10 +45,10 @@ constexpr uint32_t VB = 256;  // rows per task block
 constexpr uint32_t VW = VB / 64;
 
 __device__ __forceinline__ uint32_t vposition_in(const uint64_t *bm, const uint32_t *rank, uint32_t v) {
-  const uint64_t word = bm[v >> 6];
+  const uint64_t word = ldg(bm + (v >> 6));
   const uint32_t b = v & 63;
   if (!((word >> b) & 1ull)) return NOT_FOUND;
-  return rank[v >> 6] + (uint32_t)__popcll(word & ((1ull << b) - 1));
+  return ldg(rank + (v >> 6)) + (uint32_t)__popcll(word & ((1ull << b) - 1));
 }
 
 // task g (global block index of the launch) -> item t, row block within item
@@ -166,9 +166,10 @@ k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p, c
     if (valid) {
       dlen = 0xFFFFFFFFu;
       for (uint32_t j = 0; j < nb; ++j) {
-        const uint32_t p = row[L.key_col[j]];
-        const uint64_t b = L.toff[j][p];
-        const uint32_t len = (uint32_t)(L.toff[j][p + 1] - b);
+        const uint32_t p = ldg(row + L.key_col[j]);
+        const uint64_t *off = L.toff[j];
+        const uint64_t b = ldg(off + p);
+        const uint32_t len = (uint32_t)(ldg(off + p + 1) - b);
         if (len < dlen) {
           dlen = len;
           dj = j;
@@ -192,14 +193,15 @@ k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p, c
       uint32_t o = 0;
       for (uint32_t j = 0; j < nb; ++j) {
         if (j == dj) continue;
-        const uint32_t p = row[L.key_col[j]];
-        const uint64_t b = L.toff[j][p];
+        const uint32_t p = ldg(row + L.key_col[j]);
+        const uint64_t *off = L.toff[j];
+        const uint64_t b = ldg(off + p);
         obase[o * 64 + l] = list_addr<OOC>(L, j, p, b);
-        olen[o * 64 + l] = (uint32_t)(L.toff[j][p + 1] - b);
+        olen[o * 64 + l] = (uint32_t)(ldg(off + p + 1) - b);
         ++o;
       }
       for (uint32_t c = 0; c < ndup; ++c) {
-        const uint32_t vid = L.dup_can[c][row[L.dup_col[c]]];
+        const uint32_t vid = ldg(L.dup_can[c] + ldg(row + L.dup_col[c]));
         dpos[c * 64 + l] = vposition_in(L.bm_new, L.rank_new, vid);
       }
     }
@@ -219,8 +221,8 @@ k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p, c
       bool keep0 = s + l < wtotal, keep1 = s + 64 + l < wtotal;
       if (keep0) rw0 = seg_of(st[w], s + l);
       if (keep1) rw1 = seg_of(st[w], s + 64 + l);
-      if (keep0) val0 = reinterpret_cast<const uint32_t *>(drv[w][rw0])[s + l - st[w][rw0]];
-      if (keep1) val1 = reinterpret_cast<const uint32_t *>(drv[w][rw1])[s + 64 + l - st[w][rw1]];
+      if (keep0) val0 = ldg(reinterpret_cast<const uint32_t *>(drv[w][rw0]) + (s + l - st[w][rw0]));
+      if (keep1) val1 = ldg(reinterpret_cast<const uint32_t *>(drv[w][rw1]) + (s + 64 + l - st[w][rw1]));
       for (uint32_t c = 0; c < ndup; ++c) {
         if (dpos[c * 64 + rw0] == val0) keep0 = false;
         if (dpos[c * 64 + rw1] == val1) keep1 = false;
@@ -256,7 +258,7 @@ k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p, c
         for (uint32_t q = l; q < nk * rl; q += 64) {
           const uint32_t i = q / rl, c = q - i * rl;
           if (base + i < it.out_cap)
-            it.out[(base + i) * rl + c] = c < k ? it.rows[(rbase + sv_row[w][i]) * k + c] : sv_val[w][i];
+            stg(it.out + (base + i) * rl + c, c < k ? ldg(it.rows + (rbase + sv_row[w][i]) * k + c) : sv_val[w][i]);
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -371,9 +373,9 @@ k_vext_heavy(const VItem *__restrict__ items, const HeavyRow *__restrict__ heavy
         uint32_t pos = 0;
         if (l < nsub) {
           const uint64_t last = (S + 128ull * (l + 1) < S1 ? S + 128ull * (l + 1) : S1) - 1;
-          pos = upper_bound_u32(l0p, l0n, dlist[last]);
+          pos = upper_bound_u32(l0p, l0n, ldg(dlist + last));
         } else if (l == 63) {
-          pos = lower_bound_u32(l0p, l0n, dlist[S]);
+          pos = lower_bound_u32(l0p, l0n, ldg(dlist + S));
         }
         wend[w][l] = pos;
         __builtin_amdgcn_wave_barrier();
@@ -383,8 +385,8 @@ k_vext_heavy(const VItem *__restrict__ items, const HeavyRow *__restrict__ heavy
         const uint64_t s = S + 128ull * j;
         bool keep0 = s + l < S1, keep1 = s + 64 + l < S1;
         uint32_t val0 = 0, val1 = 0;
-        if (keep0) val0 = dlist[s + l];
-        if (keep1) val1 = dlist[s + 64 + l];
+        if (keep0) val0 = ldg(dlist + s + l);
+        if (keep1) val1 = ldg(dlist + s + 64 + l);
         for (uint32_t c = 0; c < ndup; ++c) {
           if (dpos[w][c] == val0) keep0 = false;
           if (dpos[w][c] == val1) keep1 = false;
@@ -392,10 +394,10 @@ k_vext_heavy(const VItem *__restrict__ items, const HeavyRow *__restrict__ heavy
         if (nb > 1) {
           const uint32_t a = j ? wend[w][j - 1] : wend[w][63], wn = wend[w][j] - a;
           if (wn <= HWIN) {
-            for (uint32_t i = l; i < wn; i += 64) win[w][i] = l0p[a + i];
+            for (uint32_t i = l; i < wn; i += 64) win[w][i] = ldg(l0p + a + i);
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            contains_pair(win[w], wn, val0, keep0, win[w], wn, val1, keep1);
+            contains_pair<false>(win[w], wn, val0, keep0, win[w], wn, val1, keep1);
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
           } else {
@@ -424,7 +426,7 @@ k_vext_heavy(const VItem *__restrict__ items, const HeavyRow *__restrict__ heavy
           const uint32_t rl = k + 1;
           for (uint32_t q = l; q < nk * rl; q += 64) {
             const uint32_t r = q / rl, c = q - r * rl;
-            if (base + r < it.out_cap) it.out[(base + r) * rl + c] = c < k ? prow[w][c] : hsv[w][r];
+            if (base + r < it.out_cap) stg(it.out + (base + r) * rl + c, c < k ? prow[w][c] : hsv[w][r]);
           }
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");

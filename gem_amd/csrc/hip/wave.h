@@ -9,6 +9,20 @@ namespace gemd {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Load / store through the global address space.  Pointers read from memory
+// (list addresses, row pointers) are generic to the compiler, which then
+// emits flat instructions: those also count against the LDS wait counter, so
+// every wait on them waits for LDS traffic too.  All such pointers here are
+// device memory or mapped host memory, which global instructions reach.
+template <class T>
+__device__ __forceinline__ T ldg(const T *p) {
+  return *(const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ void stg(T *p, T v) {
+  *(__attribute__((address_space(1))) T *)p = v;
+}
+
 // popcount of the bits of `m` below this lane (v_mbcnt_lo/hi)
 __device__ __forceinline__ uint32_t rank_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -95,7 +109,7 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t 
   uint32_t lo = 0;
   while (n) {
     const uint32_t h = n >> 1;
-    if (a[lo + h] < v) {
+    if (ldg(a + lo + h) < v) {
       lo += h + 1;
       n -= h + 1;
     } else {
@@ -108,7 +122,7 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t 
   uint32_t lo = 0;
   while (n) {
     const uint32_t h = n >> 1;
-    if (a[lo + h] <= v) {
+    if (ldg(a + lo + h) <= v) {
       lo += h + 1;
       n -= h + 1;
     } else {
@@ -124,6 +138,8 @@ __device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t 
 // so their loads are in flight together, and a hit is seen when it is probed
 // (a lower bound i with a[i] == v is always probed: the range's right end
 // only moves onto i by probing it), so no load follows the search.
+// GLOBAL: the lists are in device or mapped host memory (else in LDS).
+template <bool GLOBAL = true>
 __device__ __forceinline__ void contains_pair(const uint32_t *a0, uint32_t n0, uint32_t v0, bool &k0,
                                               const uint32_t *a1, uint32_t n1, uint32_t v1, bool &k1) {
   uint32_t lo0 = 0, lo1 = 0;
@@ -133,8 +149,8 @@ __device__ __forceinline__ void contains_pair(const uint32_t *a0, uint32_t n0, u
   while (n0 | n1) {
     const uint32_t h0 = n0 >> 1, h1 = n1 >> 1;
     uint32_t x0 = 0, x1 = 0;
-    if (n0) x0 = a0[lo0 + h0];
-    if (n1) x1 = a1[lo1 + h1];
+    if (n0) x0 = GLOBAL ? ldg(a0 + lo0 + h0) : a0[lo0 + h0];
+    if (n1) x1 = GLOBAL ? ldg(a1 + lo1 + h1) : a1[lo1 + h1];
     if (n0) {
       if (x0 < v0) {
         lo0 += h0 + 1;
