@@ -51,6 +51,9 @@ __device__ __forceinline__ uint32_t vposition_in(const uint64_t *bm, const uint3
   return ldg(rank + (v >> 6)) + (uint32_t)__popcll(word & ((1ull << b) - 1));
 }
 
+// A level descriptor read through the global address space (see ldg).
+using GLevel = const __attribute__((address_space(1))) VLevel;
+
 // task g (global block index of the launch) -> item t, row block within item
 __device__ __forceinline__ uint32_t find_item(const uint64_t *blk_prefix, uint32_t nitems, uint64_t g) {
   uint32_t lo = 0, hi = nitems;  // largest t with blk_prefix[t] <= g
@@ -79,15 +82,16 @@ __global__ void __launch_bounds__(VB) k_vlen(const VItem *__restrict__ items, ui
     if (g >= total) break;
     const uint32_t t = find_item(blk_prefix, nitems, g);
     const VItem it = items[t];
-    const VLevel &L = *it.lv;
+    GLevel &L = *(GLevel *)it.lv;
     const uint64_t r = it.row_begin + (g - blk_prefix[t]) * VB + threadIdx.x;
     uint64_t u = 0;
     if (r < it.row_end) {
       const uint32_t *row = it.rows + r * it.k;
       u = ~0ull;
       for (uint32_t j = 0; j < L.nb; ++j) {
-        const uint32_t p = row[L.key_col[j]];
-        const uint64_t len = L.toff[j][p + 1] - L.toff[j][p];
+        const uint32_t p = ldg(row + L.key_col[j]);
+        const uint64_t *off = L.toff[j];
+        const uint64_t len = ldg(off + p + 1) - ldg(off + p);
         u = len < u ? len : u;
       }
     }
@@ -102,8 +106,8 @@ __global__ void __launch_bounds__(VB) k_vlen(const VItem *__restrict__ items, ui
 // Device address of the list of key p in backward table j (b = its first
 // offset): in HBM, or through the streamed tables' block map.
 template <bool OOC>
-__device__ __forceinline__ uint64_t list_addr(const VLevel &L, uint32_t j, uint32_t p, uint64_t b) {
-  if (OOC) return L.vmap[j][p >> L.kshift[j]] + 4 * b;
+__device__ __forceinline__ uint64_t list_addr(GLevel &L, uint32_t j, uint32_t p, uint64_t b) {
+  if (OOC) return ldg(L.vmap[j] + (p >> L.kshift[j])) + 4 * b;
   return (uint64_t)(uintptr_t)(L.tval[j] + b);
 }
 
@@ -154,7 +158,7 @@ k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p, c
     const uint64_t g = q / VW;
     const uint32_t t = find_item(blk_prefix, nitems, g);
     const VItem it = items[t];
-    const VLevel &L = *it.lv;
+    GLevel &L = *(GLevel *)it.lv;
     const uint32_t nb = L.nb, ndup = L.ndup, k = it.k;
     const uint64_t rbase = it.row_begin + (g - blk_prefix[t]) * VB + (q % VW) * 64;
     const uint64_t r = rbase + l;
@@ -322,16 +326,17 @@ k_vext_heavy(const VItem *__restrict__ items, const HeavyRow *__restrict__ heavy
     const uint32_t h = find_item(pre, n, wt);
     const HeavyRow e = heavy[h];
     const VItem it = items[e.t];
-    const VLevel &L = *it.lv;
+    GLevel &L = *(GLevel *)it.lv;
     const uint32_t nb = L.nb, ndup = L.ndup, k = it.k;
     const uint32_t *row = it.rows + e.r * k;
     // the row's lists (every lane computes the same; lane j keeps list j)
     uint32_t dlen = 0xFFFFFFFFu, dj = 0, dp = 0;
     uint64_t dbase = 0;
     for (uint32_t j = 0; j < nb; ++j) {
-      const uint32_t p = row[L.key_col[j]];
-      const uint64_t b = L.toff[j][p];
-      const uint32_t len = (uint32_t)(L.toff[j][p + 1] - b);
+      const uint32_t p = ldg(row + L.key_col[j]);
+      const uint64_t *off = L.toff[j];
+      const uint64_t b = ldg(off + p);
+      const uint32_t len = (uint32_t)(ldg(off + p + 1) - b);
       if (len < dlen) {
         dlen = len;
         dj = j;
@@ -342,16 +347,16 @@ k_vext_heavy(const VItem *__restrict__ items, const HeavyRow *__restrict__ heavy
     const uint32_t *dlist = reinterpret_cast<const uint32_t *>(list_addr<OOC>(L, dj, dp, dbase));
     if (l < nb) {
       const uint32_t j = l;
-      const uint32_t p = row[L.key_col[j]];
-      const uint64_t b = L.toff[j][p];
+      const uint32_t p = ldg(row + L.key_col[j]);
+      const uint64_t b = ldg(L.toff[j] + p);
       const uint32_t o = j < dj ? j : j - 1;  // other lists, in order, without the driver
       if (j != dj) {
         oaddr[w][o] = list_addr<OOC>(L, j, p, b);
-        olen[w][o] = (uint32_t)(L.toff[j][p + 1] - b);
+        olen[w][o] = (uint32_t)(ldg(L.toff[j] + p + 1) - b);
       }
     }
-    if (l < ndup) dpos[w][l] = vposition_in(L.bm_new, L.rank_new, L.dup_can[l][row[L.dup_col[l]]]);
-    if (!FINAL && l < k) prow[w][l] = row[l];
+    if (l < ndup) dpos[w][l] = vposition_in(L.bm_new, L.rank_new, ldg(L.dup_can[l] + ldg(row + L.dup_col[l])));
+    if (!FINAL && l < k) prow[w][l] = ldg(row + l);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const uint64_t s0 = (uint64_t)(wt - pre[h]) * chunk;
@@ -461,18 +466,18 @@ __global__ void __launch_bounds__(VB) k_vcount1(const VItem *__restrict__ items,
     if (g >= total) break;
     const uint32_t t = find_item(blk_prefix, nitems, g);
     const VItem it = items[t];
-    const VLevel &L = *it.lv;
+    GLevel &L = *(GLevel *)it.lv;
     const uint64_t r = it.row_begin + (g - blk_prefix[t]) * VB + threadIdx.x;
     uint64_t cnt = 0;
     if (r < it.row_end) {
       const uint32_t *row = it.rows + r * it.k;
-      const uint32_t p = row[L.key_col[0]];
-      const uint64_t b = L.toff[0][p], e = L.toff[0][p + 1];
+      const uint32_t p = ldg(row + L.key_col[0]);
+      const uint64_t b = ldg(L.toff[0] + p), e = ldg(L.toff[0] + p + 1);
       cnt = e - b;
       const uint32_t *lst = nullptr;
       if (L.ndup) lst = reinterpret_cast<const uint32_t *>(list_addr<OOC>(L, 0, p, b));
       for (uint32_t c = 0; c < L.ndup; ++c) {
-        const uint32_t pos = vposition_in(L.bm_new, L.rank_new, L.dup_can[c][row[L.dup_col[c]]]);
+        const uint32_t pos = vposition_in(L.bm_new, L.rank_new, ldg(L.dup_can[c] + ldg(row + L.dup_col[c])));
         if (pos != NOT_FOUND && sorted_contains(lst, e - b, pos)) --cnt;
       }
     }
@@ -502,14 +507,14 @@ __global__ void __launch_bounds__(VB) k_vmark(const VItem *__restrict__ items, u
   for (uint64_t g = g0 + blockIdx.x; g < g1; g += gridDim.x) {
     const uint32_t t = find_item(blk_prefix, nitems, g);
     const VItem it = items[t];
-    const VLevel &L = *it.lv;
+    GLevel &L = *(GLevel *)it.lv;
     const uint64_t r = it.row_begin + (g - blk_prefix[t]) * VB + threadIdx.x;
     // k_vcount1 reads list values only to remove equal-label columns
     if (r >= it.row_end || (count_only && L.ndup == 0)) continue;
     const uint32_t *row = it.rows + r * it.k;
     const uint32_t gt = (uint32_t)(base + g - wa);
     for (uint32_t j = 0; j < L.nb; ++j) {
-      const uint32_t b = L.gblk[j] + (row[L.key_col[j]] >> L.kshift[j]);
+      const uint32_t b = L.gblk[j] + (ldg(row + L.key_col[j]) >> L.kshift[j]);
       if (first[b] > gt) atomicMin(&first[b], gt);
       if (last[b] < gt) atomicMax(&last[b], gt);
     }

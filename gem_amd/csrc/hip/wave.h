@@ -89,19 +89,20 @@ __device__ __forceinline__ uint32_t seg_of(const uint32_t *starts, uint32_t item
   return lo;
 }
 
-// Binary search for v in sorted a[0..n); true if present.
+// Binary search for v in sorted a[0..n) (device or mapped host memory); true
+// if present.
 template <class T>
 __device__ __forceinline__ bool sorted_contains(const T *a, uint64_t n, T v) {
   uint64_t lo = 0, hi = n;
   while (lo < hi) {
     uint64_t mid = (lo + hi) >> 1;
-    T x = a[mid];
+    T x = ldg(a + mid);
     if (x < v)
       lo = mid + 1;
     else
       hi = mid;
   }
-  return lo < n && a[lo] == v;
+  return lo < n && ldg(a + lo) == v;
 }
 
 // First index of sorted a[0..n) whose value is >= v (lower) or > v (upper).
