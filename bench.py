@@ -60,10 +60,10 @@ def cpu_model():
     return platform.processor()
 
 
-def profiled_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` from the committed
-    rocprofv3 counter summaries (FETCH_SIZE + WRITE_SIZE, calibrated against a
-    copy of known size), or None when no summary covers it."""
+def profiled(key, kernel, workload):
+    """A per-launch figure of `kernel` on `workload` recorded in the committed
+    rocprofv3 summaries under profiles/ as a line `<key> <kernel> <workload> N`,
+    or None when no summary covers it."""
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
         return None
@@ -72,9 +72,16 @@ def profiled_traffic(kernel, workload):
             continue
         for line in open(os.path.join(pdir, fn)):
             f = line.split()
-            if len(f) == 4 and f[0] == "traffic_per_launch" and f[1] == kernel and f[2] == workload:
-                return int(f[3])
+            if len(f) == 4 and f[0] == key and f[1] == kernel and f[2] == workload:
+                return float(f[3])
     return None
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` (rocprofv3 FETCH_SIZE +
+    WRITE_SIZE, calibrated against a copy of known size), or None."""
+    v = profiled("traffic_per_launch", kernel, workload)
+    return int(v) if v is not None else None
 
 
 def kernel_table(stats):
@@ -350,7 +357,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": round(dom["bytes"] / max(dom["launches"], 1)),
-                     "launches": dom["launches"], "avg_launch_ms": round(dom["ms"] / max(dom["launches"], 1), 3)},
+                     "launches": dom["launches"], "avg_launch_ms": round(dom["ms"] / max(dom["launches"], 1), 3),
+                     "profiled_avg_launch_ms": (round(prof_us / 1000, 3) if (prof_us := profiled(
+                         "avg_launch_us", dom_name, args.workload)) is not None else None)},
         "cpu_baseline": cpu,
         "streamed_tables": stream,
     }

@@ -164,3 +164,17 @@ def test_order_model_evaluation_rule():
     assert ev["best_work"] == 2 ** 8 + 2 ** 5
     ev0 = evaluate(Fixed(pred), F, Y, Q, C, 0.0, 1.0, [0, 1], margin=0.0)
     assert ev0["learned_work"] == 2 ** 8 + 2 ** 5
+
+
+def test_profiles_cover_the_walk_kernels():
+    """bench.py's roofline line reads the per-launch HBM traffic and rocprofv3
+    average of its dominant kernel from profiles/: both are recorded for every
+    walk kernel of the rmat22 step."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for k in ("k_vext<write>", "k_vext<final>", "k_vext_heavy<write>", "k_vext_heavy<final>"):
+        assert bench.profiled_traffic(k, "rmat22") > 0
+        assert bench.profiled("avg_launch_us", k, "rmat22") > 0
+    assert bench.profiled_traffic("k_vext<write>", "no-such-workload") is None
