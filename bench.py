@@ -93,7 +93,8 @@ def kernel_table(stats):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2,
+                    help="timed passes over the whole query set (the value is their mean rate)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rmat22", choices=sorted(DESCR))
     ap.add_argument("--queries", type=int, default=0, help="use only the first N queries")
