@@ -106,8 +106,8 @@ def main():
                     help="queries resident at once in the batched path (0 = all; default: autotuned, else 0)")
     ap.add_argument("--no-tuning", action="store_true", help="ignore gem_amd/tuning/<workload>.json")
     ap.add_argument("--level-budget", default=None,
-                    help="bytes of pending level rows shared by the running queries (e.g. 64G; default: "
-                         "autotuned, else the context's)")
+                    help="bytes of pending level rows shared by the running queries (e.g. 64G; default: the "
+                         "tuning's; 0 = a quarter of the device memory free when the batch starts)")
     ap.add_argument("--heavy-split", default=None,
                     help="THRESHOLD,CHUNK: rows with longer driving lists are walked in CHUNK-entry pieces "
                          "(default: autotuned, else 1024,1024)")
