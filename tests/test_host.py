@@ -178,3 +178,20 @@ def test_profiles_cover_the_walk_kernels():
         assert bench.profiled_traffic(k, "rmat22") > 0
         assert bench.profiled("avg_launch_us", k, "rmat22") > 0
     assert bench.profiled_traffic("k_vext<write>", "no-such-workload") is None
+
+
+def test_benchmark_workload_is_the_described_batch():
+    """The rmat22 workload is the 512-query batch the metric is quoted on:
+    2^22 vertices, queries of 4-32 vertices, generated deterministically, and
+    disjoint from the order model's training queries (rmat22_learn: same
+    graph, other query seed)."""
+    from gem_amd.workloads import make_graph, make_queries
+    g = make_graph("rmat22")
+    assert g.num_vertices == 1 << 22
+    qs = make_queries("rmat22", g)
+    assert len(qs) == 512 and min(q.num_v for q in qs) == 4 and max(q.num_v for q in qs) == 32
+    again = make_queries("rmat22", g)
+    assert all(np.array_equal(a.edges, b.edges) and np.array_equal(a.labels, b.labels) for a, b in zip(qs, again))
+    learn = make_queries("rmat22_learn", g)
+    key = {(tuple(q.labels.tolist()), tuple(map(tuple, q.edges.tolist()))) for q in qs}
+    assert not any((tuple(q.labels.tolist()), tuple(map(tuple, q.edges.tolist()))) in key for q in learn)
