@@ -55,6 +55,12 @@ class StreamStats(C.Structure):
                 ("copy_ms", C.c_double), ("step_ms", C.c_double)]
 
 
+class MemoryStats(C.Structure):
+    _fields_ = [("in_use", C.c_uint64), ("peak_in_use", C.c_uint64), ("held", C.c_uint64), ("reusable", C.c_uint64),
+                ("level_budget", C.c_uint64), ("peak_charged", C.c_uint64), ("iterations", C.c_uint64),
+                ("forced", C.c_uint64)]
+
+
 class PhaseTimes(C.Structure):
     _fields_ = [("filter_ms", C.c_float), ("index_ms", C.c_float), ("tables_ms", C.c_float),
                 ("join_ms", C.c_float)]
@@ -96,6 +102,7 @@ SIGNATURES = [
     ("gem_hip_join", C.c_int, [C.c_void_p, u32p, C.c_uint64, C.POINTER(JoinResult), C.POINTER(StepLog),
                                u32p, C.c_uint64]),
     ("gem_hip_phase_times", C.c_int, [C.c_void_p, C.POINTER(PhaseTimes)]),
+    ("gem_hip_memory_stats", C.c_int, [C.c_void_p, C.c_int, C.POINTER(MemoryStats)]),
     ("gem_hip_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_kernel_stats", C.c_int, [C.c_void_p, C.POINTER(KernelStat), C.c_uint32, u32p]),
     ("gem_hip_match", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint64, C.POINTER(JoinResult), u32p]),

@@ -246,6 +246,21 @@ int gem_hip_join(gem_ctx *c, const uint32_t *edge_order, uint64_t cap, gem_join_
   });
 }
 
+int gem_hip_memory_stats(gem_ctx *c, int reset_peak, gem_memory_stats *out) {
+  return guard(c, [&]() {
+    if (!out) throw Error{GEM_E_INVALID, "null argument"};
+    const gemh::PoolStats p = pool_stats(reset_peak != 0);
+    out->in_use = p.in_use;
+    out->peak_in_use = p.peak_in_use;
+    out->held = p.held;
+    out->reusable = p.empty_segments + p.cached_small;
+    out->level_budget = c->last_budget;
+    out->peak_charged = c->last_sched.peak_charged;
+    out->iterations = c->last_sched.iterations;
+    out->forced = c->last_sched.forced;
+  });
+}
+
 int gem_hip_set_profiling(gem_ctx *c, int on) {
   return guard(c, [&]() {
     GEM_HIP_CHECK(hipStreamSynchronize(c->stream));

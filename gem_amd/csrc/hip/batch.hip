@@ -57,10 +57,11 @@ static void plan_query(gem_ctx *c, VQueryState &S, const QueryDev &Qd, const uin
   build_vlevels(c, S);
 }
 
-static void fill_result(const VQueryState &S, uint64_t cap, gem_join_result *r, uint64_t *level_rows) {
+static void fill_result(const VQueryState &V, uint64_t cap, gem_join_result *r, uint64_t *level_rows) {
+  const gemh::SQuery<DevBuf> &S = V.s;
   r->count = S.work_limited ? 0 : (S.saturated ? cap : S.count);
   r->saturated = S.saturated ? 1 : 0;
-  r->num_steps = S.qd->Q.nv - 1;
+  r->num_steps = V.qd->Q.nv - 1;
   r->num_embeddings_written = 0;
   r->rows_processed = S.processed;
   r->work_limited = S.work_limited ? 1 : 0;
@@ -103,6 +104,9 @@ void run_match_batch(gem_ctx *c, const gem_query_view *qs, uint32_t nq, const ui
   GEM_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
   c->times.join_ms = ms;
   prof_flush(c);
+  // the run's level buffers and tables are gone: empty segments go back to
+  // the driver, so that repeated batches start from the same pool state
+  pool_trim();
 }
 
 void run_order_candidates(gem_ctx *c, uint32_t max_orders, uint64_t seed, uint32_t *orders, int32_t *feats,
@@ -210,6 +214,9 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
   GEM_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
   c->times.join_ms = ms;
   prof_flush(c);
+  // the run's level buffers and tables are gone: empty segments go back to
+  // the driver, so that repeated batches start from the same pool state
+  pool_trim();
 }
 
 void run_vjoin(gem_ctx *c, const uint32_t *vorder, uint64_t cap, gem_join_result *out, uint64_t *level_rows) {

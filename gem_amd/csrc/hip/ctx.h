@@ -11,6 +11,8 @@
 #include <vector>
 
 #include "../host/planner.h"
+#include "../host/pool_core.h"
+#include "../host/vsched.h"
 #include "gem_hip.h"
 
 namespace gemd {
@@ -40,18 +42,25 @@ struct Error {
       throw ::gemd::Error{GEM_E_HIP, std::string("launch ") + (name) + ": " + hipGetErrorString(e_)}; \
   } while (0)
 
-// Device memory of DevBuf comes from a caching allocator over hipMalloc:
-// freed blocks are kept in size classes (four per power of two, 256 B at
-// least) and handed out again to allocations on the stream that freed them.
-// All device work of a context runs on its one stream, so a block the host
-// frees after enqueueing its last use can serve later work on that stream
-// without a wait.  A failed hipMalloc releases every cached block and retries.
+// Device memory of DevBuf comes from a caching allocator over hipMalloc
+// (host/pool_core.h): small requests from size classes, large ones carved
+// from 8 GiB segments; freed memory is handed out again to allocations on the
+// stream that freed it.  All device work of a context runs on its one stream,
+// so a block the host frees after enqueueing its last use can serve later
+// work on that stream without a wait.  A failed hipMalloc releases every
+// cached block and empty segment and retries.
 void *pool_alloc(size_t n, hipStream_t s, size_t *cls);
 void pool_free(void *p, size_t cls, hipStream_t s);
 // hipFree every cached block of stream s (its work must have finished).
 void pool_release(hipStream_t s);
-// Bytes held by the allocator for reuse (not in use by any buffer).
-size_t pool_cached_bytes();
+// Bytes the allocator holds that no buffer uses and that a large request can
+// use whole: empty segments and parked small blocks.
+size_t pool_reusable_bytes();
+// Return empty segments and parked small blocks to the driver (waits for the
+// device); the bytes released.
+uint64_t pool_trim();
+// In-use / peak / held bytes of the allocator (optionally restarting the peak).
+gemh::PoolStats pool_stats(bool reset_peak);
 
 struct DevBuf {
   void *p = nullptr;
@@ -262,6 +271,9 @@ struct gem_ctx {
   void *pinned = nullptr;                   // small pinned staging area
   size_t pinned_bytes = 0;
   gemd::Prof prof;
+  // the last batched run's level budget and scheduler statistics
+  uint64_t last_budget = 0;
+  gemh::SchedStats last_sched{};
   // order models created on this context: destroying the context releases
   // their device memory and detaches them (a later gem_mlp_destroy only frees
   // the host object, every other gem_mlp_* call returns GEM_E_STATE)

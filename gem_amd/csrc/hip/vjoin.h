@@ -4,6 +4,7 @@
 #include <memory>
 #include <vector>
 
+#include "../host/vsched.h"
 #include "ctx.h"
 
 namespace gemd {
@@ -44,29 +45,14 @@ struct HeavyRow {
   uint64_t r;
 };
 
-struct VStackItem {
-  uint32_t level;                  // the level these rows extend to next
-  std::shared_ptr<DevBuf> rows;
-  uint64_t nrows;
-  uint32_t k;                      // columns per row
-  std::vector<uint64_t> block_u;   // prefix of per-block output bounds
-  uint64_t cursor;                 // next block to process
-  bool measured;
-};
-
 struct VQueryState {
   const QueryDev *qd = nullptr;
   gemh::VertexPlan plan;
   DevBuf levels;  // VLevel[nv]
   std::vector<VLevel> hlv;  // host copy of `levels`
-  std::vector<uint32_t> max_nb, max_ndup;
-  std::vector<VStackItem> stack;
-  std::vector<uint64_t> level_rows;  // rows after each level
-  uint64_t count = 0, processed = 0;
-  // rows read plus (upper bound of) list entries walked: what the work limit
-  // bounds, since one row with hub lists can cost 10^5 entries
-  uint64_t work = 0;
-  bool saturated = false, done = false, work_limited = false;
+  std::vector<uint32_t> max_nb, max_ndup;  // per level
+  // depth-first stack of pending level ranges, counts and limits (host/vsched.h)
+  gemh::SQuery<DevBuf> s;
 };
 
 void build_vlevels(gem_ctx *c, VQueryState &S);
@@ -78,13 +64,13 @@ class VBatchRunner {
   VBatchRunner(gem_ctx *c, uint64_t cap, uint64_t work_limit, uint64_t budget_bytes, bool streamed = false)
       : c_(c), cap_(cap), work_limit_(work_limit), budget_bytes_(budget_bytes), streamed_(streamed) {}
   void run(std::vector<VQueryState *> &qs);
-  uint64_t iterations() const { return iterations_; }
+  const gemh::SchedStats &stats() const { return stats_; }
 
  private:
   gem_ctx *c_;
   uint64_t cap_, work_limit_, budget_bytes_;
   bool streamed_;
-  uint64_t iterations_ = 0;
+  gemh::SchedStats stats_;
 };
 
 }  // namespace gemd

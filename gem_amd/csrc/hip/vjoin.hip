@@ -8,7 +8,7 @@
 // closing edge prunes rows before they are ever written instead of after a
 // whole level has been materialised.
 //
-//   k_vlen    per 256-row block: sum over rows of the shortest list length
+//   k_vlen    per 64-row unit: sum over rows of the shortest list length
 //             (an upper bound on the level's output, used to size and split)
 //   k_vext    waves walk the shortest ("driver") list of each of their 64
 //             rows, 128 items per step (two per lane); each item is
@@ -23,10 +23,10 @@
 //
 // Work distribution: each launch covers many (query, level, row range) items;
 // waves pull 64-row tasks from a device counter (persistent grid), so queries
-// of very different sizes share one launch.  The host keeps, per query, a
-// depth-first stack of pending levels: a level larger than the per-query budget
-// is processed in block ranges, the deepest pending range first, which bounds
-// memory; a query stops as soon as its count exceeds the cap.
+// of very different sizes share one launch.  The host scheduler
+// (host/vsched.h) keeps, per query, a depth-first stack of pending levels and
+// cuts them into 64-row-unit ranges whose output bounds fit a memory budget
+// shared by all queries; a query stops as soon as its count exceeds the cap.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -68,10 +68,11 @@ __device__ __forceinline__ uint32_t find_item(const uint64_t *blk_prefix, uint32
 }
 
 // ------------------------------------------------------------ length pass
+// Per 64-row unit (wave w of 256-row block g -> unit_u[4g + w]): the sum over
+// its rows of the shortest backward list, an upper bound on its output rows.
 __global__ void __launch_bounds__(VB) k_vlen(const VItem *__restrict__ items, uint32_t nitems,
                                             const uint64_t *__restrict__ blk_prefix,
-                                            unsigned int *__restrict__ queue, uint64_t *__restrict__ block_u) {
-  __shared__ uint64_t sm[VW];
+                                            unsigned int *__restrict__ queue, uint64_t *__restrict__ unit_u) {
   __shared__ uint32_t task;
   const uint64_t total = blk_prefix[nitems];
   for (;;) {
@@ -96,10 +97,7 @@ __global__ void __launch_bounds__(VB) k_vlen(const VItem *__restrict__ items, ui
       }
     }
     u = wave_sum_u64(u);
-    if (threadIdx.x % 64 == 0) sm[threadIdx.x / 64] = u;
-    __syncthreads();
-    if (threadIdx.x == 0) block_u[g] = sm[0] + sm[1] + sm[2] + sm[3];
-    __syncthreads();
+    if (threadIdx.x % 64 == 0) unit_u[g * VW + threadIdx.x / 64] = u;
   }
 }
 
@@ -843,18 +841,181 @@ struct ExtSlot {
 
 }  // namespace
 
+namespace {
+
+// Device side of the scheduler (host/vsched.h): level buffers from the
+// caching allocator, length passes, extend steps through the fixed launch
+// buffers of the run (eager, captured as a hipGraph, or with streamed tables).
+struct HipSchedBackend {
+  using Buf = DevBuf;
+  using Item = gemh::SItem<DevBuf>;
+  gem_ctx *c;
+  std::vector<VQueryState *> &qs;
+  ExtSlot &slot;
+  TableCache *tc;
+  bool verbose;
+  using clk = std::chrono::steady_clock;
+  clk::time_point last_print = clk::now();
+  double t_len = 0, t_ext = 0;
+  uint64_t ext_tasks = 0;
+
+  static uint64_t buf_bytes(const DevBuf &b) { return b.bytes; }
+
+  std::shared_ptr<DevBuf> alloc(uint64_t bytes) {
+    auto b = std::make_shared<DevBuf>();
+    b->alloc(bytes, c->stream);
+    return b;
+  }
+
+  std::shared_ptr<DevBuf> shrink(const std::shared_ptr<DevBuf> &from, uint64_t bytes) {
+    auto b = std::make_shared<DevBuf>();
+    b->alloc(bytes, c->stream);
+    GEM_HIP_CHECK(hipMemcpyAsync(b->p, from->p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return b;
+  }
+
+  VItem vitem(const Item &x) const {
+    VItem it{};
+    it.lv = qs[x.q]->levels.as<VLevel>() + x.level;
+    it.rows = x.rows->as<uint32_t>();
+    it.row_begin = x.row_begin;
+    it.row_end = x.row_end;
+    it.k = x.k;
+    return it;
+  }
+
+  void length_pass(const std::vector<Item> &items, std::vector<std::vector<uint64_t>> *prefix) {
+    const auto t0 = clk::now();
+    hipStream_t s = c->stream;
+    Launch L;
+    for (const Item &x : items) L.add(vitem(x), qs[x.q]->max_nb[x.level], qs[x.q]->max_ndup[x.level]);
+    DevLaunch D;
+    upload(c, L, D);
+    const uint64_t nblk = L.blk.back(), nunit = nblk * VW;
+    DevBuf bu, bp;
+    bu.alloc(nunit * 8, s);
+    bp.alloc((nunit + 1) * 8, s);
+    {
+      ProfLaunch pl(c, K_VLEN);
+      k_vlen<<<grid_for(nblk), VB, 0, s>>>(D.items.as<VItem>(), (uint32_t)L.items.size(), D.blk.as<uint64_t>(),
+                                            D.queue.as<unsigned int>(), bu.as<uint64_t>());
+      GEM_LAUNCH_CHECK("k_vlen");
+    }
+    scan_u64(c, bu.as<uint64_t>(), bp.as<uint64_t>(), nunit);
+    uint64_t *pref = pinned<uint64_t>(c, nunit + 1);
+    GEM_HIP_CHECK(hipMemcpyAsync(pref, bp.p, (nunit + 1) * 8, hipMemcpyDeviceToHost, s));
+    GEM_HIP_CHECK(hipStreamSynchronize(s));
+    prefix->resize(items.size());
+    for (size_t t = 0; t < items.size(); ++t) {
+      const Item &x = items[t];
+      const uint64_t u0 = L.blk[t] * VW;
+      const uint64_t nu = (x.row_end - x.row_begin + gemh::SCHED_UNIT - 1) / gemh::SCHED_UNIT;
+      std::vector<uint64_t> &P = (*prefix)[t];
+      P.resize(nu + 1);
+      for (uint64_t i = 0; i <= nu; ++i) P[i] = pref[u0 + i] - pref[u0];
+      // rows + one offset pair per backward list
+      prof_bytes(c, K_VLEN, (double)(x.row_end - x.row_begin) * (4.0 * x.k + 16.0 * qs[x.q]->max_nb[x.level]));
+    }
+    t_len += std::chrono::duration<double>(clk::now() - t0).count();
+  }
+
+  void extend(const std::vector<Item> &wr_items, const std::vector<Item> &fin_items, std::vector<uint64_t> *got_wr,
+              std::vector<uint64_t> *got_fin) {
+    const auto t0 = clk::now();
+    hipStream_t s = c->stream;
+    // launch tables: write walks, final walks, final single-list counts; one
+    // u64 result counter per item, in the device header
+    Launch wr, fin, fin1;
+    std::vector<std::pair<int, size_t>> fin_at(fin_items.size());
+    size_t nctr = 0;
+    for (const Item &x : wr_items) {
+      VItem it = vitem(x);
+      it.out = x.out->as<uint32_t>();
+      it.out_cap = x.out_rows;
+      it.counter = slot.dev_counter(nctr++);
+      wr.add(it, qs[x.q]->max_nb[x.level], qs[x.q]->max_ndup[x.level]);
+    }
+    std::vector<size_t> fin_ctr(fin_items.size());
+    for (size_t t = 0; t < fin_items.size(); ++t) {
+      const Item &x = fin_items[t];
+      VItem it = vitem(x);
+      fin_ctr[t] = nctr;
+      it.counter = slot.dev_counter(nctr++);
+      if (qs[x.q]->max_nb[x.level] == 1)
+        fin1.add(it, 1, qs[x.q]->max_ndup[x.level]);
+      else
+        fin.add(it, qs[x.q]->max_nb[x.level], qs[x.q]->max_ndup[x.level]);
+    }
+    if (nctr > slot.cap) throw Error{GEM_E_STATE, "more vjoin tasks than launch slots"};
+    Launch *L3[3] = {&wr, &fin, &fin1};
+    for (int i = 0; i < 3; ++i) {
+      Launch &L = *L3[i];
+      if (L.items.size() > slot.cap) throw Error{GEM_E_STATE, "more vjoin items than launch slots"};
+      for (size_t t = 0; t < L.items.size(); ++t) slot.h_items[i][t] = L.items[t];
+      for (size_t t = 0; t < L.blk.size(); ++t) slot.h_blk[i][t] = L.blk[t];
+      if (L.blk.empty()) slot.h_blk[i][0] = 0;
+      *slot.nitems(i) = (uint32_t)L.items.size();
+      ext_tasks += L.blk.empty() ? 0 : L.blk.back();
+    }
+    // counters / work queues / walked-entry counts travel zeroed in the copy
+    std::memset(slot.h_hdr + ExtSlot::HDR_QUEUE, 0, slot.hdr_bytes - ExtSlot::HDR_QUEUE);
+    const bool prof = c->prof.on;
+    if (tc)
+      slot.launch_streamed(c, *tc);
+    else
+      slot.launch(c, c->use_graph && !prof);
+    GEM_HIP_CHECK(hipStreamSynchronize(s));
+    if (tc) tc->finish();
+    const uint64_t *hc = slot.counters();
+    got_wr->resize(wr_items.size());
+    got_fin->resize(fin_items.size());
+    for (size_t t = 0; t < wr_items.size(); ++t) {
+      (*got_wr)[t] = hc[t];
+      if (hc[t] > wr_items[t].out_rows) throw Error{GEM_E_STATE, "vjoin output exceeded its bound"};
+    }
+    for (size_t t = 0; t < fin_items.size(); ++t) (*got_fin)[t] = hc[fin_ctr[t]];
+    if (prof) {
+      const uint64_t *pc = slot.pitems();
+      // algorithmic bytes: rows read (4 B per column + an offset pair per
+      // backward list), 4 B per list entry walked, output rows written
+      auto rows_bytes = [&](const Launch &L) {
+        double b = 0;
+        for (const auto &it : L.items)
+          b += (double)(it.row_end - it.row_begin) * (4.0 * it.k + 16.0 * (double)L.max_nb);
+        return b;
+      };
+      // (the heavy-row kernels: the entries they walk and the rows they write)
+      prof_bytes(c, K_VEXT_WRITE, rows_bytes(wr) + 4.0 * (double)pc[0] + (double)pc[4]);
+      prof_bytes(c, K_VEXT_FINAL, rows_bytes(fin) + 4.0 * (double)pc[1]);
+      prof_bytes(c, K_VCOUNT1, rows_bytes(fin1));
+      prof_bytes(c, K_VEXT_HEAVY_WRITE, 4.0 * (double)pc[2] + (double)pc[5]);
+      prof_bytes(c, K_VEXT_HEAVY_FINAL, 4.0 * (double)pc[3]);
+    }
+    prof_flush(c);
+    t_ext += std::chrono::duration<double>(clk::now() - t0).count();
+  }
+
+  void on_iteration(uint64_t iter, size_t active, uint64_t live, uint64_t budget) {
+    if (!verbose || clk::now() - last_print < std::chrono::seconds(2)) return;
+    last_print = clk::now();
+    std::fprintf(stderr, "[vjoin] iteration %llu: %zu of %zu queries active, pending levels %.2f of %.2f GiB\n",
+                 (unsigned long long)iter, active, qs.size(), live / 1073741824.0, budget / 1073741824.0);
+    std::fflush(stderr);
+  }
+};
+
+}  // namespace
+
 void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   hipStream_t s = c_->stream;
   g_persistent_blocks = c_->persistent_blocks;
-  // budget_bytes_ is shared by the queries still running: a query's chunk may
-  // use total / active (at most 4 GiB, at least 1 MiB), so a long-running last
-  // query is not held to a small slice.
+  // the level budget: given, or a quarter of what the device could still
+  // give the run's level buffers (free memory plus empty pool segments)
   uint64_t total_budget = budget_bytes_;
   if (!total_budget) {
-    // a quarter of what the device could still give the run's level buffers
     size_t free_b = 0, total_b = 0;
     GEM_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-    total_budget = std::max<uint64_t>((free_b + pool_cached_bytes()) / 4, 1ull << 30);
+    total_budget = std::max<uint64_t>((free_b + pool_reusable_bytes()) / 4, 1ull << 30);
   }
   total_budget = std::max<uint64_t>(total_budget, 1ull << 20);
   // fixed launch buffers for the run (one item per query per launch at most)
@@ -876,282 +1037,52 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
     tc->attach(qs);
   }
   // seed each query with the identity level of its first vertex
+  std::vector<gemh::SQuery<DevBuf> *> sq;
   for (VQueryState *S : qs) {
-    const uint32_t a = S->plan.vorder[0];
-    const uint64_t n0 = S->qd->ncan[a];
-    S->level_rows.assign(S->qd->Q.nv, 0);
-    S->level_rows[0] = n0;
+    gemh::SQuery<DevBuf> &Q = S->s;
+    Q = gemh::SQuery<DevBuf>{};
+    Q.nv = S->qd->Q.nv;
+    Q.nb = S->max_nb;
+    Q.level_rows.assign(Q.nv, 0);
+    sq.push_back(&Q);
+    const uint64_t n0 = S->qd->ncan[S->plan.vorder[0]];
+    Q.level_rows[0] = n0;
     if (n0 == 0) {
-      S->done = true;
+      Q.done = true;
       continue;
     }
     auto buf = std::make_shared<DevBuf>();
     buf->alloc(n0 * 4, s);
     k_viota<<<(uint32_t)((n0 + 255) / 256), 256, 0, s>>>(buf->as<uint32_t>(), n0);
     GEM_LAUNCH_CHECK("k_viota");
-    S->stack.push_back(VStackItem{1, buf, n0, 1, {}, 0, false});
+    gemh::SLevel<DevBuf> L;
+    L.level = 1;
+    L.rows = std::move(buf);
+    L.nrows = n0;
+    L.k = 1;
+    Q.stack.push_back(std::move(L));
   }
-  // GEM_VERBOSE=1: a progress line on stderr every ~2 s of a long batch
   const char *verbose_env = std::getenv("GEM_VERBOSE");
   const bool verbose = verbose_env && verbose_env[0] == '1';
-  using clk = std::chrono::steady_clock;
-  auto last_print = clk::now();
-  auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
-  // where the run's time goes (reported with GEM_VERBOSE=1): host planning,
-  // length passes (launch to sync), extend steps (launch to sync), and how
-  // many iterations ran on the 1 MiB minimum chunk budget
-  double t_plan = 0, t_len = 0, t_ext = 0, t_alloc = 0;
-  uint64_t n_len = 0, n_floor = 0, ext_tasks = 0;
-  const auto run_start = clk::now();
-  for (;;) {
-    const auto it_start = clk::now();
-    std::vector<VQueryState *> act;
-    for (VQueryState *S : qs)
-      if (!S->done && !S->stack.empty()) act.push_back(S);
-    if (act.empty()) break;
-    ++iterations_;
-    // live pending rows of every query count against the budget; what is
-    // left is shared by the running queries (4 GiB per query at most, 1 MiB
-    // at least so that depth-first progress, which frees memory, continues)
-    uint64_t live = 0;
-    for (VQueryState *S : qs)
-      for (const VStackItem &it : S->stack) live += it.rows ? it.rows->bytes : 0;
-    const uint64_t left = total_budget > live ? total_budget - live : 0;
-    const uint64_t budget = std::max<uint64_t>(std::min<uint64_t>(left / act.size(), 4ull << 30), 1ull << 20);
-    if (left / act.size() < (1ull << 20)) ++n_floor;
-    if (verbose && clk::now() - last_print > std::chrono::seconds(2)) {
-      last_print = clk::now();
-      std::fprintf(stderr,
-                   "[vjoin] iteration %llu: %zu of %zu queries active, pending levels %.2f GiB, chunk budget %.1f "
-                   "MiB\n",
-                   (unsigned long long)iterations_, act.size(), qs.size(), live / 1073741824.0, budget / 1048576.0);
-      std::fflush(stderr);
-    }
-    // ---- length pass for unmeasured, non-final top items
-    Launch lenL;
-    std::vector<VQueryState *> len_owner;
-    for (VQueryState *S : act) {
-      VStackItem &top = S->stack.back();
-      const bool final_level = top.level + 1 == S->qd->Q.nv;
-      // final levels are measured only when they walk lists (nb > 1) under a
-      // cap: their blocks are then counted a cap-sized piece at a time
-      if (top.measured || (final_level && (S->max_nb[top.level] == 1 || !cap_))) continue;
-      VItem it{};
-      it.lv = S->levels.as<VLevel>() + top.level;
-      it.rows = top.rows->as<uint32_t>();
-      it.row_begin = 0;
-      it.row_end = top.nrows;
-      it.k = top.k;
-      lenL.add(it, S->max_nb[top.level], S->max_ndup[top.level]);
-      len_owner.push_back(S);
-    }
-    if (!lenL.empty()) {
-      const auto t0 = clk::now();
-      t_plan += secs(it_start, t0);
-      ++n_len;
-      DevLaunch D;
-      upload(c_, lenL, D);
-      const uint64_t nblk = lenL.blk.back();
-      DevBuf bu, bp;
-      bu.alloc(nblk * 8, s);
-      bp.alloc((nblk + 1) * 8, s);
-      {
-        ProfLaunch pl(c_, K_VLEN);
-        k_vlen<<<grid_for(nblk), VB, 0, s>>>(D.items.as<VItem>(), (uint32_t)lenL.items.size(), D.blk.as<uint64_t>(),
-                                              D.queue.as<unsigned int>(), bu.as<uint64_t>());
-        GEM_LAUNCH_CHECK("k_vlen");
-      }
-      scan_u64(c_, bu.as<uint64_t>(), bp.as<uint64_t>(), nblk);
-      std::vector<uint64_t> pref(nblk + 1);
-      GEM_HIP_CHECK(hipMemcpyAsync(pref.data(), bp.p, (nblk + 1) * 8, hipMemcpyDeviceToHost, s));
-      GEM_HIP_CHECK(hipStreamSynchronize(s));
-      for (size_t t = 0; t < len_owner.size(); ++t) {
-        VStackItem &top = len_owner[t]->stack.back();
-        const uint64_t b0 = lenL.blk[t], b1 = lenL.blk[t + 1];
-        top.block_u.assign(pref.begin() + b0, pref.begin() + b1 + 1);
-        const uint64_t base = top.block_u[0];
-        for (auto &x : top.block_u) x -= base;
-        top.measured = true;
-        // rows + one offset pair per backward list
-        prof_bytes(c_, K_VLEN, (double)top.nrows * (4.0 * top.k + 16.0 * len_owner[t]->max_nb[top.level]));
-      }
-      t_len += secs(t0, clk::now());
-    }
-    const auto t_ext_plan = clk::now();
-    // ---- extend pass: write chunks and final counts
-    Launch wr, fin, fin1;
-    struct WTask {
-      VQueryState *S;
-      std::shared_ptr<DevBuf> out;
-      uint32_t level, k;
-      uint64_t cap_rows;
-      size_t ctr;
-    };
-    struct FTask {
-      VQueryState *S;
-      size_t ctr;
-    };
-    std::vector<WTask> wt;
-    std::vector<FTask> ft;
-    std::vector<std::shared_ptr<DevBuf>> keep_alive;  // input rows stay allocated until the launches finish
-    size_t nctr = 0;  // one u64 counter per task
-    for (VQueryState *S : act) {
-      VStackItem &top = S->stack.back();
-      const uint32_t nv = S->qd->Q.nv;
-      const bool final_level = top.level + 1 == nv;
-      const uint64_t nblocks = (top.nrows + VB - 1) / VB;
-      VItem it{};
-      it.lv = S->levels.as<VLevel>() + top.level;
-      it.rows = top.rows->as<uint32_t>();
-      it.k = top.k;
-      keep_alive.push_back(top.rows);
-      if (final_level) {
-        uint64_t b1 = nblocks;
-        if (top.measured) {
-          // count a piece whose survivor bound is about twice what is still
-          // missing to exceed the cap (at least 2^26 list entries of work)
-          const uint64_t need = cap_ + 1 > S->count ? cap_ + 1 - S->count : 1;
-          const uint64_t target = std::max<uint64_t>(2 * need, 1ull << 26);
-          b1 = top.cursor + 1;
-          while (b1 < nblocks && top.block_u[b1] - top.block_u[top.cursor] < target) ++b1;
-        }
-        it.row_begin = (uint64_t)top.cursor * VB;
-        it.row_end = std::min<uint64_t>(b1 * VB, top.nrows);
-        if (top.measured) S->work += top.block_u[b1] - top.block_u[top.cursor];
-        top.cursor = b1;
-        it.counter = reinterpret_cast<unsigned long long *>((uintptr_t)nctr);  // patched below
-        ft.push_back(FTask{S, nctr++});
-        S->processed += it.row_end - it.row_begin;
-        S->work += it.row_end - it.row_begin;
-        if (S->max_nb[top.level] == 1)
-          fin1.add(it, 1, S->max_ndup[top.level]);
-        else
-          fin.add(it, S->max_nb[top.level], S->max_ndup[top.level]);
-        if (top.cursor >= nblocks) S->stack.pop_back();
-        continue;
-      }
-      // one chunk of blocks whose output bound fits the per-query budget
-      const uint64_t row_bytes = 4ull * (top.k + 1);
-      const uint64_t lim = std::max<uint64_t>(budget / row_bytes, 1);
-      uint64_t b0 = top.cursor, b1 = b0 + 1;
-      while (b1 < nblocks && top.block_u[b1 + 1] - top.block_u[b0] <= lim) ++b1;
-      const uint64_t ub = top.block_u[b1] - top.block_u[b0];
-      it.row_begin = b0 * VB;
-      it.row_end = std::min<uint64_t>(b1 * VB, top.nrows);
-      S->processed += it.row_end - it.row_begin;
-      S->work += it.row_end - it.row_begin + ub;
-      top.cursor = b1;
-      if (ub == 0) {
-        if (top.cursor >= nblocks) S->stack.pop_back();
-        continue;
-      }
-      auto out = std::make_shared<DevBuf>();
-      const auto ta = clk::now();
-      out->alloc(ub * row_bytes, s);
-      t_alloc += secs(ta, clk::now());
-      it.out = out->as<uint32_t>();
-      it.out_cap = ub;
-      it.counter = reinterpret_cast<unsigned long long *>((uintptr_t)nctr);
-      wt.push_back(WTask{S, out, top.level + 1, top.k + 1, ub, nctr++});
-      wr.add(it, S->max_nb[top.level], S->max_ndup[top.level]);
-      if (top.cursor >= nblocks) S->stack.pop_back();
-    }
-    if (nctr == 0) continue;
-    if (nctr > slot.cap || wr.items.size() > slot.cap || fin.items.size() > slot.cap || fin1.items.size() > slot.cap)
-      throw Error{GEM_E_STATE, "more vjoin tasks than launch slots"};
-    // fill the pinned launch tables: counters point into the device header,
-    // counters / work queues / walked-entry counts travel zeroed in the copy
-    Launch *L3[3] = {&wr, &fin, &fin1};
-    for (int i = 0; i < 3; ++i) {
-      Launch &L = *L3[i];
-      for (size_t t = 0; t < L.items.size(); ++t) {
-        VItem it = L.items[t];
-        it.counter = slot.dev_counter((size_t)(uintptr_t)it.counter);
-        slot.h_items[i][t] = it;
-      }
-      for (size_t t = 0; t < L.blk.size(); ++t) slot.h_blk[i][t] = L.blk[t];
-      if (L.blk.empty()) slot.h_blk[i][0] = 0;
-      *slot.nitems(i) = (uint32_t)L.items.size();
-    }
-    std::memset(slot.h_hdr + ExtSlot::HDR_QUEUE, 0, slot.hdr_bytes - ExtSlot::HDR_QUEUE);
-    const bool prof = c_->prof.on;
-    const auto t1 = clk::now();
-    t_plan += secs(t_ext_plan, t1);
-    ext_tasks += wr.blk.empty() ? 0 : wr.blk.back();
-    ext_tasks += fin.blk.empty() ? 0 : fin.blk.back();
-    ext_tasks += fin1.blk.empty() ? 0 : fin1.blk.back();
-    if (tc)
-      slot.launch_streamed(c_, *tc);
-    else
-      slot.launch(c_, c_->use_graph && !prof);
-    GEM_HIP_CHECK(hipStreamSynchronize(s));
-    t_ext += secs(t1, clk::now());
-    if (tc) tc->finish();
-    std::vector<uint64_t> hc(nctr, 0);
-    for (size_t i = 0; i < nctr; ++i) hc[i] = slot.counters()[i];
-    const uint64_t *pc = slot.pitems();
-    if (prof) {
-      // algorithmic bytes: rows read (4 B per column + an offset pair per
-      // backward list), 4 B per list entry walked, output rows written
-      auto rows_bytes = [&](const Launch &L) {
-        double b = 0;
-        for (const auto &it : L.items)
-          b += (double)(it.row_end - it.row_begin) * (4.0 * it.k + 16.0 * (double)L.max_nb);
-        return b;
-      };
-      // (the heavy-row kernels: the entries they walk and the rows they write)
-      prof_bytes(c_, K_VEXT_WRITE, rows_bytes(wr) + 4.0 * (double)pc[0] + (double)pc[4]);
-      prof_bytes(c_, K_VEXT_FINAL, rows_bytes(fin) + 4.0 * (double)pc[1]);
-      prof_bytes(c_, K_VCOUNT1, rows_bytes(fin1));
-      prof_bytes(c_, K_VEXT_HEAVY_WRITE, 4.0 * (double)pc[2] + (double)pc[5]);
-      prof_bytes(c_, K_VEXT_HEAVY_FINAL, 4.0 * (double)pc[3]);
-    }
-    for (auto &w : wt) {
-      const uint64_t got = hc[w.ctr];
-      if (got > w.cap_rows) throw Error{GEM_E_STATE, "vjoin output exceeded its bound"};
-      w.S->level_rows[w.level - 1] += got;
-      if (!got) continue;
-      // the output was allocated for the length pass's bound; when fewer than
-      // half of those rows survived, the rows move to a buffer of their size
-      // (pending rows are what the level budget is charged with)
-      std::shared_ptr<DevBuf> rows = w.out;
-      const uint64_t rb = 4ull * w.k;
-      if (2 * got <= w.cap_rows && w.cap_rows * rb >= (4ull << 20)) {
-        const auto ta = clk::now();
-        rows = std::make_shared<DevBuf>();
-        rows->alloc(got * rb, s);
-        GEM_HIP_CHECK(hipMemcpyAsync(rows->p, w.out->p, got * rb, hipMemcpyDeviceToDevice, s));
-        t_alloc += secs(ta, clk::now());
-      }
-      w.S->stack.push_back(VStackItem{w.level, std::move(rows), got, w.k, {}, 0, false});
-    }
-    for (auto &f : ft) {
-      const uint64_t got = hc[f.ctr];
-      f.S->level_rows[f.S->qd->Q.nv - 1] += got;
-      f.S->count += got;
-    }
-    for (VQueryState *S : act) {
-      if (cap_ && S->count > cap_) {
-        S->saturated = true;
-        S->done = true;
-        S->stack.clear();
-      } else if (work_limit_ && S->work > work_limit_) {
-        S->work_limited = true;
-        S->done = true;
-        S->stack.clear();
-      } else if (S->stack.empty()) {
-        S->done = true;
-      }
-    }
-    prof_flush(c_);
-  }
+  HipSchedBackend be{c_, qs, slot, tc.get(), verbose};
+  gemh::SchedConfig cfg;
+  cfg.cap = cap_;
+  cfg.work_limit = work_limit_;
+  cfg.budget = total_budget;
+  const auto t0 = std::chrono::steady_clock::now();
+  stats_ = gemh::SchedStats{};
+  gemh::run_schedule(be, sq, cfg, &stats_);
+  c_->last_sched = stats_;
+  c_->last_budget = total_budget;
   if (verbose) {
     std::fprintf(stderr,
-                 "[vjoin] run of %zu queries: %llu iterations in %.2f s: host planning %.2f s (level allocations "
-                 "%.2f s), %llu length passes %.2f s, extend steps %.2f s (%llu 256-row tasks); %llu iterations on "
-                 "the minimum chunk budget\n",
-                 qs.size(), (unsigned long long)iterations_, secs(run_start, clk::now()), t_plan, t_alloc,
-                 (unsigned long long)n_len, t_len, t_ext, (unsigned long long)ext_tasks, (unsigned long long)n_floor);
+                 "[vjoin] run of %zu queries: %llu iterations in %.2f s: %llu length passes %.2f s, extend steps "
+                 "%.2f s (%llu 256-row tasks); budget %.2f GiB, peak charged %.2f GiB, %llu forced iterations\n",
+                 qs.size(), (unsigned long long)stats_.iterations,
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),
+                 (unsigned long long)stats_.length_passes, be.t_len, be.t_ext, (unsigned long long)be.ext_tasks,
+                 total_budget / 1073741824.0, stats_.peak_charged / 1073741824.0,
+                 (unsigned long long)stats_.forced);
     std::fflush(stderr);
   }
 }

@@ -1,0 +1,361 @@
+// Host scheduler of the batched vertex-at-a-time join: per query a depth-first
+// stack of pending level ranges, the length passes that bound each level's
+// output, chunk planning against a shared memory budget, and the bookkeeping
+// of counts, caps and work limits.  Independent of HIP: the device work (and
+// its buffers) is behind a backend, so the same code runs in the library and
+// in CPU tests with a simulated backend.
+//
+// Memory rule.  Pending level rows of every query ("live") plus the output
+// buffers allocated in an iteration are charged against `budget`.  Each
+// iteration the budget left is shared by the queries that need an output
+// buffer: first an equal share each, then what the others left unused, deeper
+// levels first.  A query whose next 64-row unit does not fit what is left
+// waits.  Only when no query can take anything and no count-only work is
+// pending is the budget exceeded: the deepest pending level (whose memory is
+// released soonest) takes a chunk of at most `force_chunk` bytes, or one unit
+// if that is larger.  Such forced chunks pile up along one depth-first path at
+// most, so the charged bytes never exceed
+//   budget + (levels - 1) * max(force_chunk, largest unit bound).
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace gemh {
+
+// Rows per planning unit (one wave task of the extend kernels).
+constexpr uint32_t SCHED_UNIT = 64;
+
+struct SchedConfig {
+  uint64_t cap = 0;         // count cap (0 = none): a query stops once its count exceeds it
+  uint64_t work_limit = 0;  // 0 = none
+  uint64_t budget = 0;      // bytes for pending + newly allocated level rows
+  uint64_t max_chunk = 4ull << 30;       // largest output buffer of one query in one iteration
+  uint64_t force_chunk = 0;              // past the budget (see above); 0 = budget / 32
+  uint64_t final_piece = 1ull << 26;     // least output bound of a capped final-level piece
+  uint64_t shrink_min = 4ull << 20;      // outputs at least this large move when half unused
+};
+
+struct SchedStats {
+  uint64_t iterations = 0;
+  uint64_t length_passes = 0;
+  uint64_t forced = 0;          // iterations that forced a unit past the budget
+  uint64_t waits = 0;           // (query, iteration) pairs deferred for lack of budget
+  uint64_t peak_charged = 0;    // max over iterations of live + this iteration's outputs
+  uint64_t max_overshoot = 0;   // max over iterations of charged bytes beyond the budget
+  uint64_t max_unit_bytes = 0;  // largest output bound of a forced unit
+  uint64_t extend_items = 0;    // items handed to extend steps
+};
+
+template <class Buf>
+struct SLevel {
+  uint32_t level = 0;  // the level these rows extend to next (query vertex vorder[level])
+  std::shared_ptr<Buf> rows;
+  uint64_t nrows = 0;
+  uint32_t k = 0;                 // columns per row
+  std::vector<uint64_t> unit_u;   // prefix of per-unit output bounds (nunits + 1) once measured
+  uint64_t cursor = 0;            // next unit
+  bool measured = false;
+  uint64_t nunits() const { return (nrows + SCHED_UNIT - 1) / SCHED_UNIT; }
+};
+
+template <class Buf>
+struct SQuery {
+  uint32_t nv = 0;
+  std::vector<uint32_t> nb;  // backward lists of each level (nb[i], i >= 1)
+  std::vector<SLevel<Buf>> stack;
+  std::vector<uint64_t> level_rows;  // rows after each level
+  uint64_t count = 0, processed = 0;
+  // rows read plus (upper bound of) list entries walked: what the work limit
+  // bounds, since one row with hub lists can cost 10^5 entries
+  uint64_t work = 0;
+  bool saturated = false, done = false, work_limited = false;
+};
+
+// One unit of a launch: rows [row_begin, row_end) of a pending level of
+// query `q`, extended to `level`; `out` (write items) receives at most
+// out_rows rows of k + 1 columns.
+template <class Buf>
+struct SItem {
+  uint32_t q = 0, level = 0, k = 0;
+  const Buf *rows = nullptr;
+  uint64_t row_begin = 0, row_end = 0;
+  Buf *out = nullptr;
+  uint64_t out_rows = 0;
+};
+
+// Chunk planning of one iteration.  cand[i]: a query whose top level writes
+// output; prefix = its unit bound prefix, cursor = next unit, row_bytes =
+// bytes of one output row, depth = its level.  `left` = budget not yet
+// charged; other_progress = count-only work runs this iteration anyway.
+// Returns the end unit of each query's chunk (== cursor: it waits) and the
+// index of the query forced past the budget (-1 if none).
+struct ChunkCand {
+  const std::vector<uint64_t> *prefix;
+  uint64_t cursor, nunits, row_bytes;
+  uint32_t depth;
+};
+inline int plan_chunks(const std::vector<ChunkCand> &cand, uint64_t left, uint64_t max_chunk, uint64_t force_chunk,
+                       bool other_progress, std::vector<uint64_t> *end) {
+  const size_t n = cand.size();
+  end->resize(n);
+  for (size_t i = 0; i < n; ++i) (*end)[i] = cand[i].cursor;
+  if (!n) return -1;
+  auto bytes = [&](size_t i, uint64_t e) {
+    const std::vector<uint64_t> &P = *cand[i].prefix;
+    return (P[e] - P[cand[i].cursor]) * cand[i].row_bytes;
+  };
+  // largest e in [(*end)[i], nunits] with bytes(i, e) <= lim
+  auto reach = [&](size_t i, uint64_t lim) {
+    uint64_t lo = (*end)[i], hi = cand[i].nunits;
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo + 1) / 2;
+      if (bytes(i, mid) <= lim)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    return lo;
+  };
+  const uint64_t share = std::min<uint64_t>(left / n, max_chunk);
+  uint64_t used = 0;
+  for (size_t i = 0; i < n; ++i) {
+    (*end)[i] = reach(i, share);
+    used += bytes(i, (*end)[i]);
+  }
+  uint64_t spare = left - used;
+  if (spare) {
+    std::vector<size_t> by_depth(n);
+    for (size_t i = 0; i < n; ++i) by_depth[i] = i;
+    std::stable_sort(by_depth.begin(), by_depth.end(),
+                     [&](size_t a, size_t b) { return cand[a].depth > cand[b].depth; });
+    for (size_t j = 0; j < n && spare; ++j) {
+      const size_t i = by_depth[j];
+      const uint64_t have = bytes(i, (*end)[i]);
+      (*end)[i] = reach(i, std::min<uint64_t>(max_chunk, have + spare));
+      spare -= bytes(i, (*end)[i]) - have;
+    }
+  }
+  bool any = other_progress;
+  for (size_t i = 0; i < n && !any; ++i) any = (*end)[i] > cand[i].cursor;
+  if (any) return -1;
+  // nothing fits and nothing else moves: the deepest level goes past the
+  // budget by one chunk (ties: the smaller next unit)
+  size_t f = 0;
+  for (size_t i = 1; i < n; ++i) {
+    if (cand[i].depth > cand[f].depth ||
+        (cand[i].depth == cand[f].depth && bytes(i, cand[i].cursor + 1) < bytes(f, cand[f].cursor + 1)))
+      f = i;
+  }
+  (*end)[f] = std::max<uint64_t>(cand[f].cursor + 1, reach(f, std::min(max_chunk, force_chunk)));
+  return (int)f;
+}
+
+// Backend interface (a template parameter, not a virtual base):
+//   using Buf = ...;
+//   static uint64_t buf_bytes(const Buf &);
+//   std::shared_ptr<Buf> alloc(uint64_t bytes);
+//   // per item, the prefix of per-unit output bounds over its rows
+//   void length_pass(const std::vector<SItem<Buf>> &items, std::vector<std::vector<uint64_t>> *prefix);
+//   // rows written per write item, embeddings counted per final item
+//   void extend(const std::vector<SItem<Buf>> &wr, const std::vector<SItem<Buf>> &fin,
+//               std::vector<uint64_t> *got_wr, std::vector<uint64_t> *got_fin);
+//   // a buffer of `bytes` holding the first `bytes` of `from`
+//   std::shared_ptr<Buf> shrink(const std::shared_ptr<Buf> &from, uint64_t bytes);
+//   void on_iteration(uint64_t iter, size_t active, uint64_t live, uint64_t budget);
+//
+// The queries' stacks must be seeded (level 1 = the identity rows of the
+// first vertex's candidates) and their nb / nv filled.
+template <class BE>
+void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const SchedConfig &cfg, SchedStats *st) {
+  using Buf = typename BE::Buf;
+  using Item = SItem<Buf>;
+  const uint64_t budget = cfg.budget;
+  for (;;) {
+    std::vector<uint32_t> act;
+    for (uint32_t i = 0; i < qs.size(); ++i)
+      if (!qs[i]->done && !qs[i]->stack.empty()) act.push_back(i);
+    if (act.empty()) break;
+    ++st->iterations;
+    uint64_t live = 0;
+    for (SQuery<Buf> *S : qs)
+      for (const SLevel<Buf> &L : S->stack) live += L.rows ? BE::buf_bytes(*L.rows) : 0;
+    be.on_iteration(st->iterations, act.size(), live, budget);
+
+    // ---- length pass: output bounds of every unmeasured top level (final
+    // levels only when they walk lists under a cap: their units are then
+    // counted a cap-sized piece at a time)
+    {
+      std::vector<Item> items;
+      std::vector<uint32_t> owner;
+      for (uint32_t qi : act) {
+        SQuery<Buf> &S = *qs[qi];
+        SLevel<Buf> &top = S.stack.back();
+        const bool final_level = top.level + 1 == S.nv;
+        if (top.measured || (final_level && (S.nb[top.level] == 1 || !cfg.cap))) continue;
+        Item it;
+        it.q = qi;
+        it.level = top.level;
+        it.k = top.k;
+        it.rows = top.rows.get();
+        it.row_begin = 0;
+        it.row_end = top.nrows;
+        items.push_back(it);
+        owner.push_back(qi);
+      }
+      if (!items.empty()) {
+        ++st->length_passes;
+        std::vector<std::vector<uint64_t>> pref;
+        be.length_pass(items, &pref);
+        for (size_t t = 0; t < owner.size(); ++t) {
+          SLevel<Buf> &top = qs[owner[t]]->stack.back();
+          top.unit_u = std::move(pref[t]);
+          top.measured = true;
+        }
+      }
+    }
+
+    // ---- count-only pieces of final levels (no output memory)
+    std::vector<Item> wr, fin;
+    std::vector<std::shared_ptr<Buf>> keep_alive, outs;
+    std::vector<uint32_t> wr_owner, fin_owner;
+    std::vector<ChunkCand> cand;
+    std::vector<uint32_t> cand_owner;
+    for (uint32_t qi : act) {
+      SQuery<Buf> &S = *qs[qi];
+      SLevel<Buf> &top = S.stack.back();
+      const uint64_t nu = top.nunits();
+      if (top.level + 1 != S.nv) {
+        cand.push_back(ChunkCand{&top.unit_u, top.cursor, nu, 4ull * (top.k + 1), top.level});
+        cand_owner.push_back(qi);
+        continue;
+      }
+      uint64_t b1 = nu;
+      if (top.measured) {
+        // a piece whose output bound is about twice what is still missing
+        // to exceed the cap (at least final_piece list entries of work)
+        const uint64_t need = cfg.cap + 1 > S.count ? cfg.cap + 1 - S.count : 1;
+        const uint64_t target = std::max<uint64_t>(2 * need, cfg.final_piece);
+        const std::vector<uint64_t> &P = top.unit_u;
+        uint64_t lo = top.cursor + 1, hi = nu;  // smallest b1 with P[b1] - P[cursor] >= target, else nu
+        while (lo < hi) {
+          const uint64_t mid = lo + (hi - lo) / 2;
+          if (P[mid] - P[top.cursor] >= target)
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        b1 = lo;
+        S.work += P[b1] - P[top.cursor];
+      }
+      Item it;
+      it.q = qi;
+      it.level = top.level;
+      it.k = top.k;
+      it.rows = top.rows.get();
+      it.row_begin = top.cursor * SCHED_UNIT;
+      it.row_end = std::min<uint64_t>(b1 * SCHED_UNIT, top.nrows);
+      keep_alive.push_back(top.rows);
+      S.processed += it.row_end - it.row_begin;
+      S.work += it.row_end - it.row_begin;
+      top.cursor = b1;
+      fin.push_back(it);
+      fin_owner.push_back(qi);
+      if (top.cursor >= nu) S.stack.pop_back();
+    }
+
+    // ---- output chunks under the budget
+    const uint64_t left = budget > live ? budget - live : 0;
+    std::vector<uint64_t> end;
+    const int forced = plan_chunks(cand, left, cfg.max_chunk, cfg.force_chunk ? cfg.force_chunk : budget / 32,
+                                   !fin.empty(), &end);
+    uint64_t charged = live;
+    for (size_t c = 0; c < cand.size(); ++c) {
+      const uint32_t qi = cand_owner[c];
+      SQuery<Buf> &S = *qs[qi];
+      SLevel<Buf> &top = S.stack.back();
+      const uint64_t b0 = top.cursor, b1 = end[c];
+      if (b1 == b0) {
+        ++st->waits;
+        continue;
+      }
+      const uint64_t ub = top.unit_u[b1] - top.unit_u[b0];
+      const uint64_t rb = 4ull * (top.k + 1);
+      Item it;
+      it.q = qi;
+      it.level = top.level;
+      it.k = top.k;
+      it.rows = top.rows.get();
+      it.row_begin = b0 * SCHED_UNIT;
+      it.row_end = std::min<uint64_t>(b1 * SCHED_UNIT, top.nrows);
+      S.processed += it.row_end - it.row_begin;
+      S.work += it.row_end - it.row_begin + ub;
+      top.cursor = b1;
+      if (ub) {
+        keep_alive.push_back(top.rows);
+        std::shared_ptr<Buf> out = be.alloc(ub * rb);
+        charged += ub * rb;
+        it.out = out.get();
+        it.out_rows = ub;
+        outs.push_back(std::move(out));
+        wr.push_back(it);
+        wr_owner.push_back(qi);
+        if ((int)c == forced) st->max_unit_bytes = std::max(st->max_unit_bytes, ub * rb);
+      }
+      if (top.cursor >= top.nunits()) S.stack.pop_back();
+    }
+    if (forced >= 0) ++st->forced;
+    st->peak_charged = std::max(st->peak_charged, charged);
+    if (charged > budget) st->max_overshoot = std::max(st->max_overshoot, charged - budget);
+
+    // ---- extend step
+    if (!wr.empty() || !fin.empty()) {
+      st->extend_items += wr.size() + fin.size();
+      std::vector<uint64_t> got_wr, got_fin;
+      be.extend(wr, fin, &got_wr, &got_fin);
+      for (size_t t = 0; t < wr.size(); ++t) {
+        SQuery<Buf> &S = *qs[wr_owner[t]];
+        const uint64_t got = got_wr[t];
+        const Item &it = wr[t];
+        S.level_rows[it.level] += got;
+        std::shared_ptr<Buf> rows = std::move(outs[t]);
+        if (!got) continue;  // the buffer goes back now
+        // the output was allocated for the length pass's bound; when fewer
+        // than half of those rows survived, they move to a buffer of their
+        // size, and the bound-sized one goes back at once (pending rows are
+        // what the budget is charged with)
+        const uint64_t rb = 4ull * (it.k + 1);
+        if (2 * got <= it.out_rows && it.out_rows * rb >= cfg.shrink_min) rows = be.shrink(rows, got * rb);
+        SLevel<Buf> L;
+        L.level = it.level + 1;
+        L.rows = std::move(rows);
+        L.nrows = got;
+        L.k = it.k + 1;
+        S.stack.push_back(std::move(L));
+      }
+      for (size_t t = 0; t < fin.size(); ++t) {
+        SQuery<Buf> &S = *qs[fin_owner[t]];
+        S.level_rows[S.nv - 1] += got_fin[t];
+        S.count += got_fin[t];
+      }
+    }
+    for (uint32_t qi : act) {
+      SQuery<Buf> &S = *qs[qi];
+      if (cfg.cap && S.count > cfg.cap) {
+        S.saturated = true;
+        S.done = true;
+        S.stack.clear();
+      } else if (cfg.work_limit && S.work > cfg.work_limit) {
+        S.work_limited = true;
+        S.done = true;
+        S.stack.clear();
+      } else if (S.stack.empty()) {
+        S.done = true;
+      }
+    }
+  }
+}
+
+}  // namespace gemh

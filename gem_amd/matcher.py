@@ -8,7 +8,7 @@ import weakref
 
 import numpy as np
 
-from ._lib import (GEM_MAX_QV, BatchOpts, GemError, JoinResult, KernelStat, PhaseTimes, QueryView, StepLog,
+from ._lib import (GEM_MAX_QV, BatchOpts, GemError, JoinResult, KernelStat, MemoryStats, PhaseTimes, QueryView, StepLog,
                    StreamStats, load)
 
 NOT_FOUND = 0xFFFFFFFF
@@ -129,6 +129,15 @@ class GemContext:
 
     def set_profiling(self, on=True):
         self._check(self._lib.gem_hip_set_profiling(self._h, 1 if on else 0), "gem_hip_set_profiling")
+
+    def memory_stats(self, reset_peak=False):
+        """Device allocator figures (in_use, peak_in_use, held, reusable bytes)
+        and the last batched run's level budget, peak charged level bytes,
+        scheduler iterations and iterations forced past the budget."""
+        st = MemoryStats()
+        self._check(self._lib.gem_hip_memory_stats(self._h, 1 if reset_peak else 0, C.byref(st)),
+                    "gem_hip_memory_stats")
+        return {k: int(getattr(st, k)) for k, _ in MemoryStats._fields_}
 
     def kernel_stats(self):
         """{kernel name: dict(launches, ms, bytes)} accumulated while profiling."""

@@ -241,6 +241,26 @@ typedef struct gem_kernel_stat {
 int gem_hip_set_profiling(gem_ctx *ctx, int on);
 int gem_hip_kernel_stats(gem_ctx *ctx, gem_kernel_stat *out, uint32_t max, uint32_t *n);
 
+/* Device memory of the library's caching allocator (shared by every context
+ * of the process) and the memory figures of this context's last batched run.
+ * No reference counterpart: the reference's MemoryManager
+ * (out-of-core/lib/structure/src/memory_manager.cu:196-822) reports none. */
+typedef struct gem_memory_stats {
+  uint64_t in_use;       /* bytes of live device buffers                      */
+  uint64_t peak_in_use;  /* highest in_use since the last reset               */
+  uint64_t held;         /* bytes obtained from the driver                    */
+  uint64_t reusable;     /* held but unused: empty segments, parked blocks    */
+  uint64_t level_budget; /* last batched run: bytes its pending level rows
+                            were charged against                             */
+  uint64_t peak_charged; /* last batched run: most pending + newly allocated
+                            level bytes at once                              */
+  uint64_t iterations;   /* last batched run: scheduler iterations            */
+  uint64_t forced;       /* last batched run: iterations that went past the
+                            budget because nothing else could move           */
+} gem_memory_stats;
+/* reset_peak != 0: restart peak_in_use from the current in_use afterwards. */
+int gem_hip_memory_stats(gem_ctx *ctx, int reset_peak, gem_memory_stats *out);
+
 /* ------------------------------------------------------- batched path */
 /* The batched path has no single reference counterpart: it is the
  * reference's per-query pipeline (in-core/src/main.cu:148-236, join() at
