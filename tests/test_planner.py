@@ -74,6 +74,48 @@ def test_first_edge_rule():
     assert gem_order(q, np.array([0, 10, 10], np.uint64))[0] == 0
 
 
+def test_next_edge_tie_break_by_lattice_id_star3():
+    """Hand-derived from the reference's lattice (lattice.cpp:41-128, BFS from
+    the full edge set, removing edges in index order, ids in discovery order)
+    and find_best_edge_candidate (unifiedTrie.cu:939-1018, candidates in
+    ascending lattice id, strict >).
+
+    Star 0-1, 0-2, 0-3 (edges e0, e1, e2), equal table sizes.
+      id0 {e0,e1,e2}; level 1: id1 {e1,e2}, id2 {e0,e2}, id3 {e0,e1};
+      level 2: from id1: id4 {e2}, id5 {e1}; from id2: {e2} seen, id6 {e0}.
+    First edge: every score ties (same size, same two parents) -> e0.
+    Next: parents of {e0} (id6) are id2 {e0,e2} and id3 {e0,e1}; the tie goes
+    to the lower id, id2, i.e. e2 -- not e1, which the lowest edge index would
+    pick.  Then e1."""
+    q = Query(np.zeros(4, np.uint32), np.array([(0, 1), (0, 2), (0, 3)], np.uint32))
+    sizes = np.array([7, 7, 7], np.uint64)
+    assert gem_order(q, sizes).tolist() == [0, 2, 1]
+    assert oracle.gem_order(q, sizes).tolist() == [0, 2, 1]
+
+
+def test_next_edge_tie_break_by_lattice_id_star4():
+    """Star with four leaves (e0..e3 = 0-1, 0-2, 0-3, 0-4), equal sizes.
+      level 1: id1 {e1,e2,e3}, id2 {e0,e2,e3}, id3 {e0,e1,e3}, id4 {e0,e1,e2};
+      level 2: from id1: id5 {e2,e3}, id6 {e1,e3}, id7 {e1,e2};
+               from id2: id8 {e0,e3}, id9 {e0,e2}; from id3: id10 {e0,e1};
+      level 3: from id5: id11 {e3}, id12 {e2}; from id6: id13 {e1};
+               from id8: id14 {e0}.
+    First edge e0 (all tie).  Parents of {e0} (id14): id8 {e0,e3}, id9
+    {e0,e2}, id10 {e0,e1} -> e3.  Parents of {e0,e3} (id8): id2 {e0,e2,e3},
+    id3 {e0,e1,e3} -> e2.  Then e1: order [0, 3, 2, 1]."""
+    q = Query(np.zeros(5, np.uint32), np.array([(0, 1), (0, 2), (0, 3), (0, 4)], np.uint32))
+    sizes = np.full(4, 3, np.uint64)
+    assert gem_order(q, sizes).tolist() == [0, 3, 2, 1]
+    assert oracle.gem_order(q, sizes).tolist() == [0, 3, 2, 1]
+    # a strictly smaller table still wins over the tie-break: e1 next
+    sizes2 = np.array([3, 2, 3, 3], np.uint64)
+    # first edge: e1 has the smaller table -> e1; parents of {e1} are
+    # id6 {e1,e3}, id7 {e1,e2}, id10 {e0,e1} (equal sizes 3) -> e3, then from
+    # {e1,e3} (id6): id1 {e1,e2,e3}, id3 {e0,e1,e3} -> e2, then e0
+    assert gem_order(q, sizes2).tolist() == [1, 3, 2, 0]
+    assert oracle.gem_order(q, sizes2).tolist() == [1, 3, 2, 0]
+
+
 def test_big_query_beyond_lattice():
     """Orders for queries past the reference's 32-edge bitset are defined and
     connected."""
