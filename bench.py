@@ -3,16 +3,20 @@
 
 Default workload: the 512-query R-MAT batch (2^22 vertices, 32 Zipf labels,
 queries of 4-32 vertices).  A "step" is one pass of the batched matcher with
-model-chosen join orders over the whole query set (filter -> position index ->
-edge tables -> order candidates + MLP -> vertex-at-a-time join for every query,
-many queries per kernel launch), with the data graph already resident in HBM.
-`value` is that path's queries/s.  The plain path (the reference's per-query
-pipeline with GEM's edge-at-a-time join) is timed in the same run over a
-systematic sample of the queries, and the counts of the two are compared query
-by query.  Counts are exact; on the R-MAT workloads each query's count is
-capped at 1e9 (saturated queries report count = cap), and a query whose join
-exceeds --work-limit is reported unfinished (plain path: rows read by its
-steps; batched path: rows read plus list entries walked).
+model-ranked join orders over the whole query set (filter -> position index
+-> edge tables -> order candidates + MLP -> vertex-at-a-time join for every
+query, many queries per kernel launch), with the data graph already resident
+in HBM.  `value` is that path's queries/s.  Every query ends with its exact
+count or, above the cap (1e9 on the R-MAT workloads), saturated: a query
+whose attempt does more work than its limit starts again with the next
+ranked order and twice the limit (the count does not depend on the order).
+The timed passes run without per-launch profiling (hipGraph replay of the
+extend step); one more pass with profiling on gives the kernel table and the
+roofline.  The plain path (the reference's per-query pipeline with GEM's
+edge-at-a-time join) runs in the same process over a systematic sample of
+the queries with a work limit of its own (rows read); queries it does not
+finish make its rate, and the speedup over it, a bound.  Counts are compared
+query by query with the plain path and with the CPU oracle.
 
 Prints ONE JSON line on stdout (progress goes to stderr).
 """
@@ -90,6 +94,20 @@ def kernel_table(stats):
             for k, v in stats.items() if v["launches"]}
 
 
+def digest(res):
+    """sha256 (16 hex digits) of the per-query (count, saturated) list."""
+    import hashlib
+    return hashlib.sha256(json.dumps([[int(r["count"]), bool(r["saturated"])] for r in res]).encode()).hexdigest()[:16]
+
+
+def mem_line(ctx):
+    m = ctx.memory_stats(reset_peak=True)
+    gib = 1 << 30
+    return (f"pool high-water {m['peak_in_use'] / gib:.1f} GiB (held {m['held'] / gib:.1f} GiB, in use after "
+            f"{m['in_use'] / gib:.2f} GiB); level budget {m['level_budget'] / gib:.1f} GiB, peak charged "
+            f"{m['peak_charged'] / gib:.1f} GiB, {m['iterations']} iterations, {m['forced']} forced")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,9 +117,15 @@ def main():
     ap.add_argument("--workload", default="rmat22", choices=sorted(DESCR))
     ap.add_argument("--queries", type=int, default=0, help="use only the first N queries")
     ap.add_argument("--cap", type=float, default=1e9)
-    ap.add_argument("--work-limit", type=float, default=None,
-                    help="work a query's join may do before it is reported unfinished (default: 5e9 on rmat22 "
-                         "and ooc24, 2e10 on rmat20)")
+    ap.add_argument("--first-attempt", type=float, default=2e9,
+                    help="work (rows read + list entries walked) of a query's first attempt in the batched path; "
+                         "attempt a may do first*2^a before the query starts again with its next ranked order "
+                         "(0 = no restarts)")
+    ap.add_argument("--work-limit", type=float, default=0,
+                    help="total work over a batched query's attempts after which it is reported unfinished "
+                         "(0 = none)")
+    ap.add_argument("--plain-work-limit", type=float, default=5e9,
+                    help="rows the plain path's steps may read per query before it is reported unfinished")
     ap.add_argument("--group", type=int, default=None,
                     help="queries resident at once in the batched path (0 = all; default: autotuned, else 0)")
     ap.add_argument("--no-tuning", action="store_true", help="ignore gem_amd/tuning/<workload>.json")
@@ -112,16 +136,17 @@ def main():
                     help="THRESHOLD,CHUNK: rows with longer driving lists are walked in CHUNK-entry pieces "
                          "(default: autotuned, else 1024,1024)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path pass")
-    ap.add_argument("--no-ablation", action="store_true", help="skip the pass with GEM-derived orders")
-    ap.add_argument("--plain-sample", type=int, default=64,
+    ap.add_argument("--no-ablation", action="store_true", help="skip the pass without the order model")
+    ap.add_argument("--no-profile-pass", action="store_true", help="skip the profiled pass (no roofline)")
+    ap.add_argument("--plain-sample", type=int, default=32,
                     help="run the plain path on a systematic sample of this many queries (every k-th; "
                          "0 = all queries)")
     ap.add_argument("--model", default=None,
                     help="trained order model (.npz); default: the one trained for the workload's graph "
-                         "(gem_amd/models/order_mlp_<graph>.npz); absent = GEM-derived orders")
+                         "(gem_amd/models/order_mlp_<graph>.npz); absent = candidates in generation order")
     ap.add_argument("--orders", type=int, default=None,
-                    help="candidate orders scored per query (default: autotuned, else 16)")
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-oracle work (0 = skip)")
+                    help="candidate orders ranked per query (default: autotuned, else 8)")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-oracle work (0 = skip)")
     ap.add_argument("--cpu-work-limit", type=float, default=2e9,
                     help="intermediate rows after which one oracle query is abandoned")
     ap.add_argument("--stream-cache", default=None,
@@ -147,15 +172,13 @@ def main():
     if args.queries:
         qs = qs[:args.queries]
     cap = int(args.cap) if args.workload != "er10k" else 0
-    if args.work_limit is None:
-        args.work_limit = {"rmat20": 2e10, "er10k": 0}.get(args.workload, 5e9)
     wl = int(args.work_limit)
+    first = int(args.first_attempt)
     log(f"[bench] {args.workload}: |V|={g.num_vertices} |E|={g.num_edges} queries={len(qs)} "
         f"dense={sum(q.dense for q in qs)} built in {time.perf_counter() - t0:.1f}s")
 
     ctx = GemContext(0)
     ctx.load_data(g)
-    ctx.set_work_limit(wl)
     # the autotuner's choices for this workload, unless given on the command line
     from gem_amd.autotune import apply, load_tuning
     tuning = {} if args.no_tuning else load_tuning(os.path.join(ROOT, "gem_amd", "tuning", f"{args.workload}.json"))
@@ -166,7 +189,7 @@ def main():
     if args.group is None:
         args.group = int(tuning.get("group", 0))
     if args.orders is None:
-        args.orders = int(tuning.get("max_orders", 16))
+        args.orders = int(tuning.get("max_orders", 8))
     level_budget = parse_bytes(args.level_budget) if args.level_budget else int(tuning.get("level_budget", 0))
     if tuning:
         log(f"[bench] autotuned: {tuning}")
@@ -184,35 +207,37 @@ def main():
         log(f"[bench] order model {args.model}: trained on {model.meta.get('workload')} "
             f"({model.meta.get('tuples')} tuples)")
     else:
-        log("[bench] no order model: batched path runs GEM-derived orders")
+        log("[bench] no order model: candidate orders run in generation order (GEM's first)")
 
-    def batched_pass():
-        return match_batch_learned(ctx, qs, model, max_orders=args.orders, cap=cap, work_limit=wl,
-                                   level_budget=level_budget,
-                                   group=args.group)
+    def batched_pass(m=model, queries=qs):
+        return match_batch_learned(ctx, queries, m, max_orders=args.orders, cap=cap, work_limit=wl,
+                                   level_budget=level_budget, group=args.group, first_attempt=first)
 
+    ctx.memory_stats(reset_peak=True)
     for w in range(args.warmup):
         t = time.perf_counter()
         batched_pass()
-        log(f"[bench] batched warmup pass {w}: {time.perf_counter() - t:.2f}s")
-    ctx.set_profiling(True)
+        log(f"[bench] batched warmup pass {w}: {time.perf_counter() - t:.2f}s; {mem_line(ctx)}")
     if stream_cache:
         ctx.stream_stats()  # reset
-    t = time.perf_counter()
+    # timed passes: no per-launch profiling (the extend step replays its hipGraph)
+    ctx.set_profiling(False)
+    pass_s, digests = [], []
     batched = None
     emb = 0
     for s in range(args.steps):
+        t = time.perf_counter()
         res = batched_pass()
+        pass_s.append(time.perf_counter() - t)
         emb += sum(r["count"] for r in res)
+        digests.append(digest(res))
         if batched is None:
             batched = res
-        log(f"[bench] batched timed pass {s} done at {time.perf_counter() - t:.2f}s")
-    elapsed = time.perf_counter() - t
-    stats = ctx.kernel_stats()
-    ctx.set_profiling(False)
+        log(f"[bench] batched timed pass {s}: {pass_s[-1]:.2f}s, digest {digests[-1]}, unfinished "
+            f"{sum(r['work_limited'] for r in res)}, restarts {sum(r['restarts'] for r in res)}; {mem_line(ctx)}")
+    elapsed = sum(pass_s)
     qps = len(qs) * args.steps / elapsed
     eps = emb / elapsed
-    log("[bench] batched kernels:", json.dumps(kernel_table(stats)))
 
     # streamed tables: copy statistics of the timed passes, and the same
     # batched pass with the tables in HBM for the counts and the cost
@@ -233,65 +258,85 @@ def main():
                   "blocks_loaded": ss["blocks_loaded"], "blocks_hit": ss["blocks_hit"], "groups": ss["groups"],
                   "serial_groups": ss["serial_groups"],
                   "in_hbm_queries_per_s": round(len(qs) / th, 4),
-                  "counts_equal_in_hbm": "{}/{} queries finished by both".format(
-                      sum(1 for a, b in zip(batched, res_h) if not a["work_limited"] and not b["work_limited"]
-                          and (a["count"], a["saturated"]) == (b["count"], b["saturated"])),
-                      sum(1 for a, b in zip(batched, res_h) if not a["work_limited"] and not b["work_limited"]))}
+                  "counts_equal_in_hbm": digest(res_h) == digests[0]}
         log(f"[bench] streamed tables: {stream}")
 
-    # ablation: the same batched path with GEM-derived orders, one pass
-    gem_batched = None
+    # one more pass with per-launch profiling: kernel table and roofline
+    stats, prof_s = {}, None
+    if not args.no_profile_pass:
+        ctx.set_profiling(True)
+        t = time.perf_counter()
+        res_p = batched_pass()
+        prof_s = time.perf_counter() - t
+        stats = ctx.kernel_stats()
+        ctx.set_profiling(False)
+        digests.append(digest(res_p))
+        log(f"[bench] profiled pass: {prof_s:.2f}s; kernels: {json.dumps(kernel_table(stats))}")
+
+    # ablation: the same batched path without the order model (candidates in
+    # generation order, GEM's first, same restart rule), one pass
+    no_model = None
     if model is not None and not args.no_ablation:
         t = time.perf_counter()
-        res_g = match_batch_learned(ctx, qs, None, cap=cap, work_limit=wl, level_budget=level_budget,
-                                    group=args.group)
+        res_g = batched_pass(m=None)
         tg = time.perf_counter() - t
-        gem_batched = {"queries_per_s": round(len(qs) / tg, 4), "seconds": round(tg, 3),
-                       "rows_processed": int(sum(r["rows_processed"] for r in res_g)),
-                       "rows_processed_learned": int(sum(r["rows_processed"] for r in batched)),
-                       "unfinished": sum(r["work_limited"] for r in res_g),
-                       "counts_equal": sum(1 for a, b in zip(res_g, batched)
-                                           if not a["work_limited"] and not b["work_limited"]
-                                           and (a["count"], a["saturated"]) == (b["count"], b["saturated"]))}
-        log(f"[bench] batched with GEM order: {gem_batched}")
+        no_model = {"queries_per_s": round(len(qs) / tg, 4), "seconds": round(tg, 3),
+                    "work": int(sum(r["work"] for r in res_g)),
+                    "work_learned": int(sum(r["work"] for r in batched)),
+                    "rows_processed": int(sum(r["rows_processed"] for r in res_g)),
+                    "rows_processed_learned": int(sum(r["rows_processed"] for r in batched)),
+                    "restarts": sum(r["restarts"] for r in res_g),
+                    "unfinished": sum(r["work_limited"] for r in res_g),
+                    "counts_equal": digest(res_g) == digests[0]}
+        log(f"[bench] batched without the order model: {no_model}")
 
-    # roofline: the kernel with the most time in the timed region
-    dom_name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
-    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else 0.0
-    traffic = profiled_traffic(dom_name, args.workload)
+    # roofline: the kernel with the most time in the profiled pass
+    roof = None
+    if stats:
+        dom_name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
+        achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else 0.0
+        prof_us = profiled("avg_launch_us", dom_name, args.workload)
+        roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic(dom_name, args.workload),
+                "algorithmic_bytes_per_launch": round(dom["bytes"] / max(dom["launches"], 1)),
+                "launches": dom["launches"], "avg_launch_ms": round(dom["ms"] / max(dom["launches"], 1), 3),
+                "profiled_avg_launch_ms": round(prof_us / 1000, 3) if prof_us is not None else None}
 
     # ---- plain path, one pass over a systematic sample (every k-th query)
     plain = None
+    n = len(qs)
+    k = max(1, -(-n // args.plain_sample)) if args.plain_sample else 1
+    idx = list(range(0, n, k))
     if not args.no_plain:
-        n = len(qs)
-        k = max(1, -(-n // args.plain_sample)) if args.plain_sample else 1
-        idx = list(range(0, n, k))
         sample = [qs[i] for i in idx]
+        ctx.set_work_limit(int(args.plain_work_limit))
         t = time.perf_counter()
         plain_res = []
         for j, q in enumerate(sample):
             plain_res.append(ctx.match_full(q, cap=cap))
-            if (j + 1) % 10 == 0:
+            if (j + 1) % 8 == 0:
                 log(f"[bench]   plain {j + 1}/{len(sample)} at {time.perf_counter() - t:.1f}s")
         pt = time.perf_counter() - t
+        ctx.set_work_limit(0)
         # the batched learned path over exactly the same queries, for a matched ratio
         t = time.perf_counter()
-        matched = match_batch_learned(ctx, sample, model, max_orders=args.orders, cap=cap, work_limit=wl,
-                                      level_budget=level_budget,
-                                      group=args.group)
+        batched_pass(queries=sample)
         mt = time.perf_counter() - t
-        both = [(batched[i], b) for i, b in zip(idx, plain_res)
-                if not batched[i]["work_limited"] and not b["work_limited"]]
+        both = [(batched[i], b) for i, b in zip(idx, plain_res) if not b["work_limited"]]
         agree = sum(1 for a, b in both if (a["count"], a["saturated"]) == (b["count"], b["saturated"]))
+        unf = sum(r["work_limited"] for r in plain_res)
         plain = {"queries_per_s": round(len(sample) / pt, 4), "embeddings_per_s": round(
             sum(r["count"] for r in plain_res) / pt, 1), "seconds": round(pt, 3),
                  "sample": f"every {k}-th query: {len(sample)} of {n}",
-                 "unfinished": sum(r["work_limited"] for r in plain_res),
+                 "work_limit_rows": int(args.plain_work_limit),
+                 "unfinished": unf,
+                 "rate_is": "an upper bound (unfinished queries stopped early)" if unf else "exact",
                  "batched_on_sample_queries_per_s": round(len(sample) / mt, 4),
-                 "counts_equal_to_batched": f"{agree}/{len(both)} queries finished by both"}
+                 "counts_equal_to_batched": f"{agree}/{len(both)} queries the plain path finished"}
         log(f"[bench] plain: {plain}")
 
-    # ---- CPU baseline: the oracle on a bounded prefix of the same queries
+    # ---- CPU baseline: the oracle on a bounded prefix of the same sample
     cpu = None
     if args.cpu_budget > 0:
         import numpy as np
@@ -302,10 +347,8 @@ def main():
         # budget is spent; the edge order is GEM's (from the oracle's own table
         # sizes, outside the timed call: the lattice form is exponential in the
         # edges of the larger queries)
-        n = len(qs)
-        k = max(1, -(-n // args.plain_sample)) if args.plain_sample else 1
         spent, done, abandoned, mism = 0.0, 0, 0, 0
-        for i in range(0, n, k):
+        for i in idx:
             q = qs[i]
             if spent >= args.cpu_budget:
                 break
@@ -319,7 +362,7 @@ def main():
                 continue
             done += 1
             gr = batched[i]
-            if not gr["work_limited"] and (r["count"], r["saturated"]) != (gr["count"], gr["saturated"]):
+            if (r["count"], r["saturated"]) != (gr["count"], gr["saturated"]):
                 mism += 1
         threads = oracle.max_threads()
         cpu = {"value": round(done / spent, 4) if spent > 0 else 0.0, "unit": "queries/s", "cores": threads,
@@ -330,6 +373,7 @@ def main():
                          f"equal to GPU on {done - mism}/{done}"}
         log(f"[bench] cpu oracle: {cpu}")
 
+    plain_q = plain["queries_per_s"] if plain else None
     out = {
         "metric": METRIC,
         "value": round(qps, 4),
@@ -345,22 +389,26 @@ def main():
         "data": "synthetic (seeded generators)",
         "config": {"workload": args.workload, "description": DESCR[args.workload], "queries": len(qs),
                    "path": "batched vertex-at-a-time join, " + (
-                       "learned order (MLP over candidate orders)" if model is not None else "GEM-derived order"),
-                   "cap": cap,
-                   "work_limit": wl,
+                       "learned order (MLP-ranked candidate orders)" if model is not None
+                       else "candidate orders in generation order"),
+                   "cap": cap, "first_attempt_work": first, "work_limit": wl,
                    "tuning": tuning or None},
         "embeddings_per_s": round(eps, 1),
+        "pass_seconds": [round(x, 3) for x in pass_s],
         "saturated_queries": sum(r["saturated"] for r in batched),
+        "exact_queries": sum(not r["saturated"] and not r["work_limited"] for r in batched),
         "unfinished_queries": sum(r["work_limited"] for r in batched),
+        "restarts": sum(r["restarts"] for r in batched),
+        "counts_digest": digests[0],
+        "digests_agree": len(set(digests)) == 1,
+        "no_model": no_model,
+        "learned_gain": round(no_model["seconds"] / (elapsed / args.steps), 3) if no_model else None,
         "plain": plain,
-        "batched_gem_order": gem_batched,
-        "speedup_vs_plain": round(qps / plain["queries_per_s"], 3) if plain else None,
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": round(dom["bytes"] / max(dom["launches"], 1)),
-                     "launches": dom["launches"], "avg_launch_ms": round(dom["ms"] / max(dom["launches"], 1), 3),
-                     "profiled_avg_launch_ms": (round(prof_us / 1000, 3) if (prof_us := profiled(
-                         "avg_launch_us", dom_name, args.workload)) is not None else None)},
+        "speedup_vs_plain": round(qps / plain_q, 3) if plain_q else None,
+        "speedup_is": ("a lower bound: the plain rate counts queries it stopped at its work limit"
+                       if plain and plain["unfinished"] else "exact") if plain else None,
+        "roofline": roof,
+        "profiled_pass_seconds": round(prof_s, 3) if prof_s is not None else None,
         "cpu_baseline": cpu,
         "streamed_tables": stream,
     }

@@ -32,7 +32,7 @@ class CsrView(C.Structure):
 class JoinResult(C.Structure):
     _fields_ = [("count", C.c_uint64), ("saturated", C.c_uint32), ("num_steps", C.c_uint32),
                 ("num_embeddings_written", C.c_uint64), ("rows_processed", C.c_uint64),
-                ("work_limited", C.c_uint32), ("reserved", C.c_uint32), ("work", C.c_uint64)]
+                ("work_limited", C.c_uint32), ("restarts", C.c_uint32), ("work", C.c_uint64)]
 
 
 class StepLog(C.Structure):
@@ -42,7 +42,8 @@ class StepLog(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("cap", C.c_uint64), ("work_limit", C.c_uint64), ("level_budget", C.c_uint64),
-                ("group", C.c_uint32), ("reserved", C.c_uint32)]
+                ("group", C.c_uint32), ("restart_growth", C.c_uint32), ("first_attempt", C.c_uint64),
+                ("restart_width", C.c_uint32), ("max_attempts", C.c_uint32)]
 
 
 class KernelStat(C.Structure):
@@ -92,6 +93,7 @@ SIGNATURES = [
     ("gem_hip_set_persistent_blocks", C.c_int, [C.c_void_p, C.c_uint32]),
     ("gem_hip_set_heavy_split", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     ("gem_hip_set_graph", C.c_int, [C.c_void_p, C.c_int]),
+    ("gem_hip_set_max_active", C.c_int, [C.c_void_p, C.c_uint32]),
     ("gem_hip_set_table_stream", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     ("gem_hip_stream_stats", C.c_int, [C.c_void_p, C.POINTER(StreamStats)]),
     ("gem_hip_load_data_csr", C.c_int, [C.c_void_p, C.POINTER(CsrView)]),

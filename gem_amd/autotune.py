@@ -53,6 +53,8 @@ def apply(ctx, cfg):
         ctx.set_persistent_blocks(cfg["persistent_blocks"])
     if "heavy_t" in cfg:
         ctx.set_heavy_split(cfg["heavy_t"], cfg.get("heavy_chunk", 1024))
+    if "max_active" in cfg:
+        ctx.set_max_active(cfg["max_active"])
 
 
 def main():

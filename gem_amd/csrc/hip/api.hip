@@ -94,6 +94,12 @@ int gem_hip_set_heavy_split(gem_ctx *c, uint32_t threshold, uint32_t chunk) {
   return GEM_OK;
 }
 
+int gem_hip_set_max_active(gem_ctx *c, uint32_t queries) {
+  if (!c) return GEM_E_INVALID;
+  c->max_active = queries;
+  return GEM_OK;
+}
+
 int gem_hip_set_graph(gem_ctx *c, int on) {
   if (!c) return GEM_E_INVALID;
   c->use_graph = on != 0;
@@ -241,7 +247,7 @@ int gem_hip_join(gem_ctx *c, const uint32_t *edge_order, uint64_t cap, gem_join_
     out->num_embeddings_written = jo.emb_written;
     out->rows_processed = jo.rows_processed;
     out->work_limited = jo.work_limited ? 1 : 0;
-    out->reserved = 0;
+    out->restarts = 0;
     out->work = jo.rows_processed;
   });
 }
@@ -349,7 +355,7 @@ int gem_hip_match(gem_ctx *c, const gem_query_view *q, uint64_t cap, gem_join_re
     out->num_embeddings_written = 0;
     out->rows_processed = jo.rows_processed;
     out->work_limited = jo.work_limited ? 1 : 0;
-    out->reserved = 0;
+    out->restarts = 0;
     out->work = jo.rows_processed;
   });
 }

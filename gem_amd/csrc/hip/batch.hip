@@ -51,10 +51,22 @@ static void plan_query(gem_ctx *c, VQueryState &S, const QueryDev &Qd, const uin
     vorder.assign(vorder_in, vorder_in + Q.nv);
   else
     vorder = gem_vertex_order(Qd);
-  if (gemh::make_vertex_plan(Q, vorder, &S.plan))
-    throw Error{GEM_E_INVALID, "invalid vertex order (leaf first, or a vertex without an earlier neighbour)"};
   S.qd = &Qd;
-  build_vlevels(c, S);
+  replan(c, S, vorder);
+}
+
+// Seed of a query's candidate orders: a hash of its labels and edges, so the
+// candidates do not depend on where the query sits in a batch.
+static uint64_t query_seed(const gemh::QueryInfo &Q) {
+  uint64_t h = 0x9e3779b97f4a7c15ull ^ ((uint64_t)Q.nv << 32) ^ Q.ne;
+  auto mix = [&](uint64_t x) {
+    h ^= x + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+  };
+  for (uint32_t u = 0; u < Q.nv; ++u) mix(Q.lab[u]);
+  for (const auto &e : Q.edges) mix(((uint64_t)e.first << 32) | e.second);
+  return h;
 }
 
 static void fill_result(const VQueryState &V, uint64_t cap, gem_join_result *r, uint64_t *level_rows) {
@@ -65,7 +77,7 @@ static void fill_result(const VQueryState &V, uint64_t cap, gem_join_result *r, 
   r->num_embeddings_written = 0;
   r->rows_processed = S.processed;
   r->work_limited = S.work_limited ? 1 : 0;
-  r->reserved = 0;
+  r->restarts = S.attempts;
   r->work = S.work;
   if (level_rows)
     for (uint32_t i = 0; i < GEM_MAX_QV; ++i) level_rows[i] = i < S.level_rows.size() ? S.level_rows[i] : 0;
@@ -157,8 +169,9 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
       t_filter += secs(ta, tb);
       t_tables += secs(tb, tc);
       const gemh::QueryStats qsx = query_stats(c, qd[i]);
-      gemh::candidate_orders(qd[i].Q, qsx, gem_vertex_order(qd[i]), mlp ? max_orders : 1, 0x9e3779b9ull + g0 + i,
-                             &cand[i]);
+      // without a model, candidates beyond GEM's order matter only to restarts
+      const uint32_t norders = (mlp || o.first_attempt) ? max_orders : 1;
+      gemh::candidate_orders(qd[i].Q, qsx, gem_vertex_order(qd[i]), norders, query_seed(qd[i].Q), &cand[i]);
       if (mlp) {
         for (auto &vo : cand[i]) {
           const size_t off = feats.size();
@@ -169,30 +182,58 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
       }
       t_orders += secs(tc, clk::now());
     }
-    std::vector<uint32_t> pick(gn, 0);
+    // Ranking of each query's candidates: without a model, as generated (GEM's
+    // order first); with one, by predicted cost, except that GEM's order
+    // (candidate 0) stays first unless the cheapest prediction beats it by
+    // more than the model's margin.  The first ranked order runs; restarts
+    // take the next ones.
+    std::vector<std::vector<uint32_t>> rank(gn);
+    for (uint32_t i = 0; i < gn; ++i) {
+      rank[i].resize(cand[i].size());
+      for (uint32_t j = 0; j < cand[i].size(); ++j) rank[i][j] = j;
+    }
     if (mlp && !owner.empty()) {
       std::vector<float> pred(owner.size());
       const int rc = gem_mlp_predict(mlp, feats.data(), (uint32_t)owner.size(), pred.data());
       if (rc) throw Error{rc, "order model: " + c->err};
-      // the cheapest prediction wins if it beats GEM's order (candidate 0)
-      // by more than the model's margin; otherwise GEM's order runs
       const float margin = mlp_margin(mlp);
       size_t k = 0;
       for (uint32_t i = 0; i < gn; ++i) {
-        const float gem = pred[k];
-        float best = gem;
-        for (uint32_t j = 0; j < cand[i].size(); ++j, ++k)
-          if (pred[k] < best) {
-            best = pred[k];
-            pick[i] = j;
-          }
-        if (!(best < gem - margin)) pick[i] = 0;
+        const float *p = pred.data() + k;
+        k += cand[i].size();
+        std::vector<uint32_t> &R = rank[i];
+        std::stable_sort(R.begin(), R.end(), [&](uint32_t a, uint32_t b) { return p[a] < p[b]; });
+        if (!(p[R[0]] < p[0] - margin)) {
+          R.erase(std::find(R.begin(), R.end(), 0u));
+          R.insert(R.begin(), 0u);
+        }
       }
     }
     for (uint32_t i = 0; i < gn; ++i) {
-      plan_query(c, st[i], qd[i], cand[i][pick[i]].data());
-      ptrs.push_back(&st[i]);
-      if (chosen) chosen[g0 + i] = pick[i];
+      VQueryState &V = st[i];
+      V.qd = &qd[i];
+      V.first_attempt = o.first_attempt;
+      V.growth = o.restart_growth ? o.restart_growth : 4;
+      V.width = o.restart_width ? o.restart_width : 1;
+      V.max_attempts = o.max_attempts;
+      for (uint32_t j : rank[i]) {
+        V.ranked.push_back(cand[i][j]);
+        V.ranked_idx.push_back(j);
+      }
+      // size the launch buffers for every order a restart may run
+      if (attempt_limit(V, 0))
+        for (size_t j = 1; j < V.ranked.size(); ++j) {
+          gemh::VertexPlan P;
+          if (gemh::make_vertex_plan(qd[i].Q, V.ranked[j], &P))
+            throw Error{GEM_E_STATE, "invalid candidate vertex order"};
+          for (size_t l = 1; l < P.levels.size(); ++l) {
+            V.mnb_all = std::max(V.mnb_all, P.levels[l].nb);
+            V.mnd_all = std::max(V.mnd_all, P.levels[l].ndup);
+          }
+        }
+      replan(c, V, V.ranked[0]);
+      V.cur = 0;
+      ptrs.push_back(&V);
     }
     const char *verbose_env = std::getenv("GEM_VERBOSE");
     if (verbose_env && verbose_env[0] == '1') {
@@ -205,8 +246,10 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
     }
     VBatchRunner runner(c, o.cap, o.work_limit, budget, streamed);
     runner.run(ptrs);
-    for (uint32_t i = 0; i < gn; ++i)
+    for (uint32_t i = 0; i < gn; ++i) {
       fill_result(st[i], o.cap, &results[g0 + i], level_rows ? level_rows + (uint64_t)(g0 + i) * GEM_MAX_QV : nullptr);
+      if (chosen) chosen[g0 + i] = st[i].ranked_idx[st[i].cur];
+    }
   }
   GEM_HIP_CHECK(hipEventRecord(c->ev[7], s));
   GEM_HIP_CHECK(hipStreamSynchronize(s));

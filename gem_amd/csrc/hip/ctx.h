@@ -261,6 +261,7 @@ struct gem_ctx {
   uint64_t work_limit_rows = 0;             // 0 = none
   uint32_t persistent_blocks = 2048;        // grid of the batched path's persistent kernels
   uint32_t heavy_t = 1024, heavy_chunk = 1024;  // batched walks: long driving lists split over waves
+  uint32_t max_active = 64;                 // batched path: queries running at once (0 = all)
   bool use_graph = false;                   // replay the batched extend step as a hipGraph
   // batched path with streamed edge tables: values in pinned host memory,
   // a device cache of stream_cache_bytes in slots of stream_slot_bytes

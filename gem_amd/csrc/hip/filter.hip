@@ -261,22 +261,39 @@ __global__ void k_position_lookup(const uint64_t *__restrict__ bitmap, const uin
   pos[i] = p;
 }
 
-void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T) { run_filter(c, c->q, qv, T); }
+// Device buffers of one filter run.
+struct FilterBufs {
+  FilterQuery *fq = nullptr;
+  uint8_t *slot_of = nullptr;
+  uint64_t *bitmap = nullptr;      // u64[nv * W]
+  uint32_t *tile_count = nullptr;  // u32[nv * W]
+  uint32_t *rank = nullptr;        // u32[nv * (W + 1)]
+  uint32_t *can = nullptr;         // candidate lists at can_off[u]
+  const uint64_t *can_off = nullptr;
+  uint32_t *heavy_list = nullptr;  // u32[n]
+  uint32_t *heavy_n = nullptr;     // u32
+  uint32_t *hub_list = nullptr;    // u32[hub_cap]
+  uint32_t *hub_n = nullptr;       // u32
+  uint32_t *hub_cnt = nullptr;     // u32[hub_cap * GEM_MAX_QV]
+  uint32_t hub_cap = 1;
+};
 
-void run_filter(gem_ctx *c, QueryDev &Qd, const gem_query_view *qv, uint32_t T) {
-  if (!c->g.loaded) throw Error{GEM_E_STATE, "no data graph loaded"};
-  Qd = QueryDev();
-  std::string why;
-  int st = gemh::analyse_query(qv, &Qd.Q, &why);
-  if (st) throw Error{st, why};
-  const gemh::QueryInfo &Q = Qd.Q;
-  const GraphDev &G = c->g;
-  const uint32_t nv = Q.nv;
-  const uint32_t W = (G.n + 63) / 64;
-  Qd.W = W;
-
-  // query label slots in first-seen order (in-core/lib/io/src/io.cu:135-161)
+// Host side of a filter run: the query's label slots and per-vertex label
+// requirements, the dense-label -> slot map, the candidate-list offsets (upper
+// bound per list = frequency of the vertex's label) and the adjacency slots of
+// label-matching vertices (for the algorithmic bytes).
+struct FilterPrep {
   FilterQuery fq{};
+  std::vector<uint8_t> slot_of;
+  std::vector<uint64_t> can_off;
+  double adj = 0;
+};
+
+static void filter_prepare(const GraphDev &G, const gemh::QueryInfo &Q, uint32_t T, uint32_t W, FilterPrep *P) {
+  const uint32_t nv = Q.nv;
+  // query label slots in first-seen order (in-core/lib/io/src/io.cu:135-161)
+  FilterQuery &fq = P->fq;
+  fq = FilterQuery{};
   fq.nv = nv;
   fq.T = T;
   fq.W = W;
@@ -303,25 +320,130 @@ void run_filter(gem_ctx *c, QueryDev &Qd, const gem_query_view *qv, uint32_t T) 
   }
   // dense data label -> query slot
   const uint32_t nlab = (uint32_t)G.label_freq.size();
-  std::vector<uint8_t> slot_of(std::max<uint32_t>(nlab, 1), (uint8_t)NO_SLOT);
+  P->slot_of.assign(std::max<uint32_t>(nlab, 1), (uint8_t)NO_SLOT);
   for (uint32_t s = 0; s < fq.nslots; ++s) {
     auto it = G.label_id.find(slot_label[s]);
-    if (it != G.label_id.end()) slot_of[it->second] = (uint8_t)s;
+    if (it != G.label_id.end()) P->slot_of[it->second] = (uint8_t)s;
   }
   // candidate storage: upper bound = frequency of u's label
-  Qd.can_off.assign(nv + 1, 0);
+  P->can_off.assign(nv + 1, 0);
   for (uint32_t u = 0; u < nv; ++u) {
     auto it = G.label_id.find(Q.lab[u]);
     uint64_t ub = it == G.label_id.end() ? 0 : G.label_freq[it->second];
-    Qd.can_off[u + 1] = Qd.can_off[u] + ub;
+    P->can_off[u + 1] = P->can_off[u] + ub;
   }
+  P->adj = 0;
+  for (uint32_t sl = 0; sl < fq.nslots; ++sl) {
+    auto it = G.label_id.find(slot_label[sl]);
+    if (it != G.label_id.end()) P->adj += (double)G.label_deg[it->second];
+  }
+}
+
+// |Can(u)| = rank[u][W] for every u.
+__global__ void k_gather_counts(const uint32_t *__restrict__ rank, uint32_t nv, uint32_t W, uint32_t *__restrict__ out) {
+  for (uint32_t u = threadIdx.x; u < nv; u += blockDim.x) out[u] = rank[(uint64_t)u * (W + 1) + W];
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// The filter's kernels on the context stream: per-tile bitmaps and counts
+// (light vertices in k_filter, heavy ones in k_filter_heavy, hubs in chunks),
+// the rank scan, and the compaction of Can(u).  No allocation; the counters
+// are cleared by a kernel (no memset node when a caller captures the stream).
+static void filter_launch(gem_ctx *c, uint32_t nv, uint32_t W, const FilterBufs &B) {
+  const GraphDev &G = c->g;
+  hipStream_t s = c->stream;
+  k_zero_u32<<<1, 64, 0, s>>>(B.heavy_n, 1);
+  k_zero_u32<<<1, 64, 0, s>>>(B.hub_n, 1);
+  if (G.n_hub)
+    k_zero_u32<<<(uint32_t)std::min<uint64_t>(((uint64_t)B.hub_cap * GEM_MAX_QV + 255) / 256, 1024), 256, 0, s>>>(
+        B.hub_cnt, (uint64_t)B.hub_cap * GEM_MAX_QV);
+  GEM_LAUNCH_CHECK("k_zero_u32");
+  const uint32_t blocks = (W + FILTER_WAVES - 1) / FILTER_WAVES;
+  {
+    ProfLaunch pl(c, K_FILTER);
+    if (G.lab8)
+      k_filter<uint8_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+                                                             G.lab.as<uint8_t>(), G.n, B.fq, B.slot_of, B.bitmap,
+                                                             B.tile_count, B.heavy_list, B.heavy_n);
+    else
+      k_filter<uint32_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+                                                              G.lab.as<uint32_t>(), G.n, B.fq, B.slot_of, B.bitmap,
+                                                              B.tile_count, B.heavy_list, B.heavy_n);
+    GEM_LAUNCH_CHECK("k_filter");
+  }
+  const uint32_t maxc = (uint32_t)((G.max_deg + FILTER_HUB_DEG - 1) / FILTER_HUB_DEG);
+  auto *bm = reinterpret_cast<unsigned long long *>(B.bitmap);
+  {
+    ProfLaunch pl(c, K_FILTER_HEAVY);
+    const uint32_t hb = 1024;  // persistent: 4096 waves walk the heavy list
+    if (G.lab8)
+      k_filter_heavy<uint8_t><<<hb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+                                                               G.lab.as<uint8_t>(), B.fq, B.slot_of, bm, B.tile_count,
+                                                               B.heavy_list, B.heavy_n, B.hub_list, B.hub_n,
+                                                               B.hub_cap);
+    else
+      k_filter_heavy<uint32_t><<<hb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+                                                                G.lab.as<uint32_t>(), B.fq, B.slot_of, bm,
+                                                                B.tile_count, B.heavy_list, B.heavy_n, B.hub_list,
+                                                                B.hub_n, B.hub_cap);
+    GEM_LAUNCH_CHECK("k_filter_heavy");
+    if (G.n_hub) {
+      const uint32_t tb = (uint32_t)std::min<uint64_t>(((uint64_t)G.n_hub * maxc + FILTER_WAVES - 1) / FILTER_WAVES,
+                                                       2048);
+      const uint32_t db = (G.n_hub + FILTER_WAVES - 1) / FILTER_WAVES;
+      if (G.lab8) {
+        k_filter_hub<uint8_t><<<tb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+                                                               G.lab.as<uint8_t>(), B.fq, B.slot_of, B.hub_list,
+                                                               B.hub_n, B.hub_cap, maxc, B.hub_cnt);
+        k_filter_hub_decide<uint8_t><<<db, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.lab.as<uint8_t>(), B.fq,
+                                                                      B.slot_of, B.hub_list, B.hub_n, B.hub_cap,
+                                                                      B.hub_cnt, bm, B.tile_count);
+      } else {
+        k_filter_hub<uint32_t><<<tb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+                                                                G.lab.as<uint32_t>(), B.fq, B.slot_of, B.hub_list,
+                                                                B.hub_n, B.hub_cap, maxc, B.hub_cnt);
+        k_filter_hub_decide<uint32_t><<<db, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.lab.as<uint32_t>(),
+                                                                       B.fq, B.slot_of, B.hub_list, B.hub_n,
+                                                                       B.hub_cap, B.hub_cnt, bm, B.tile_count);
+      }
+      GEM_LAUNCH_CHECK("k_filter_hub");
+    }
+  }
+  scan_segments_u32(c, B.tile_count, B.rank, nv, W);
+  if (B.can) {
+    ProfLaunch pl(c, K_COMPACT);
+    k_compact<<<(W + 3) / 4, 256, 0, s>>>(B.bitmap, B.rank, nv, W, B.can_off, B.can);
+    GEM_LAUNCH_CHECK("k_compact");
+  }
+}
+
+void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T) { run_filter(c, c->q, qv, T); }
+
+void run_filter(gem_ctx *c, QueryDev &Qd, const gem_query_view *qv, uint32_t T) {
+  if (!c->g.loaded) throw Error{GEM_E_STATE, "no data graph loaded"};
+  Qd = QueryDev();
+  std::string why;
+  int st = gemh::analyse_query(qv, &Qd.Q, &why);
+  if (st) throw Error{st, why};
+  const gemh::QueryInfo &Q = Qd.Q;
+  const GraphDev &G = c->g;
+  const uint32_t nv = Q.nv;
+  const uint32_t W = (G.n + 63) / 64;
+  Qd.W = W;
+  FilterPrep P;
+  filter_prepare(G, Q, T, W, &P);
+  Qd.can_off = P.can_off;
 
   hipStream_t s = c->stream;
   GEM_HIP_CHECK(hipEventRecord(c->ev[0], s));
   Qd.fq.alloc(sizeof(FilterQuery), s);
-  Qd.slot_of.alloc(slot_of.size(), s);
-  GEM_HIP_CHECK(hipMemcpyAsync(Qd.fq.p, &fq, sizeof(fq), hipMemcpyHostToDevice, s));
-  GEM_HIP_CHECK(hipMemcpyAsync(Qd.slot_of.p, slot_of.data(), slot_of.size(), hipMemcpyHostToDevice, s));
+  Qd.slot_of.alloc(P.slot_of.size(), s);
+  GEM_HIP_CHECK(hipMemcpyAsync(Qd.fq.p, &P.fq, sizeof(FilterQuery), hipMemcpyHostToDevice, s));
+  GEM_HIP_CHECK(hipMemcpyAsync(Qd.slot_of.p, P.slot_of.data(), P.slot_of.size(), hipMemcpyHostToDevice, s));
   Qd.bitmap.alloc((uint64_t)nv * W * 8, s);
   DevBuf tile_count;
   tile_count.alloc((uint64_t)nv * W * 4, s);
@@ -330,96 +452,39 @@ void run_filter(gem_ctx *c, QueryDev &Qd, const gem_query_view *qv, uint32_t T) 
   DevBuf can_off;
   can_off.alloc((nv + 1) * 8, s);
   GEM_HIP_CHECK(hipMemcpyAsync(can_off.p, Qd.can_off.data(), (nv + 1) * 8, hipMemcpyHostToDevice, s));
-
-  const uint32_t blocks = (W + FILTER_WAVES - 1) / FILTER_WAVES;
-  DevBuf heavy_list, heavy_n;
+  DevBuf heavy_list, heavy_n, hub_list, hub_n, hub_cnt;
   heavy_list.alloc(std::max<uint64_t>(G.n, 1) * 4, s);
   heavy_n.alloc(4, s);
-  GEM_HIP_CHECK(hipMemsetAsync(heavy_n.p, 0, 4, s));
-  {
-    ProfLaunch pl(c, K_FILTER);
-    if (G.lab8)
-      k_filter<uint8_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(
-          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint8_t>(), G.n, Qd.fq.as<FilterQuery>(),
-          Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<uint64_t>(), tile_count.as<uint32_t>(), heavy_list.as<uint32_t>(),
-          heavy_n.as<uint32_t>());
-    else
-      k_filter<uint32_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(
-          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint32_t>(), G.n, Qd.fq.as<FilterQuery>(),
-          Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<uint64_t>(), tile_count.as<uint32_t>(), heavy_list.as<uint32_t>(),
-          heavy_n.as<uint32_t>());
-    GEM_LAUNCH_CHECK("k_filter");
-  }
   const uint32_t hub_cap = std::max<uint32_t>(G.n_hub, 1);
-  const uint32_t maxc = (uint32_t)((G.max_deg + FILTER_HUB_DEG - 1) / FILTER_HUB_DEG);
-  DevBuf hub_list, hub_n, hub_cnt;
   hub_list.alloc((uint64_t)hub_cap * 4, s);
   hub_n.alloc(4, s);
   hub_cnt.alloc((uint64_t)hub_cap * GEM_MAX_QV * 4, s);
-  GEM_HIP_CHECK(hipMemsetAsync(hub_n.p, 0, 4, s));
-  if (G.n_hub) GEM_HIP_CHECK(hipMemsetAsync(hub_cnt.p, 0, (uint64_t)hub_cap * GEM_MAX_QV * 4, s));
-  {
-    ProfLaunch pl(c, K_FILTER_HEAVY);
-    const uint32_t hb = 1024;  // persistent: 4096 waves walk the heavy list
-    if (G.lab8)
-      k_filter_heavy<uint8_t><<<hb, FILTER_WAVES * 64, 0, s>>>(
-          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint8_t>(), Qd.fq.as<FilterQuery>(),
-          Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<unsigned long long>(), tile_count.as<uint32_t>(),
-          heavy_list.as<uint32_t>(), heavy_n.as<uint32_t>(), hub_list.as<uint32_t>(), hub_n.as<uint32_t>(),
-          hub_cap);
-    else
-      k_filter_heavy<uint32_t><<<hb, FILTER_WAVES * 64, 0, s>>>(
-          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint32_t>(), Qd.fq.as<FilterQuery>(),
-          Qd.slot_of.as<uint8_t>(), Qd.bitmap.as<unsigned long long>(), tile_count.as<uint32_t>(),
-          heavy_list.as<uint32_t>(), heavy_n.as<uint32_t>(), hub_list.as<uint32_t>(), hub_n.as<uint32_t>(),
-          hub_cap);
-    GEM_LAUNCH_CHECK("k_filter_heavy");
-    if (G.n_hub) {
-      const uint32_t tb = (uint32_t)std::min<uint64_t>(((uint64_t)G.n_hub * maxc + FILTER_WAVES - 1) / FILTER_WAVES,
-                                                       2048);
-      const uint32_t db = (G.n_hub + FILTER_WAVES - 1) / FILTER_WAVES;
-      if (G.lab8) {
-        k_filter_hub<uint8_t><<<tb, FILTER_WAVES * 64, 0, s>>>(
-            G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint8_t>(), Qd.fq.as<FilterQuery>(),
-            Qd.slot_of.as<uint8_t>(), hub_list.as<uint32_t>(), hub_n.as<uint32_t>(), hub_cap, maxc,
-            hub_cnt.as<uint32_t>());
-        k_filter_hub_decide<uint8_t><<<db, FILTER_WAVES * 64, 0, s>>>(
-            G.off.as<uint64_t>(), G.lab.as<uint8_t>(), Qd.fq.as<FilterQuery>(), Qd.slot_of.as<uint8_t>(),
-            hub_list.as<uint32_t>(), hub_n.as<uint32_t>(), hub_cap, hub_cnt.as<uint32_t>(),
-            Qd.bitmap.as<unsigned long long>(), tile_count.as<uint32_t>());
-      } else {
-        k_filter_hub<uint32_t><<<tb, FILTER_WAVES * 64, 0, s>>>(
-            G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), G.lab.as<uint32_t>(), Qd.fq.as<FilterQuery>(),
-            Qd.slot_of.as<uint8_t>(), hub_list.as<uint32_t>(), hub_n.as<uint32_t>(), hub_cap, maxc,
-            hub_cnt.as<uint32_t>());
-        k_filter_hub_decide<uint32_t><<<db, FILTER_WAVES * 64, 0, s>>>(
-            G.off.as<uint64_t>(), G.lab.as<uint32_t>(), Qd.fq.as<FilterQuery>(), Qd.slot_of.as<uint8_t>(),
-            hub_list.as<uint32_t>(), hub_n.as<uint32_t>(), hub_cap, hub_cnt.as<uint32_t>(),
-            Qd.bitmap.as<unsigned long long>(), tile_count.as<uint32_t>());
-      }
-      GEM_LAUNCH_CHECK("k_filter_hub");
-    }
-  }
+  FilterBufs B;
+  B.fq = Qd.fq.as<FilterQuery>();
+  B.slot_of = Qd.slot_of.as<uint8_t>();
+  B.bitmap = Qd.bitmap.as<uint64_t>();
+  B.tile_count = tile_count.as<uint32_t>();
+  B.rank = Qd.rank.as<uint32_t>();
+  B.can = Qd.can.as<uint32_t>();
+  B.can_off = can_off.as<uint64_t>();
+  B.heavy_list = heavy_list.as<uint32_t>();
+  B.heavy_n = heavy_n.as<uint32_t>();
+  B.hub_list = hub_list.as<uint32_t>();
+  B.hub_n = hub_n.as<uint32_t>();
+  B.hub_cnt = hub_cnt.as<uint32_t>();
+  B.hub_cap = hub_cap;
+  filter_launch(c, nv, W, B);
   // algorithmic bytes: 16 B per vertex (label, degree, offset pair), 8 B per
   // adjacency slot of a label-matching vertex (neighbour id + its label), the
   // candidate bitmaps and per-tile counts written
-  double adj = 0;
-  for (uint32_t sl = 0; sl < fq.nslots; ++sl) {
-    auto it = G.label_id.find(slot_label[sl]);
-    if (it != G.label_id.end()) adj += (double)G.label_deg[it->second];
-  }
-  prof_bytes(c, K_FILTER, 16.0 * G.n + 8.0 * adj + nv * (double)W * 8 + nv * (double)W * 4);
-  scan_segments_u32(c, tile_count.as<uint32_t>(), Qd.rank.as<uint32_t>(), nv, W);
-  {
-    ProfLaunch pl(c, K_COMPACT);
-    k_compact<<<(W + 3) / 4, 256, 0, s>>>(Qd.bitmap.as<uint64_t>(), Qd.rank.as<uint32_t>(), nv, W,
-                                          can_off.as<uint64_t>(), Qd.can.as<uint32_t>());
-    GEM_LAUNCH_CHECK("k_compact");
-  }
-  // |Can(u)| = rank[u][W]
+  prof_bytes(c, K_FILTER, 16.0 * G.n + 8.0 * P.adj + nv * (double)W * 8 + nv * (double)W * 4);
+  // |Can(u)| = rank[u][W], gathered by one kernel and copied back at once
+  DevBuf dn;
+  dn.alloc(4ull * nv, s);
+  k_gather_counts<<<1, 64, 0, s>>>(Qd.rank.as<uint32_t>(), nv, W, dn.as<uint32_t>());
+  GEM_LAUNCH_CHECK("k_gather_counts");
   uint32_t *h = pinned<uint32_t>(c, nv);
-  for (uint32_t u = 0; u < nv; ++u)
-    GEM_HIP_CHECK(hipMemcpyAsync(h + u, Qd.rank.as<uint32_t>() + (uint64_t)u * (W + 1) + W, 4, hipMemcpyDeviceToHost, s));
+  GEM_HIP_CHECK(hipMemcpyAsync(h, dn.p, 4ull * nv, hipMemcpyDeviceToHost, s));
   GEM_HIP_CHECK(hipEventRecord(c->ev[1], s));
   GEM_HIP_CHECK(hipStreamSynchronize(s));
   Qd.ncan.assign(h, h + nv);

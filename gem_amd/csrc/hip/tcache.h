@@ -55,6 +55,9 @@ class TableCache {
   // Assign global block ids to the queries' tables, size the two slot pools,
   // then patch and upload the level descriptors (vmap, kshift, gblk).
   void attach(std::vector<VQueryState *> &qs);
+  // Point query i's level descriptors (its current plan) at the block map
+  // and upload them (after attach, and after a restart replanned it).
+  void bind(VQueryState &S, size_t i);
 
   uint32_t *d_first() const { return first_.as<uint32_t>(); }
   uint32_t *d_last() const { return last_.as<uint32_t>(); }
@@ -79,6 +82,7 @@ class TableCache {
   void finish();
 
  private:
+  std::vector<std::vector<uint64_t>> base_;  // first global block of each (query, directed table)
   gem_ctx *c_;
   uint64_t cache_bytes_;
   uint64_t sbytes_[2];      // slot bytes of the small and the large class

@@ -121,7 +121,8 @@ void TableCache::attach(std::vector<VQueryState *> &qs) {
   hipStream_t s = c_->stream;
   // global block ids, table by table; blocks keep the device-visible address
   // of their values in pinned host memory
-  std::vector<std::vector<uint64_t>> base(qs.size());
+  std::vector<std::vector<uint64_t>> &base = base_;
+  base.assign(qs.size(), {});
   nblk_ = 0;
   for (size_t i = 0; i < qs.size(); ++i) {
     const QueryDev &Qd = *qs[i]->qd;
@@ -189,23 +190,26 @@ void TableCache::attach(std::vector<VQueryState *> &qs) {
   for (uint64_t b = 0; b < nblk_; ++b) hv[b] = guard_addr() - 4 * boff_[b];
   GEM_HIP_CHECK(hipMemcpyAsync(vmap_.p, hv.data(), 8 * n, hipMemcpyHostToDevice, s));
   // level descriptors read the map
-  for (size_t i = 0; i < qs.size(); ++i) {
-    VQueryState &S = *qs[i];
-    const QueryDev &Qd = *S.qd;
-    for (uint32_t lv = 1; lv < Qd.Q.nv; ++lv) {
-      const gemh::VLevelPlan &lp = S.plan.levels[lv];
-      VLevel &L = S.hlv[lv];
-      for (uint32_t j = 0; j < lp.nb; ++j) {
-        const uint32_t d = lp.directed[j];
-        L.vmap[j] = vmap_.as<uint64_t>() + base[i][d];
-        L.kshift[j] = Qd.tab[d].kshift;
-        L.gblk[j] = (uint32_t)base[i][d];
-      }
-    }
-    GEM_HIP_CHECK(hipMemcpyAsync(S.levels.p, S.hlv.data(), sizeof(VLevel) * S.hlv.size(), hipMemcpyHostToDevice, s));
-  }
+  for (size_t i = 0; i < qs.size(); ++i) bind(*qs[i], i);
   // the copy stream uses buffers allocated on the context stream
   GEM_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void TableCache::bind(VQueryState &S, size_t i) {
+  const QueryDev &Qd = *S.qd;
+  if (i >= base_.size()) throw Error{GEM_E_STATE, "table cache: query not attached"};
+  for (uint32_t lv = 1; lv < Qd.Q.nv; ++lv) {
+    const gemh::VLevelPlan &lp = S.plan.levels[lv];
+    VLevel &L = S.hlv[lv];
+    for (uint32_t j = 0; j < lp.nb; ++j) {
+      const uint32_t d = lp.directed[j];
+      L.vmap[j] = vmap_.as<uint64_t>() + base_[i][d];
+      L.kshift[j] = Qd.tab[d].kshift;
+      L.gblk[j] = (uint32_t)base_[i][d];
+    }
+  }
+  GEM_HIP_CHECK(hipMemcpyAsync(S.levels.p, S.hlv.data(), sizeof(VLevel) * S.hlv.size(), hipMemcpyHostToDevice,
+                               c_->stream));
 }
 
 void TableCache::reset_marks() {

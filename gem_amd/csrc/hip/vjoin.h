@@ -53,9 +53,27 @@ struct VQueryState {
   std::vector<uint32_t> max_nb, max_ndup;  // per level
   // depth-first stack of pending level ranges, counts and limits (host/vsched.h)
   gemh::SQuery<DevBuf> s;
+  // Restarts (learned batched path): the vertex orders in the order they are
+  // tried and their candidate indices; ranked[cur] is the one running.
+  // Attempt a runs ranked[a % size] and may add
+  // first_attempt * growth^(a / width) work (first_attempt == 0: no restarts;
+  // attempt max_attempts - 1, when max_attempts != 0, has no limit).
+  // mnb_all / mnd_all: the most backward lists and equal-label columns of a
+  // level over every ranked order (the launch buffers are sized for them).
+  std::vector<std::vector<uint32_t>> ranked;
+  std::vector<uint32_t> ranked_idx;
+  uint32_t cur = 0;
+  uint64_t first_attempt = 0;
+  uint32_t growth = 4, width = 1;
+  uint32_t max_attempts = 0;
+  uint32_t mnb_all = 1, mnd_all = 0;
 };
 
 void build_vlevels(gem_ctx *c, VQueryState &S);
+// The vertex plan of `vorder` for S (S.qd set) and its device level table.
+void replan(gem_ctx *c, VQueryState &S, const std::vector<uint32_t> &vorder);
+// Work limit of attempt a (0 = none).
+uint64_t attempt_limit(const VQueryState &S, uint32_t a);
 
 class VBatchRunner {
  public:

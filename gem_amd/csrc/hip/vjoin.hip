@@ -29,6 +29,7 @@
 // shared by all queries; a query stops as soon as its count exceeds the cap.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -563,6 +564,51 @@ void build_vlevels(gem_ctx *c, VQueryState &S) {
   S.hlv = std::move(lv);
 }
 
+void replan(gem_ctx *c, VQueryState &S, const std::vector<uint32_t> &vorder) {
+  if (gemh::make_vertex_plan(S.qd->Q, vorder, &S.plan))
+    throw Error{GEM_E_INVALID, "invalid vertex order (leaf first, or a vertex without an earlier neighbour)"};
+  build_vlevels(c, S);
+  S.mnb_all = std::max(S.mnb_all, *std::max_element(S.max_nb.begin(), S.max_nb.end()));
+  S.mnd_all = std::max(S.mnd_all, *std::max_element(S.max_ndup.begin(), S.max_ndup.end()));
+}
+
+uint64_t attempt_limit(const VQueryState &S, uint32_t a) {
+  if (!S.first_attempt || S.ranked.size() < 2) return 0;
+  if (S.max_attempts && a + 1 >= S.max_attempts) return 0;
+  const uint32_t rounds = a / std::max<uint32_t>(S.width, 1);
+  const double lim = (double)S.first_attempt * std::pow((double)std::max<uint32_t>(S.growth, 2), (double)rounds);
+  if (lim >= 9.0e18) return 0;  // beyond 2^63: no limit
+  return (uint64_t)lim;
+}
+
+// (Re)start a query's attempt: the identity level of its first vertex's
+// candidates, no count yet.
+static void seed_query(gem_ctx *c, VQueryState &V) {
+  hipStream_t s = c->stream;
+  gemh::SQuery<DevBuf> &Q = V.s;
+  Q.nb = V.max_nb;
+  Q.count = 0;
+  Q.fin_bound = Q.fin_got = 0;
+  Q.stack.clear();
+  Q.level_rows.assign(Q.nv, 0);
+  const uint64_t n0 = V.qd->ncan[V.plan.vorder[0]];
+  Q.level_rows[0] = n0;
+  if (n0 == 0) {
+    Q.done = true;
+    return;
+  }
+  auto buf = std::make_shared<DevBuf>();
+  buf->alloc(n0 * 4, s);
+  k_viota<<<(uint32_t)((n0 + 255) / 256), 256, 0, s>>>(buf->as<uint32_t>(), n0);
+  GEM_LAUNCH_CHECK("k_viota");
+  gemh::SLevel<DevBuf> L;
+  L.level = 1;
+  L.rows = std::move(buf);
+  L.nrows = n0;
+  L.k = 1;
+  Q.stack.push_back(std::move(L));
+}
+
 namespace {
 
 struct Launch {
@@ -995,6 +1041,33 @@ struct HipSchedBackend {
     t_ext += std::chrono::duration<double>(clk::now() - t0).count();
   }
 
+  void admit(uint32_t qi) { seed_query(c, *qs[qi]); }
+
+  // The attempt of query qi went past its limit (its levels are dropped):
+  // the next ranked order, with the next attempt's limit.
+  bool restart(uint32_t qi) {
+    VQueryState &V = *qs[qi];
+    gemh::SQuery<DevBuf> &Q = V.s;
+    if (!attempt_limit(V, 0)) return false;
+    const uint32_t a = Q.attempts + 1;
+    if (V.max_attempts && a >= V.max_attempts) return false;
+    V.cur = a % (uint32_t)V.ranked.size();
+    replan(c, V, V.ranked[V.cur]);
+    if (tc) tc->bind(V, qi);
+    if (V.mnb_all > slot.max_nb || V.mnd_all > slot.max_ndup)
+      throw Error{GEM_E_STATE, "restart order needs more launch LDS than the run was sized for"};
+    Q.attempts = a;
+    Q.attempt_start = Q.work;
+    Q.attempt_limit = attempt_limit(V, a);
+    seed_query(c, V);
+    if (verbose) {
+      std::fprintf(stderr, "[vjoin] query %u: attempt %u with order %u, limit %.3g\n", qi, a, V.ranked_idx[V.cur],
+                   (double)Q.attempt_limit);
+      std::fflush(stderr);
+    }
+    return true;
+  }
+
   void on_iteration(uint64_t iter, size_t active, uint64_t live, uint64_t budget) {
     if (!verbose || clk::now() - last_print < std::chrono::seconds(2)) return;
     last_print = clk::now();
@@ -1007,7 +1080,6 @@ struct HipSchedBackend {
 }  // namespace
 
 void VBatchRunner::run(std::vector<VQueryState *> &qs) {
-  hipStream_t s = c_->stream;
   g_persistent_blocks = c_->persistent_blocks;
   // the level budget: given, or a quarter of what the device could still
   // give the run's level buffers (free memory plus empty pool segments)
@@ -1022,11 +1094,17 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   ExtSlot slot;
   {
     uint32_t mnb = 1, mnd = 0;
-    for (VQueryState *S : qs)
+    for (VQueryState *S : qs) {
       for (size_t i = 0; i < S->max_nb.size(); ++i) {
         mnb = std::max(mnb, S->max_nb[i]);
         mnd = std::max(mnd, S->max_ndup[i]);
       }
+      // every order a restart may switch to
+      if (attempt_limit(*S, 0)) {
+        mnb = std::max(mnb, S->mnb_all);
+        mnd = std::max(mnd, S->mnd_all);
+      }
+    }
     slot.init(c_, (uint32_t)qs.size(), mnb, mnd);
   }
   // streamed tables: a block cache shared by the run's queries
@@ -1036,7 +1114,8 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
     tc = std::make_unique<TableCache>(c_, c_->stream_cache_bytes, c_->stream_slot_bytes);
     tc->attach(qs);
   }
-  // seed each query with the identity level of its first vertex
+  // queries are seeded with the identity level of their first vertex when
+  // the scheduler starts them (HipSchedBackend::admit)
   std::vector<gemh::SQuery<DevBuf> *> sq;
   for (VQueryState *S : qs) {
     gemh::SQuery<DevBuf> &Q = S->s;
@@ -1044,23 +1123,8 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
     Q.nv = S->qd->Q.nv;
     Q.nb = S->max_nb;
     Q.level_rows.assign(Q.nv, 0);
+    Q.attempt_limit = attempt_limit(*S, 0);
     sq.push_back(&Q);
-    const uint64_t n0 = S->qd->ncan[S->plan.vorder[0]];
-    Q.level_rows[0] = n0;
-    if (n0 == 0) {
-      Q.done = true;
-      continue;
-    }
-    auto buf = std::make_shared<DevBuf>();
-    buf->alloc(n0 * 4, s);
-    k_viota<<<(uint32_t)((n0 + 255) / 256), 256, 0, s>>>(buf->as<uint32_t>(), n0);
-    GEM_LAUNCH_CHECK("k_viota");
-    gemh::SLevel<DevBuf> L;
-    L.level = 1;
-    L.rows = std::move(buf);
-    L.nrows = n0;
-    L.k = 1;
-    Q.stack.push_back(std::move(L));
   }
   const char *verbose_env = std::getenv("GEM_VERBOSE");
   const bool verbose = verbose_env && verbose_env[0] == '1';
@@ -1069,6 +1133,7 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   cfg.cap = cap_;
   cfg.work_limit = work_limit_;
   cfg.budget = total_budget;
+  cfg.max_active = c_->max_active;
   const auto t0 = std::chrono::steady_clock::now();
   stats_ = gemh::SchedStats{};
   gemh::run_schedule(be, sq, cfg, &stats_);
@@ -1077,12 +1142,14 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   if (verbose) {
     std::fprintf(stderr,
                  "[vjoin] run of %zu queries: %llu iterations in %.2f s: %llu length passes %.2f s, extend steps "
-                 "%.2f s (%llu 256-row tasks); budget %.2f GiB, peak charged %.2f GiB, %llu forced iterations\n",
+                 "%.2f s (%llu 256-row tasks); budget %.2f GiB, peak charged %.2f GiB, %llu forced iterations, "
+                 "%llu restarts, %llu queries started (at most %u at once)\n",
                  qs.size(), (unsigned long long)stats_.iterations,
                  std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),
                  (unsigned long long)stats_.length_passes, be.t_len, be.t_ext, (unsigned long long)be.ext_tasks,
                  total_budget / 1073741824.0, stats_.peak_charged / 1073741824.0,
-                 (unsigned long long)stats_.forced);
+                 (unsigned long long)stats_.forced, (unsigned long long)stats_.restarts,
+                 (unsigned long long)stats_.admitted, cfg.max_active);
     std::fflush(stderr);
   }
 }

@@ -362,7 +362,21 @@ void candidate_orders(const QueryInfo &q, const QueryStats &s, const std::vector
       if (o == vo) return;
     out->push_back(vo);
   };
-  add(gem_vo);
+  // The same order with the query's leaves moved to the end, the leaf with
+  // the largest fan-out from its neighbour last.  Valid for any valid order (a
+  // leaf is never a non-leaf's only earlier neighbour); a leaf constrains no
+  // later vertex, and a last level with a single list is counted without a
+  // walk, so saturating queries reach the cap with few rows.
+  auto leaves_last = [&](const std::vector<uint32_t> &vo) {
+    std::vector<uint32_t> head, leaves;
+    for (uint32_t u : vo) (q.deg[u] == 1 && u != vo[0] ? leaves : head).push_back(u);
+    std::stable_sort(leaves.begin(), leaves.end(), [&](uint32_t a, uint32_t b) {
+      return fanout(q, s, q.adj[a][0].first, a) < fanout(q, s, q.adj[b][0].first, b);
+    });
+    head.insert(head.end(), leaves.begin(), leaves.end());
+    return head;
+  };
+  std::vector<std::vector<uint32_t>> greedy;
   // greedy by the running estimate: prefer vertices closing more edges, then
   // the smallest estimated growth
   std::vector<uint32_t> starts;
@@ -400,7 +414,7 @@ void candidate_orders(const QueryInfo &q, const QueryStats &s, const std::vector
       vo.push_back(pick);
       in[pick] = 1;
     }
-    add(vo);
+    greedy.push_back(vo);
     // a second variant: pure smallest-growth (ignoring closing-edge count)
     std::vector<uint32_t> vo2{starts[si]};
     std::fill(in.begin(), in.end(), 0);
@@ -425,7 +439,19 @@ void candidate_orders(const QueryInfo &q, const QueryStats &s, const std::vector
       vo2.push_back(pick);
       in[pick] = 1;
     }
-    add(vo2);
+    greedy.push_back(vo2);
+  }
+  // GEM's order first (the fallback), then the greedy orders, each preceded
+  // by its leaves-last form
+  add(gem_vo);
+  if (!greedy.empty()) {
+    add(leaves_last(greedy[0]));
+    add(greedy[0]);
+  }
+  add(leaves_last(gem_vo));
+  for (size_t i = 1; i < greedy.size(); ++i) {
+    add(leaves_last(greedy[i]));
+    add(greedy[i]);
   }
   // seeded random connected orders
   Rng rng(seed);

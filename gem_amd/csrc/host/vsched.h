@@ -7,15 +7,25 @@
 //
 // Memory rule.  Pending level rows of every query ("live") plus the output
 // buffers allocated in an iteration are charged against `budget`.  Each
-// iteration the budget left is shared by the queries that need an output
-// buffer: first an equal share each, then what the others left unused, deeper
-// levels first.  A query whose next 64-row unit does not fit what is left
-// waits.  Only when no query can take anything and no count-only work is
-// pending is the budget exceeded: the deepest pending level (whose memory is
-// released soonest) takes a chunk of at most `force_chunk` bytes, or one unit
-// if that is larger.  Such forced chunks pile up along one depth-first path at
-// most, so the charged bytes never exceed
+// running query has a quota: budget / max_active while queries wait to start,
+// budget / (queries running) once none waits (quotas only grow then).  A
+// query's next chunk may use (quota - its pending bytes) / (levels it still
+// has to write) bytes, so the levels after it find room too and a deep query
+// keeps descending in sizeable chunks instead of the budget filling up with
+// shallow levels.  A query whose next 64-row unit does not fit that share but
+// fits its whole free quota takes that one unit; otherwise it waits.  Only
+// when no query can take anything and no count-only work is pending is the
+// budget exceeded: the deepest pending level (whose memory is released
+// soonest) takes a chunk of at most `force_chunk` bytes, or one unit if that
+// is larger.  Such forced chunks pile up along one depth-first path at most,
+// so the charged bytes never exceed
 //   budget + (levels - 1) * max(force_chunk, largest unit bound).
+//
+// Restarts.  A query may carry a limit on the work of its current attempt
+// (attempt_limit).  When an attempt goes past it, its pending levels are
+// dropped and the backend may start the query again (another vertex order, a
+// larger limit); the final count does not depend on the order, so whichever
+// attempt ends gives the query's result.
 #pragma once
 
 #include <algorithm>
@@ -36,6 +46,10 @@ struct SchedConfig {
   uint64_t force_chunk = 0;              // past the budget (see above); 0 = budget / 32
   uint64_t final_piece = 1ull << 26;     // least output bound of a capped final-level piece
   uint64_t shrink_min = 4ull << 20;      // outputs at least this large move when half unused
+  // queries running at once (0 = all): the others wait in index order and
+  // start as running ones end (continuous admission; a query's first level
+  // is seeded when it starts)
+  uint32_t max_active = 0;
 };
 
 struct SchedStats {
@@ -47,6 +61,8 @@ struct SchedStats {
   uint64_t max_overshoot = 0;   // max over iterations of charged bytes beyond the budget
   uint64_t max_unit_bytes = 0;  // largest output bound of a forced unit
   uint64_t extend_items = 0;    // items handed to extend steps
+  uint64_t restarts = 0;        // attempts ended at their limit and started again
+  uint64_t admitted = 0;        // queries started
 };
 
 template <class Buf>
@@ -72,6 +88,13 @@ struct SQuery {
   // bounds, since one row with hub lists can cost 10^5 entries
   uint64_t work = 0;
   bool saturated = false, done = false, work_limited = false;
+  // current attempt: work at its start and the work it may add (0 = no
+  // limit); attempts = restarts so far
+  uint64_t attempt_start = 0, attempt_limit = 0;
+  uint32_t attempts = 0;
+  // measured final-level pieces of the current attempt: output bounds and
+  // embeddings counted (their ratio sizes the next capped piece)
+  uint64_t fin_bound = 0, fin_got = 0;
 };
 
 // One unit of a launch: rows [row_begin, row_end) of a pending level of
@@ -88,14 +111,18 @@ struct SItem {
 
 // Chunk planning of one iteration.  cand[i]: a query whose top level writes
 // output; prefix = its unit bound prefix, cursor = next unit, row_bytes =
-// bytes of one output row, depth = its level.  `left` = budget not yet
-// charged; other_progress = count-only work runs this iteration anyway.
-// Returns the end unit of each query's chunk (== cursor: it waits) and the
-// index of the query forced past the budget (-1 if none).
+// bytes of one output row, depth = its level, live = bytes of its pending
+// levels, quota = what they may hold, levels_left = levels it still has to
+// write, this one included.  `left` = budget not yet charged; other_progress
+// = count-only work runs this iteration anyway.  Returns the end unit of each
+// query's chunk (== cursor: it waits) and the index of the query forced past
+// the budget (-1 if none).
 struct ChunkCand {
   const std::vector<uint64_t> *prefix;
   uint64_t cursor, nunits, row_bytes;
   uint32_t depth;
+  uint64_t live, quota;
+  uint32_t levels_left;
 };
 inline int plan_chunks(const std::vector<ChunkCand> &cand, uint64_t left, uint64_t max_chunk, uint64_t force_chunk,
                        bool other_progress, std::vector<uint64_t> *end) {
@@ -107,9 +134,9 @@ inline int plan_chunks(const std::vector<ChunkCand> &cand, uint64_t left, uint64
     const std::vector<uint64_t> &P = *cand[i].prefix;
     return (P[e] - P[cand[i].cursor]) * cand[i].row_bytes;
   };
-  // largest e in [(*end)[i], nunits] with bytes(i, e) <= lim
+  // largest e in [cursor, nunits] with bytes(i, e) <= lim
   auto reach = [&](size_t i, uint64_t lim) {
-    uint64_t lo = (*end)[i], hi = cand[i].nunits;
+    uint64_t lo = cand[i].cursor, hi = cand[i].nunits;
     while (lo < hi) {
       const uint64_t mid = lo + (hi - lo + 1) / 2;
       if (bytes(i, mid) <= lim)
@@ -119,24 +146,22 @@ inline int plan_chunks(const std::vector<ChunkCand> &cand, uint64_t left, uint64
     }
     return lo;
   };
-  const uint64_t share = std::min<uint64_t>(left / n, max_chunk);
-  uint64_t used = 0;
-  for (size_t i = 0; i < n; ++i) {
-    (*end)[i] = reach(i, share);
-    used += bytes(i, (*end)[i]);
-  }
-  uint64_t spare = left - used;
-  if (spare) {
-    std::vector<size_t> by_depth(n);
-    for (size_t i = 0; i < n; ++i) by_depth[i] = i;
-    std::stable_sort(by_depth.begin(), by_depth.end(),
-                     [&](size_t a, size_t b) { return cand[a].depth > cand[b].depth; });
-    for (size_t j = 0; j < n && spare; ++j) {
-      const size_t i = by_depth[j];
-      const uint64_t have = bytes(i, (*end)[i]);
-      (*end)[i] = reach(i, std::min<uint64_t>(max_chunk, have + spare));
-      spare -= bytes(i, (*end)[i]) - have;
+  // deepest levels first: their memory is released soonest
+  std::vector<size_t> order(n);
+  for (size_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cand[a].depth > cand[b].depth; });
+  uint64_t room = left;
+  for (size_t i : order) {
+    const ChunkCand &c = cand[i];
+    const uint64_t free_q = c.quota > c.live ? c.quota - c.live : 0;
+    const uint64_t share = std::min<uint64_t>({max_chunk, free_q / std::max<uint32_t>(c.levels_left, 1), room});
+    uint64_t e = reach(i, share);
+    if (e == c.cursor && c.cursor < c.nunits) {
+      const uint64_t one = bytes(i, c.cursor + 1);
+      if (one <= std::min<uint64_t>({free_q, room, max_chunk})) e = c.cursor + 1;
     }
+    (*end)[i] = e;
+    room -= bytes(i, e);
   }
   bool any = other_progress;
   for (size_t i = 0; i < n && !any; ++i) any = (*end)[i] > cand[i].cursor;
@@ -165,23 +190,53 @@ inline int plan_chunks(const std::vector<ChunkCand> &cand, uint64_t left, uint64
 //   // a buffer of `bytes` holding the first `bytes` of `from`
 //   std::shared_ptr<Buf> shrink(const std::shared_ptr<Buf> &from, uint64_t bytes);
 //   void on_iteration(uint64_t iter, size_t active, uint64_t live, uint64_t budget);
+//   // query q's attempt went past its limit and its stack is empty: seed it
+//   // again for a new attempt (count, level_rows, attempt fields) and return
+//   // true, or return false to end it unfinished
+//   bool restart(uint32_t q);
+//   // query q starts: seed its first level (or mark it done); only called
+//   // for queries whose stack is empty and that are not done
+//   void admit(uint32_t q);
 //
-// The queries' stacks must be seeded (level 1 = the identity rows of the
-// first vertex's candidates) and their nb / nv filled.
+// The queries' nb / nv must be filled; a query whose stack is already seeded
+// (level 1 = the identity rows of the first vertex's candidates) when the
+// schedule starts is not passed to admit().
 template <class BE>
 void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const SchedConfig &cfg, SchedStats *st) {
   using Buf = typename BE::Buf;
   using Item = SItem<Buf>;
   const uint64_t budget = cfg.budget;
+  auto live_bytes = [&]() {
+    uint64_t b = 0;
+    for (SQuery<Buf> *S : qs)
+      for (const SLevel<Buf> &L : S->stack) b += L.rows ? BE::buf_bytes(*L.rows) : 0;
+    return b;
+  };
+  std::vector<uint8_t> started(qs.size(), 0);
+  size_t next = 0;
   for (;;) {
+    // ---- admission (index order)
+    {
+      size_t running = 0;
+      for (uint32_t i = 0; i < qs.size(); ++i)
+        if (started[i] && !qs[i]->done && !qs[i]->stack.empty()) ++running;
+      while (next < qs.size() && (!cfg.max_active || running < cfg.max_active)) {
+        const uint32_t qi = (uint32_t)next++;
+        started[qi] = 1;
+        if (qs[qi]->done) continue;
+        if (qs[qi]->stack.empty()) be.admit(qi);
+        if (!qs[qi]->done && !qs[qi]->stack.empty()) {
+          ++running;
+          ++st->admitted;
+        }
+      }
+    }
     std::vector<uint32_t> act;
     for (uint32_t i = 0; i < qs.size(); ++i)
-      if (!qs[i]->done && !qs[i]->stack.empty()) act.push_back(i);
+      if (started[i] && !qs[i]->done && !qs[i]->stack.empty()) act.push_back(i);
     if (act.empty()) break;
     ++st->iterations;
-    uint64_t live = 0;
-    for (SQuery<Buf> *S : qs)
-      for (const SLevel<Buf> &L : S->stack) live += L.rows ? BE::buf_bytes(*L.rows) : 0;
+    const uint64_t live = live_bytes();
     be.on_iteration(st->iterations, act.size(), live, budget);
 
     // ---- length pass: output bounds of every unmeasured top level (final
@@ -221,23 +276,37 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
     std::vector<Item> wr, fin;
     std::vector<std::shared_ptr<Buf>> keep_alive, outs;
     std::vector<uint32_t> wr_owner, fin_owner;
+    std::vector<uint64_t> fin_bounds;  // output bound of each measured final piece (0: unmeasured)
     std::vector<ChunkCand> cand;
     std::vector<uint32_t> cand_owner;
+    // quotas: fixed while queries wait to start, then the budget over the
+    // queries still running
+    const uint64_t quota = budget / std::max<uint64_t>(next < qs.size() && cfg.max_active ? cfg.max_active
+                                                                                          : act.size(), 1);
     for (uint32_t qi : act) {
       SQuery<Buf> &S = *qs[qi];
       SLevel<Buf> &top = S.stack.back();
       const uint64_t nu = top.nunits();
       if (top.level + 1 != S.nv) {
-        cand.push_back(ChunkCand{&top.unit_u, top.cursor, nu, 4ull * (top.k + 1), top.level});
+        uint64_t mine = 0;
+        for (const SLevel<Buf> &L : S.stack) mine += L.rows ? BE::buf_bytes(*L.rows) : 0;
+        cand.push_back(ChunkCand{&top.unit_u, top.cursor, nu, 4ull * (top.k + 1), top.level, mine, quota,
+                                 S.nv - 1 - top.level});
         cand_owner.push_back(qi);
         continue;
       }
-      uint64_t b1 = nu;
+      uint64_t b1 = nu, piece_bound = 0;
       if (top.measured) {
-        // a piece whose output bound is about twice what is still missing
-        // to exceed the cap (at least final_piece list entries of work)
+        // a piece whose output bound should just exceed the cap: what is
+        // still missing, over the share of bounds that earlier pieces turned
+        // into embeddings (1 before any), plus 10% (at least final_piece
+        // list entries of work)
         const uint64_t need = cfg.cap + 1 > S.count ? cfg.cap + 1 - S.count : 1;
-        const uint64_t target = std::max<uint64_t>(2 * need, cfg.final_piece);
+        const double ratio = S.fin_bound >= (1ull << 20) && S.fin_got
+                                 ? std::min(1.0, (double)S.fin_got / (double)S.fin_bound)
+                                 : 1.0;
+        const double want = 1.1 * (double)need / ratio;
+        const uint64_t target = std::max<uint64_t>(want >= 1.8e19 ? ~0ull : (uint64_t)want, cfg.final_piece);
         const std::vector<uint64_t> &P = top.unit_u;
         uint64_t lo = top.cursor + 1, hi = nu;  // smallest b1 with P[b1] - P[cursor] >= target, else nu
         while (lo < hi) {
@@ -248,7 +317,8 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
             lo = mid + 1;
         }
         b1 = lo;
-        S.work += P[b1] - P[top.cursor];
+        piece_bound = P[b1] - P[top.cursor];
+        S.work += piece_bound;
       }
       Item it;
       it.q = qi;
@@ -263,6 +333,7 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
       top.cursor = b1;
       fin.push_back(it);
       fin_owner.push_back(qi);
+      fin_bounds.push_back(piece_bound);
       if (top.cursor >= nu) S.stack.pop_back();
     }
 
@@ -339,6 +410,10 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
         SQuery<Buf> &S = *qs[fin_owner[t]];
         S.level_rows[S.nv - 1] += got_fin[t];
         S.count += got_fin[t];
+        if (fin_bounds[t]) {
+          S.fin_bound += fin_bounds[t];
+          S.fin_got += got_fin[t];
+        }
       }
     }
     for (uint32_t qi : act) {
@@ -351,6 +426,14 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
         S.work_limited = true;
         S.done = true;
         S.stack.clear();
+      } else if (S.attempt_limit && S.work - S.attempt_start > S.attempt_limit) {
+        S.stack.clear();
+        if (be.restart(qi)) {
+          ++st->restarts;
+        } else {
+          S.work_limited = true;
+          S.done = true;
+        }
       } else if (S.stack.empty()) {
         S.done = true;
       }

@@ -162,12 +162,18 @@ def choose(model, feats, qidx, margin=0.0):
     return {q: (best[q][0] if best[q][1] < first[q][1] - margin else first[q][0]) for q in best}
 
 
-def match_batch_learned(ctx, queries, model=None, max_orders=16, cap=0, work_limit=0, level_budget=0, group=0):
-    """Batched path with model-chosen orders (model None = GEM order)."""
+def match_batch_learned(ctx, queries, model=None, max_orders=16, cap=0, work_limit=0, level_budget=0, group=0,
+                        first_attempt=0, restart_growth=4, restart_width=1, max_attempts=0):
+    """Batched path with model-ranked orders (model None = candidates in
+    generation order, GEM's first).  first_attempt > 0: attempt a of a query
+    runs its (a mod n)-th ranked order and may do first_attempt *
+    restart_growth^(a // restart_width) work before the query starts again
+    (see gem_hip_match_batch_learned); work_limit bounds the total."""
     lib = load()
     nq = len(queries)
     views = (QueryView * max(nq, 1))(*[q.view() for q in queries])
-    opts = BatchOpts(int(cap), int(work_limit), int(level_budget), int(group), 0)
+    opts = BatchOpts(int(cap), int(work_limit), int(level_budget), int(group), int(restart_growth), int(first_attempt),
+                     int(restart_width), int(max_attempts))
     res = (JoinResult * max(nq, 1))()
     chosen = np.zeros(max(nq, 1), np.uint32)
     ctx._check(lib.gem_hip_match_batch_learned(ctx._h, views, nq, model._h if model is not None else None,
@@ -176,7 +182,8 @@ def match_batch_learned(ctx, queries, model=None, max_orders=16, cap=0, work_lim
                "gem_hip_match_batch_learned")
     return [{"count": int(res[i].count), "saturated": bool(res[i].saturated),
              "work_limited": bool(res[i].work_limited), "rows_processed": int(res[i].rows_processed),
-             "work": int(res[i].work), "chosen": int(chosen[i])} for i in range(nq)]
+             "work": int(res[i].work), "restarts": int(res[i].restarts), "chosen": int(chosen[i])}
+            for i in range(nq)]
 
 
 __all__ = ["OrderModel", "order_candidates", "collect", "choose", "match_batch_learned", "cost_label", "GemError"]

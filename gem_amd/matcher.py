@@ -161,6 +161,10 @@ class GemContext:
         `chunk`-entry pieces spread over waves (load balance only)."""
         self._check(self._lib.gem_hip_set_heavy_split(self._h, int(threshold), int(chunk)), "gem_hip_set_heavy_split")
 
+    def set_max_active(self, queries):
+        """Queries of a batched run holding join levels at once (0 = all)."""
+        self._check(self._lib.gem_hip_set_max_active(self._h, int(queries)), "gem_hip_set_max_active")
+
     def set_graph(self, on=True):
         """Replay the batched path's extend step as a captured hipGraph."""
         self._check(self._lib.gem_hip_set_graph(self._h, 1 if on else 0), "gem_hip_set_graph")
@@ -220,7 +224,7 @@ class GemContext:
                     a = np.ascontiguousarray(vo, dtype=np.uint32)
                     keep.append(a)
                     vo_arr[i] = _p32(a)
-        opts = BatchOpts(int(cap), int(work_limit), int(level_budget), int(group), 0)
+        opts = BatchOpts(int(cap), int(work_limit), int(level_budget), int(group), 0, 0, 0, 0)
         res = (JoinResult * max(nq, 1))()
         lr = np.zeros(max(nq, 1) * GEM_MAX_QV, np.uint64)
         self._check(self._lib.gem_hip_match_batch(self._h, views, nq, vo_arr, C.byref(opts), res, _p64(lr)),

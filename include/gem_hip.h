@@ -185,7 +185,9 @@ typedef struct gem_join_result {
   uint64_t num_embeddings_written; /* rows stored to emb_out            */
   uint64_t rows_processed; /* rows read by the join's steps              */
   uint32_t work_limited;   /* 1: stopped at the work limit, count unknown */
-  uint32_t reserved;
+  uint32_t restarts;       /* batched learned path: attempts that went past
+                              their work limit and were started again with
+                              the next ranked order (0 elsewhere)          */
   uint64_t work;           /* what the work limit bounds: rows read (plain
                               path); rows read + list entries walked, upper
                               bound (batched path)                         */
@@ -271,12 +273,22 @@ int gem_hip_memory_stats(gem_ctx *ctx, int reset_peak, gem_memory_stats *out);
 typedef struct gem_batch_opts {
   uint64_t cap;          /* per-query count cap (0 = none)                  */
   uint64_t work_limit;   /* work a query may do before it stops unfinished:
-                            rows read plus list entries walked (0 = none)  */
+                            rows read plus list entries walked, summed over
+                            its attempts (0 = none)                        */
   uint64_t level_budget; /* bytes for pending level chunks, shared by the
                             group's running queries (0 = a quarter of the
                             device memory free when the join starts)       */
   uint32_t group;        /* queries resident on the GPU at once (0 = all)   */
-  uint32_t reserved;
+  uint32_t restart_growth; /* learned path: factor between the work limits
+                            of successive rounds of attempts (0 = 4)       */
+  uint64_t first_attempt; /* learned path: work limit of the first round's
+                            attempts (0 = no restarts).  Attempt a runs the
+                            (a mod n)-th ranked order and may do
+                            first_attempt * growth^(a / width) work before
+                            the query starts again with the next one      */
+  uint32_t restart_width; /* learned path: attempts per round (0 = 1)       */
+  uint32_t max_attempts;  /* learned path: attempts per query (0 = no bound;
+                            otherwise the last one has no limit)           */
 } gem_batch_opts;
 
 /* Match nq queries.  vertex_orders (nullable, and each entry nullable) gives a
@@ -303,6 +315,13 @@ int gem_hip_set_heavy_split(gem_ctx *ctx, uint32_t threshold, uint32_t chunk);
  * and replayed; off (default): launched eagerly.  Kernel profiling forces the
  * eager form. */
 int gem_hip_set_graph(gem_ctx *ctx, int on);
+/* Queries of a batched run that hold join levels at once (default 64; 0 =
+ * all): the others wait and start, in order, as running ones end (continuous
+ * admission); each running query's pending rows are held to
+ * level budget / this many bytes while queries wait, and to an equal share of
+ * the budget over the running queries after that.  Counts do not depend on
+ * it.  Set by the autotuner. */
+int gem_hip_set_max_active(gem_ctx *ctx, uint32_t queries);
 
 /* Out-of-core edge tables for the batched paths (replaces the reference's
  * out-of-core MemoryManager / LRUEvictPolicy, out-of-core/lib/structure/src/
@@ -379,10 +398,17 @@ int gem_mlp_train(gem_mlp *m, const int32_t *feats, const float *targets, uint32
 int gem_hip_order_candidates(gem_ctx *ctx, uint32_t max_orders, uint64_t seed, uint32_t *orders, int32_t *feats,
                              uint32_t *n);
 
-/* Batched path with model-chosen orders: for each query the candidates are
- * scored by `mlp` and the lowest predicted cost is run (mlp == NULL: GEM
- * order).  chosen (nullable) receives, per query, the index of the chosen
- * candidate (0 = GEM's order). */
+/* Batched path with model-chosen orders: for each query up to max_orders
+ * candidate orders (seeded by a hash of the query) are ranked by `mlp`'s
+ * predicted cost, GEM's order first unless a candidate beats it by more than
+ * the model's margin (mlp == NULL: in generation order, GEM's first), and the
+ * first ranked order runs.  With opts->first_attempt != 0 a query whose
+ * attempt exceeds its work limit starts again with the next ranked order
+ * (rounds of restart_width attempts, each round's limit restart_growth times
+ * the last), so every query ends with a count unless opts->work_limit (total
+ * over its attempts) stops it.  chosen (nullable) receives, per query,
+ * the candidate index of the order that produced the result (0 = GEM's
+ * order). */
 int gem_hip_match_batch_learned(gem_ctx *ctx, const gem_query_view *queries, uint32_t nq, gem_mlp *mlp,
                                 uint32_t max_orders, const gem_batch_opts *opts, gem_join_result *results,
                                 uint64_t *level_rows, uint32_t *chosen);

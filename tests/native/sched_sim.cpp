@@ -8,7 +8,7 @@
 // memory is touched).  The program runs the scheduler over many queries for
 // several passes on one pool and prints one JSON object per pass.
 //
-//   sched_sim run <queries> <budget_bytes> <passes> <seed>
+//   sched_sim run <queries> <budget_bytes> <passes> <seed> [max_active]
 //   sched_sim plan      (fixed plan_chunks cases, one JSON line)
 #include <cinttypes>
 #include <cstdio>
@@ -132,9 +132,11 @@ struct SimBackend {
     }
   }
   void on_iteration(uint64_t, size_t, uint64_t, uint64_t) {}
+  bool restart(uint32_t) { return false; }
+  void admit(uint32_t) {}
 };
 
-int run(int nq, uint64_t budget, int passes, uint64_t seed) {
+int run(int nq, uint64_t budget, int passes, uint64_t seed, uint32_t max_active) {
   FakeDevice dev;
   gemh::PoolCore pool(gemh::PoolBacking{fake_alloc, fake_release, fake_sync, &dev});
   for (int pass = 0; pass < passes; ++pass) {
@@ -166,6 +168,7 @@ int run(int nq, uint64_t budget, int passes, uint64_t seed) {
     cfg.work_limit = 2000000000ull;
     cfg.budget = budget;
     cfg.max_chunk = 1ull << 30;
+    cfg.max_active = max_active;
     gemh::SchedStats st;
     const gemh::PoolStats before = pool.stats();
     pool.reset_peak();
@@ -208,31 +211,38 @@ int plan_cases() {
     for (size_t i = 0; i < end.size(); ++i) std::printf("%s%" PRIu64, i ? ", " : "", end[i]);
     std::printf("]}");
   };
-  // prefixes in rows; row_bytes 8
-  const std::vector<uint64_t> a = {0, 10, 20, 30, 40, 50};    // 10 rows per unit
+  // prefixes in rows; row_bytes 8; fields: prefix, cursor, nunits, row bytes,
+  // depth, live, quota, levels left
+  const std::vector<uint64_t> a = {0, 10, 20, 30, 40, 50};    // 10 rows (80 B) per unit
   const std::vector<uint64_t> b = {0, 100, 100, 100, 1100};   // a hub unit at the end
   const std::vector<uint64_t> z = {0, 0, 0, 5};               // zero-bound units, then 5
   std::printf("{");
-  // equal shares: left 400 B -> 200 B each -> 25 rows -> a: 2 units (20 rows), b: 0 units
-  // (100 rows = 800 B > 200); spare 240 B goes deeper first (b, depth 3) -> b still 0 ->
-  // a extends to 50 rows = 400 <= 160 + 240
-  show("share", {{&a, 0, 5, 8, 1}, {&b, 0, 4, 8, 3}}, 400, 1 << 20, 64, false);
+  // quotas of 400 B: a (two levels to write) may use 200 B -> 2 units; b's
+  // first unit (800 B) exceeds its whole free quota -> it waits
+  show("share", {{&a, 0, 5, 8, 1, 0, 400, 2}, {&b, 0, 4, 8, 3, 0, 400, 1}}, 800, 1 << 20, 64, false);
   std::printf(", ");
   // nothing fits, nothing else runs: the deepest (b, depth 3) is forced one unit
   // (force chunk 64 B < its 800 B unit)
-  show("forced", {{&a, 0, 5, 8, 1}, {&b, 0, 4, 8, 3}}, 0, 1 << 20, 64, false);
+  show("forced", {{&a, 0, 5, 8, 1, 0, 400, 2}, {&b, 0, 4, 8, 3, 0, 400, 1}}, 0, 1 << 20, 64, false);
   std::printf(", ");
   // nothing fits but count-only work runs: everyone waits
-  show("wait", {{&a, 0, 5, 8, 1}, {&b, 0, 4, 8, 3}}, 0, 1 << 20, 64, true);
+  show("wait", {{&a, 0, 5, 8, 1, 0, 400, 2}, {&b, 0, 4, 8, 3, 0, 400, 1}}, 0, 1 << 20, 64, true);
   std::printf(", ");
   // zero-bound units cost nothing: taken even with no budget
-  show("zero", {{&z, 0, 3, 8, 2}}, 0, 1 << 20, 64, true);
+  show("zero", {{&z, 0, 3, 8, 2, 0, 0, 1}}, 0, 1 << 20, 64, true);
   std::printf(", ");
   // forced with a large force chunk: b takes units 1-3 (100 rows = 800 B), not the hub
-  show("force_chunk", {{&a, 0, 5, 8, 1}, {&b, 0, 4, 8, 3}}, 0, 1 << 20, 1000, false);
+  show("force_chunk", {{&a, 0, 5, 8, 1, 0, 400, 2}, {&b, 0, 4, 8, 3, 0, 400, 1}}, 0, 1 << 20, 1000, false);
   std::printf(", ");
   // max_chunk caps a single query's share
-  show("max_chunk", {{&a, 0, 5, 8, 1}}, 1 << 20, 160, 64, false);
+  show("max_chunk", {{&a, 0, 5, 8, 1, 0, 1 << 20, 1}}, 1 << 20, 160, 64, false);
+  std::printf(", ");
+  // a share (100 / 4 levels = 25 B) below one unit (80 B) that fits the whole
+  // free quota: one unit
+  show("one_unit", {{&a, 0, 5, 8, 1, 0, 100, 4}}, 1 << 20, 1 << 20, 64, false);
+  std::printf(", ");
+  // pending rows count against the quota: 400 - 300 = 100 B -> one unit
+  show("quota_live", {{&a, 0, 5, 8, 1, 300, 400, 1}}, 1 << 20, 1 << 20, 64, false);
   std::printf("}\n");
   return 0;
 }
@@ -276,7 +286,7 @@ int main(int argc, char **argv) {
   if (argc >= 2 && !std::strcmp(argv[1], "pool")) return pool_cases();
   if (argc >= 6 && !std::strcmp(argv[1], "run"))
     return run(std::atoi(argv[2]), std::strtoull(argv[3], nullptr, 10), std::atoi(argv[4]),
-               std::strtoull(argv[5], nullptr, 10));
-  std::fprintf(stderr, "usage: sched_sim run <queries> <budget> <passes> <seed> | sched_sim plan\n");
+               std::strtoull(argv[5], nullptr, 10), argc >= 7 ? (uint32_t)std::atoi(argv[6]) : 0u);
+  std::fprintf(stderr, "usage: sched_sim run <queries> <budget> <passes> <seed> [max_active] | sched_sim plan\n");
   return 2;
 }

@@ -31,9 +31,9 @@ def _run(exe, *args):
 
 def test_plan_chunks_cases(sim):
     (r,) = _run(sim, "plan")
-    # equal shares, then the spare to the deeper level (which cannot use it:
-    # its first unit is 800 B), then to the other
-    assert r["share"] == {"forced": -1, "end": [5, 0]}
+    # per-query quota split over the levels still to write; a unit larger
+    # than the whole free quota waits
+    assert r["share"] == {"forced": -1, "end": [2, 0]}
     # no budget, no count-only work: the deepest level is forced one unit
     assert r["forced"] == {"forced": 1, "end": [0, 1]}
     # no budget but count-only work runs: everyone waits
@@ -44,6 +44,10 @@ def test_plan_chunks_cases(sim):
     assert r["force_chunk"] == {"forced": 1, "end": [0, 3]}
     # one query's chunk never exceeds max_chunk
     assert r["max_chunk"] == {"forced": -1, "end": [2]}
+    # a share below one unit takes one unit when the free quota holds it
+    assert r["one_unit"] == {"forced": -1, "end": [1]}
+    # pending rows are charged to the quota
+    assert r["quota_live"] == {"forced": -1, "end": [1]}
 
 
 def test_pool_class_boundary(sim):
@@ -88,3 +92,17 @@ def test_scheduler_large_budget_never_forces(sim):
     assert p["forced"] == 0
     assert p["max_overshoot"] == 0
     assert p["pool_in_use_after"] == 0 and p["held_after_trim"] == 0
+
+
+def test_scheduler_continuous_admission(sim):
+    """160 queries, at most 16 running at once (the others start as running
+    ones end): every query ends, the charged bytes stay within the same
+    bound, and the pool returns to its starting state."""
+    budget = 256 << 20
+    passes = _run(sim, "run", 160, budget, 2, 13, 16)
+    for p in passes:
+        headroom = (p["max_nv"] - 1) * max(p["force_chunk"], p["max_unit_bytes"])
+        assert p["peak_charged"] <= budget + headroom
+        assert p["pending"] == 0
+        assert p["saturated"] + p["work_limited"] + p["finished"] == p["queries"]
+        assert p["pool_in_use_after"] == 0 and p["held_after_trim"] == 0
