@@ -98,6 +98,8 @@ SIGNATURES = [
     ("gem_hip_stream_stats", C.c_int, [C.c_void_p, C.POINTER(StreamStats)]),
     ("gem_hip_load_data_csr", C.c_int, [C.c_void_p, C.POINTER(CsrView)]),
     ("gem_hip_filter", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, u32p, u32p, C.c_uint64]),
+    ("gem_hip_filter_noalloc", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, C.c_void_p, C.c_void_p,
+                                         C.c_uint64]),
     ("gem_hip_position_lookup", C.c_int, [C.c_void_p, C.c_uint32, u32p, C.c_uint64, u32p]),
     ("gem_hip_build_edge_tables", C.c_int, [C.c_void_p, u64p]),
     ("gem_hip_read_edge_table", C.c_int, [C.c_void_p, C.c_uint32, u64p, u32p]),

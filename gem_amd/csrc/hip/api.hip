@@ -43,6 +43,7 @@ int gem_hip_create(int device, gem_ctx **out) {
   int st = guard(c, [&]() {
     GEM_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) GEM_HIP_CHECK(hipEventCreate(&e));
+    filter_scratch_init(device);
   });
   if (st) {
     gem_hip_destroy(c);
@@ -202,6 +203,14 @@ int gem_hip_filter(gem_ctx *c, const gem_query_view *q, uint32_t threshold, uint
       }
       GEM_HIP_CHECK(hipStreamSynchronize(c->stream));
     }
+  });
+}
+
+int gem_hip_filter_noalloc(gem_ctx *c, const gem_query_view *q, uint32_t threshold, uint32_t *d_num_can,
+                           uint32_t *d_can, uint64_t can_pitch) {
+  return guard(c, [&]() {
+    if (threshold != 0) throw Error{GEM_E_INVALID, "only threshold 0 is supported"};
+    run_filter_noalloc(c, q, threshold, d_num_can, d_can, can_pitch);
   });
 }
 

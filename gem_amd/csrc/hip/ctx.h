@@ -359,6 +359,13 @@ void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T);
 void run_tables(gem_ctx *c);
 // the same phases on an explicit per-query state (several queries resident)
 void run_filter(gem_ctx *c, QueryDev &Qd, const gem_query_view *qv, uint32_t T);
+// The filter into caller-owned device arrays with the library's fixed scratch
+// array (no allocation); filter_scratch_init finds that array's address on
+// the current device (once per device, at context creation).
+void filter_scratch_init(int device);
+uint64_t filter_scratch_bytes(const GraphDev &G, uint32_t nv);
+void run_filter_noalloc(gem_ctx *c, const gem_query_view *qv, uint32_t T, uint32_t *d_num_can, uint32_t *d_can,
+                        uint64_t can_pitch);
 void run_tables(gem_ctx *c, QueryDev &Qd);
 void run_position_lookup(gem_ctx *c, uint32_t u, const uint32_t *h_vertices, uint64_t n, uint32_t *h_pos);
 

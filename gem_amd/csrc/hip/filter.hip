@@ -20,6 +20,9 @@
 // into a private LDS column (label-major, lane-minor: conflict-free); a
 // higher-degree vertex is counted by the whole wave, 64 neighbours per step,
 // one ballot + popcount per query label (no atomics).
+#include <mutex>
+#include <string>
+
 #include "ctx.h"
 #include "wave.h"
 
@@ -261,7 +264,8 @@ __global__ void k_position_lookup(const uint64_t *__restrict__ bitmap, const uin
   pos[i] = p;
 }
 
-// Device buffers of one filter run.
+// Device buffers of one filter run (library-owned, or carved from the
+// no-allocation scratch array).
 struct FilterBufs {
   FilterQuery *fq = nullptr;
   uint8_t *slot_of = nullptr;
@@ -497,6 +501,103 @@ void run_filter(gem_ctx *c, QueryDev &Qd, const gem_query_view *qv, uint32_t T) 
   c->times.filter_ms = ms;
   c->times.index_ms = 0;  // the index (bitmap + rank) is a by-product of the filter
   Qd.filtered = true;
+}
+
+// ------------------------------------------------ no-allocation variant
+// Scratch of gem_hip_filter_noalloc: the query description, the label map,
+// candidate-list offsets, bitmaps, tile counts, ranks and the heavy/hub lists
+// of one call, carved from one fixed device array (each device holds its own
+// copy until the process ends).  Calls that overlap on one device share it and
+// race.
+constexpr uint64_t FILTER_SCRATCH_BYTES = 64ull << 20;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[FILTER_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[SCRATCH_MAX_DEVICES];  // device address of `scratch`, per device
+
+void filter_scratch_init(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES) throw Error{GEM_E_LIMIT, "device id beyond the scratch table"};
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    throw Error{GEM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+  g_scratch_ptr[device] = static_cast<unsigned char *>(p);
+}
+
+uint64_t filter_scratch_bytes(const GraphDev &G, uint32_t nv) {
+  const uint64_t W = (G.n + 63) / 64;
+  const uint64_t hub_cap = std::max<uint32_t>(G.n_hub, 1);
+  auto up = [](uint64_t x) { return (x + 255) / 256 * 256; };
+  return up(sizeof(FilterQuery)) + up(std::max<uint64_t>(G.label_freq.size(), 1)) + up(8ull * (nv + 1)) +
+         up(8ull * nv * W) + up(4ull * nv * W) + up(4ull * nv * (W + 1)) + up(4ull * std::max<uint64_t>(G.n, 1)) +
+         up(4) + up(4 * hub_cap) + up(4) + up(4ull * hub_cap * GEM_MAX_QV);
+}
+
+void run_filter_noalloc(gem_ctx *c, const gem_query_view *qv, uint32_t T, uint32_t *d_num_can, uint32_t *d_can,
+                        uint64_t can_pitch) {
+  if (!c->g.loaded) throw Error{GEM_E_STATE, "no data graph loaded"};
+  if (!d_num_can) throw Error{GEM_E_INVALID, "d_num_can is null"};
+  if (c->device < 0 || c->device >= SCRATCH_MAX_DEVICES || !g_scratch_ptr[c->device])
+    throw Error{GEM_E_STATE, "scratch not initialised on this device"};
+  gemh::QueryInfo Q;
+  std::string why;
+  int st = gemh::analyse_query(qv, &Q, &why);
+  if (st) throw Error{st, why};
+  const GraphDev &G = c->g;
+  const uint32_t nv = Q.nv;
+  const uint32_t W = (G.n + 63) / 64;
+  const uint64_t need = filter_scratch_bytes(G, nv);
+  if (need > FILTER_SCRATCH_BYTES)
+    throw Error{GEM_E_LIMIT, "gem_hip_filter_noalloc: this graph and query need " + std::to_string(need) +
+                                 " bytes of scratch, more than its " + std::to_string(FILTER_SCRATCH_BYTES)};
+  FilterPrep P;
+  filter_prepare(G, Q, T, W, &P);
+  // candidate lists at u * can_pitch in the caller's array: each list needs
+  // room for its label's frequency (the bound the kernels write within)
+  std::vector<uint64_t> can_off(nv + 1, 0);
+  for (uint32_t u = 0; u < nv; ++u) {
+    const uint64_t ub = P.can_off[u + 1] - P.can_off[u];
+    if (d_can && ub > can_pitch)
+      throw Error{GEM_E_INVALID, "can_pitch is smaller than the frequency of a query vertex's label"};
+    can_off[u] = (uint64_t)u * can_pitch;
+  }
+  can_off[nv] = (uint64_t)nv * can_pitch;
+  // carve the scratch array
+  unsigned char *base = g_scratch_ptr[c->device];
+  uint64_t at = 0;
+  auto take = [&](uint64_t bytes) {
+    unsigned char *p = base + at;
+    at += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  const uint32_t hub_cap = std::max<uint32_t>(G.n_hub, 1);
+  FilterBufs B;
+  B.fq = reinterpret_cast<FilterQuery *>(take(sizeof(FilterQuery)));
+  B.slot_of = take(P.slot_of.size());
+  uint64_t *d_can_off = reinterpret_cast<uint64_t *>(take(8ull * (nv + 1)));
+  B.can_off = d_can_off;
+  B.bitmap = reinterpret_cast<uint64_t *>(take(8ull * nv * W));
+  B.tile_count = reinterpret_cast<uint32_t *>(take(4ull * nv * W));
+  B.rank = reinterpret_cast<uint32_t *>(take(4ull * nv * (W + 1)));
+  B.heavy_list = reinterpret_cast<uint32_t *>(take(4ull * std::max<uint32_t>(G.n, 1)));
+  B.heavy_n = reinterpret_cast<uint32_t *>(take(4));
+  B.hub_list = reinterpret_cast<uint32_t *>(take(4ull * hub_cap));
+  B.hub_n = reinterpret_cast<uint32_t *>(take(4));
+  B.hub_cnt = reinterpret_cast<uint32_t *>(take(4ull * hub_cap * GEM_MAX_QV));
+  B.hub_cap = hub_cap;
+  if (at > FILTER_SCRATCH_BYTES) throw Error{GEM_E_STATE, "scratch layout exceeds its size"};
+  B.can = d_can;  // null: counts only (no compaction)
+  hipStream_t s = c->stream;
+  GEM_HIP_CHECK(hipMemcpyAsync(B.fq, &P.fq, sizeof(FilterQuery), hipMemcpyHostToDevice, s));
+  GEM_HIP_CHECK(hipMemcpyAsync(B.slot_of, P.slot_of.data(), P.slot_of.size(), hipMemcpyHostToDevice, s));
+  GEM_HIP_CHECK(hipMemcpyAsync(d_can_off, can_off.data(), 8ull * (nv + 1), hipMemcpyHostToDevice, s));
+  filter_launch(c, nv, W, B);
+  k_gather_counts<<<1, 64, 0, s>>>(B.rank, nv, W, d_num_can);
+  GEM_LAUNCH_CHECK("k_gather_counts");
+  // the host arrays above are read by the copies: wait before they go
+  GEM_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void run_position_lookup(gem_ctx *c, uint32_t u, const uint32_t *h_vertices, uint64_t n, uint32_t *h_pos) {

@@ -157,6 +157,22 @@ int gem_hip_load_data_csr(gem_ctx *ctx, const gem_csr_view *g);
 int gem_hip_filter(gem_ctx *ctx, const gem_query_view *q, uint32_t threshold,
                    uint32_t *h_num_can, uint32_t *h_can, uint64_t can_pitch);
 
+/* gem_hip_filter without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Outputs go
+ * to caller-owned DEVICE arrays: d_num_can[num_v] and, when d_can is
+ * non-null, Can(u) ascending at d_can + u * can_pitch (can_pitch must be at
+ * least the data frequency of every query vertex's label).  The filter's
+ * intermediate buffers live in one fixed 64 MiB device array per device
+ * (held until the process ends; its address is found when a context is
+ * created): a graph and query needing more (about 12 B per data vertex per
+ * query vertex plus 4 B per data vertex) fail with GEM_E_LIMIT.  Calls that
+ * overlap on one device share that array and race.  The context's current
+ * query (gem_hip_filter's state) is not touched.  Returns when the outputs are
+ * written. */
+int gem_hip_filter_noalloc(gem_ctx *ctx, const gem_query_view *q, uint32_t threshold,
+                           uint32_t *d_num_can, uint32_t *d_can, uint64_t can_pitch);
+
 /* Position index probe (the result the reference's cuckoo hash returns,
  * in-core/lib/structure/include/unifiedTrie.cuh:84-125): pos_out[i] =
  * index of vertices[i] in Can(u), or 0xFFFFFFFF when it is not a candidate. */
