@@ -39,6 +39,8 @@ DESCR = {
               "queries (50 dense, 50 sparse); count capped at 1e9",
     "rmat22": "R-MAT 2^22 vertices, avg degree 16, 32 Zipf labels; 512 random-walk queries of 4-32 vertices; "
               "count capped at 1e9",
+    "rmat20_learn": "R-MAT 2^20 vertices, avg degree 16, 16 Zipf labels; 256 random-walk queries of 8-24 vertices "
+                    "(learned-order workload); count capped at 1e9",
     "ooc24": "R-MAT 2^24 vertices, avg degree 64, 64 Zipf labels; 32 sparse random-walk 12-vertex queries; "
              "edge tables streamed from pinned host memory through a device cache; count capped at 1e9",
 }
@@ -117,10 +119,12 @@ def main():
     ap.add_argument("--workload", default="rmat22", choices=sorted(DESCR))
     ap.add_argument("--queries", type=int, default=0, help="use only the first N queries")
     ap.add_argument("--cap", type=float, default=1e9)
-    ap.add_argument("--first-attempt", type=float, default=2e9,
+    ap.add_argument("--first-attempt", type=float, default=1e8,
                     help="work (rows read + list entries walked) of a query's first attempt in the batched path; "
-                         "attempt a may do first*2^a before the query starts again with its next ranked order "
-                         "(0 = no restarts)")
+                         "attempt a runs its (a mod n)-th ranked order and may do first*growth^(a // width) before "
+                         "the query starts again (0 = no restarts)")
+    ap.add_argument("--restart-growth", type=int, default=4)
+    ap.add_argument("--restart-width", type=int, default=1)
     ap.add_argument("--work-limit", type=float, default=0,
                     help="total work over a batched query's attempts after which it is reported unfinished "
                          "(0 = none)")
@@ -201,7 +205,8 @@ def main():
     from gem_amd.learned import OrderModel, match_batch_learned
     model = None
     if args.model is None:
-        args.model = os.path.join(ROOT, "gem_amd", "models", f"order_mlp_{args.workload}.npz")
+        graph_name = {"rmat20_learn": "rmat20"}.get(args.workload, args.workload)
+        args.model = os.path.join(ROOT, "gem_amd", "models", f"order_mlp_{graph_name}.npz")
     if args.model and os.path.exists(args.model):
         model = OrderModel.load(ctx, args.model)
         log(f"[bench] order model {args.model}: trained on {model.meta.get('workload')} "
@@ -211,7 +216,8 @@ def main():
 
     def batched_pass(m=model, queries=qs):
         return match_batch_learned(ctx, queries, m, max_orders=args.orders, cap=cap, work_limit=wl,
-                                   level_budget=level_budget, group=args.group, first_attempt=first)
+                                   level_budget=level_budget, group=args.group, first_attempt=first,
+                                   restart_growth=args.restart_growth, restart_width=args.restart_width)
 
     ctx.memory_stats(reset_peak=True)
     for w in range(args.warmup):
@@ -391,7 +397,8 @@ def main():
                    "path": "batched vertex-at-a-time join, " + (
                        "learned order (MLP-ranked candidate orders)" if model is not None
                        else "candidate orders in generation order"),
-                   "cap": cap, "first_attempt_work": first, "work_limit": wl,
+                   "cap": cap, "restarts": {"first_attempt_work": first, "growth": args.restart_growth,
+                                            "width": args.restart_width}, "work_limit": wl,
                    "tuning": tuning or None},
         "embeddings_per_s": round(eps, 1),
         "pass_seconds": [round(x, 3) for x in pass_s],

@@ -127,7 +127,8 @@ void run_order_candidates(gem_ctx *c, uint32_t max_orders, uint64_t seed, uint32
   if (!Qd.tabled) throw Error{GEM_E_STATE, "edge tables have not been built"};
   const gemh::QueryStats st = query_stats(c, Qd);
   std::vector<std::vector<uint32_t>> cand;
-  gemh::candidate_orders(Qd.Q, st, gem_vertex_order(Qd), max_orders, seed, &cand);
+  // seed 0: the query's own hash, as the batched path seeds it
+  gemh::candidate_orders(Qd.Q, st, gem_vertex_order(Qd), max_orders, seed ? seed : query_seed(Qd.Q), &cand);
   for (size_t i = 0; i < cand.size(); ++i) {
     if (orders) std::copy(cand[i].begin(), cand[i].end(), orders + i * Qd.Q.nv);
     if (feats) gemh::order_features(Qd.Q, st, cand[i], feats + i * gemh::FEAT_DIM);

@@ -1,21 +1,27 @@
-"""Learned join order: the order-cost MLP (GPU kernels in libgem_hip.so) and
-its training loop.
+"""Learned join order: the order-cost MLP (GPU kernels in libgem_hip.so),
+its training data and the restart schedule it ranks orders for.
 
-Training data are (order features, cost) pairs the matcher logs itself: for
-each training query the library proposes candidate vertex orders
-(gem_hip_order_candidates: GEM's order first, then estimate-greedy and seeded
-random orders), the vertex-at-a-time matcher runs each one, and the cost
-label is log2(1 + work), work = rows read plus list entries walked.  The
-network is fit by full-batch Adam on the GPU; at match time the batched path
-runs the candidate with the lowest predicted cost when it beats GEM's order by
-more than the model's margin, else GEM's order (also the order when no model
-is given).
+Training data are tuples the matcher logs itself: for each training query the
+library proposes candidate vertex orders (gem_hip_order_candidates: GEM's
+order first, then estimate-greedy orders, each also with the query's leaves
+moved last, then seeded random orders), the batched vertex-at-a-time matcher
+runs every (query, order) pair at the benchmark's count cap, and logs the
+rows each level produced (the intermediate-result sizes) and the work it did
+(rows read plus list entries walked, the cost the restart limits bound).
+The label of a tuple is log2(1 + work) minus the least such value over its
+query's orders, so the network learns how the orders of one query compare.
+The network is fit by full-batch Adam on the GPU with the library's own
+forward/backward kernels.  At match time the batched path ranks each query's
+candidates by predicted cost and runs them in that order under geometrically
+growing work limits (restarts), so a poor first choice costs a bounded amount
+of work; without a model the candidates run in generation order (GEM's
+first).
 """
 import ctypes as C
 
 import numpy as np
 
-from ._lib import GEM_FEAT_DIM, BatchOpts, GemError, JoinResult, QueryView, load
+from ._lib import GEM_FEAT_DIM, GEM_MAX_QV, BatchOpts, GemError, JoinResult, QueryView, load
 
 HIDDEN = 128
 
@@ -115,51 +121,92 @@ def order_candidates(ctx, query, max_orders=16, seed=0):
 
 
 def cost_label(work):
-    """Training label of one (query, order) run: log2(1 + work), work = rows
-    read plus list entries walked by the vertex-at-a-time join."""
+    """log2(1 + work), work = rows read plus list entries walked by the
+    vertex-at-a-time join."""
     return float(np.log2(1.0 + float(work)))
 
 
-def collect(ctx, queries, max_orders=16, cap=0, work_limit=0, seed=0, log=None):
-    """Run every candidate order of every query with the vertex-at-a-time
-    matcher and log (features, cost).  Returns (feats [N x D] int32,
-    costs [N] float32, query index [N], candidate index [N], counts [N])."""
-    F, Y, QI, CI, CNT = [], [], [], [], []
-    old_limit = None
-    if work_limit:
-        ctx.set_work_limit(work_limit)
+def collect(ctx, queries, max_orders=8, cap=0, work_limit=0, group=64, log=None):
+    """Run every candidate order of every query through the batched matcher
+    (group (query, order) pairs per run) and log the tuples.  Returns a dict
+    of arrays over tuples: feats [N x D] int32, work [N] (rows read plus list
+    entries walked; work_limit when stopped there), limited [N] bool, level_rows
+    [N x GEM_MAX_QV] (rows after each level), count [N] (-1 when stopped),
+    query [N], cand [N] (candidate index; 0 = GEM's order), nv [N]."""
+    pairs, F, Q, Ci = [], [], [], []
     for qi, q in enumerate(queries):
         ctx.filter(q, want_lists=False)
         ctx.build_edge_tables()
-        orders, feats = order_candidates(ctx, q, max_orders, seed + qi)
+        orders, feats = order_candidates(ctx, q, max_orders, 0)
         for ci in range(len(orders)):
-            r = ctx.vjoin(orders[ci], cap=cap)
+            pairs.append((q, orders[ci]))
             F.append(feats[ci])
-            Y.append(cost_label(r["work"]))
-            QI.append(qi)
-            CI.append(ci)
+            Q.append(qi)
+            Ci.append(ci)
+    W, LIM, LR, CNT, NV = [], [], [], [], []
+    for b in range(0, len(pairs), group):
+        part = pairs[b:b + group]
+        res = ctx.match_batch([p[0] for p in part], vertex_orders=[p[1] for p in part], cap=cap,
+                              work_limit=work_limit, group=group)
+        for (q, _), r in zip(part, res):
+            W.append(min(r["work"], work_limit) if (work_limit and r["work_limited"]) else r["work"])
+            LIM.append(bool(r["work_limited"]))
+            lr = np.zeros(GEM_MAX_QV, np.uint64)
+            lr[:len(r["level_rows"])] = r["level_rows"]
+            LR.append(lr)
             CNT.append(-1 if r["work_limited"] else r["count"])
-        if log and (qi + 1) % 16 == 0:
-            log(f"[learn] logged {qi + 1}/{len(queries)} queries, {len(Y)} tuples")
-    if work_limit:
-        ctx.set_work_limit(0)
-    del old_limit
-    return (np.array(F, np.int32).reshape(-1, GEM_FEAT_DIM), np.array(Y, np.float32), np.array(QI),
-            np.array(CI), np.array(CNT, np.int64))
+            NV.append(q.num_v)
+        if log:
+            log(f"[learn] ran {min(b + group, len(pairs))}/{len(pairs)} (query, order) pairs")
+    return {"feats": np.array(F, np.int32).reshape(-1, GEM_FEAT_DIM), "work": np.array(W, np.float64),
+            "limited": np.array(LIM, bool), "level_rows": np.array(LR, np.uint64).reshape(-1, GEM_MAX_QV),
+            "count": np.array(CNT, np.int64), "query": np.array(Q, np.int64), "cand": np.array(Ci, np.int64),
+            "nv": np.array(NV, np.int64)}
 
 
-def choose(model, feats, qidx, margin=0.0):
-    """Per query, the candidate with the lowest predicted cost, if it beats the
-    query's first candidate (GEM's order) by more than `margin` (model output
-    units); else the first candidate."""
-    pred = model.predict(feats)
-    first, best = {}, {}
-    for i, (q, p) in enumerate(zip(qidx, pred)):
-        if q not in first:
-            first[q] = (i, p)
-        if q not in best or p < best[q][1]:
-            best[q] = (i, p)
-    return {q: (best[q][0] if best[q][1] < first[q][1] - margin else first[q][0]) for q in best}
+def relative_targets(work, query):
+    """Per tuple: log2(1 + work) minus the least value over its query's tuples."""
+    y = np.log2(1.0 + np.asarray(work, np.float64))
+    best = {}
+    for q, v in zip(query, y):
+        best[q] = min(best.get(q, np.inf), v)
+    return (y - np.array([best[q] for q in query])).astype(np.float32)
+
+
+def rank_orders(pred, query, cand, margin=0.0):
+    """Per query: candidate indices by predicted cost (ascending, ties by
+    index); GEM's order (candidate 0) stays first unless the cheapest
+    prediction beats it by more than `margin` (model output units).  The
+    rule gem_hip_match_batch_learned applies."""
+    out = {}
+    for q in np.unique(query):
+        m = np.flatnonzero(query == q)
+        c, p = cand[m], pred[m]
+        r = [int(c[i]) for i in np.argsort(p, kind="stable")]
+        p0 = p[c == 0][0]
+        if not p.min() < p0 - margin:
+            r.remove(0)
+            r.insert(0, 0)
+        out[int(q)] = r
+    return out
+
+
+def schedule_work(costs, ranking, first_attempt, growth=4, width=1, limit=None):
+    """Work the restart schedule spends on one query: attempt a runs
+    ranking[a % n] and may do first_attempt * growth^(a // width) work; the
+    first attempt whose order costs at most that ends the query.  costs[c] =
+    work of candidate c run to its end (>= limit when the collection stopped
+    it: such an order counts as never ending)."""
+    spent = 0.0
+    n = len(ranking)
+    for a in range(64):
+        c = ranking[a % n]
+        budget = float(first_attempt) * float(growth) ** (a // max(width, 1))
+        w = costs[c]
+        if (limit is None or w < limit) and w <= budget:
+            return spent + w
+        spent += budget
+    return spent
 
 
 def match_batch_learned(ctx, queries, model=None, max_orders=16, cap=0, work_limit=0, level_budget=0, group=0,
@@ -186,4 +233,5 @@ def match_batch_learned(ctx, queries, model=None, max_orders=16, cap=0, work_lim
             for i in range(nq)]
 
 
-__all__ = ["OrderModel", "order_candidates", "collect", "choose", "match_batch_learned", "cost_label", "GemError"]
+__all__ = ["OrderModel", "order_candidates", "collect", "relative_targets", "rank_orders", "schedule_work",
+           "match_batch_learned", "cost_label", "GemError"]

@@ -1,19 +1,23 @@
-"""Train the order-cost MLP on costs the matcher logs itself.
+"""Train the order-cost MLP on tuples the matcher logs itself.
 
-    python -m gem_amd.train_order_model --workload rmat20_learn --out gem_amd/models/order_mlp_rmat20.npz
+    python -m gem_amd.train_order_model --workload rmat22_learn --num-queries 768 \
+        --out gem_amd/models/order_mlp_rmat22.npz
+    python -m gem_amd.train_order_model --workload rmat20_train --num-queries 1280 --orders 16 \
+        --out gem_amd/models/order_mlp_rmat20.npz
 
-For every training query the library proposes candidate vertex orders (GEM's
-order first) with their int32 features; every (query, order) pair is run by
-the batched vertex-at-a-time matcher (many pairs per launch), which logs the
-rows each level produced and the work it did (rows read plus list entries
-walked); the cost label is log2(1 + work).  The MLP is then fit on the GPU
-(full-batch Adam, the library's own forward/backward kernels) on a random 70%
-of the queries, stopped at the step count whose choices do the least work on
-another 15% (checked every --eval-every steps), and evaluated on the last 15%:
-total work of the model's choice vs GEM's order.  With --val-frac 0 every step
-runs on 80% and the other 20% are the evaluation set (the shipped rmat22 model:
---val-frac 0 --iters 600 --lr 2e-3 on the 256 default queries).  The weights, the label
-normalisation and the evaluation are saved to an .npz file.
+For every training query the library proposes candidate vertex orders with
+their int32 features; the batched vertex-at-a-time matcher runs every (query,
+order) pair at the benchmark's count cap (1e9) with a work limit, and logs the
+rows of each level and the work done (gem_amd.learned.collect).  Labels are
+log2(1 + work) relative to the query's cheapest order.  The MLP is fit on the
+GPU (full-batch Adam, the library's own forward/backward kernels) on 70% of
+the queries; the step count is the one whose rankings make the restart
+schedule (gem_amd.learned.schedule_work, the rule the batched path applies)
+spend the least work on another 15%; the last 15% are the held-out
+evaluation: work of the schedule with the model's ranking against the
+candidates' generation order (GEM's first) and against the best candidate of
+each query.  The weights, label normalisation, schedule and evaluation are
+saved to an .npz file; --cache keeps the collected tuples.
 """
 import argparse
 import json
@@ -24,8 +28,7 @@ import time
 import numpy as np
 
 from . import GemContext
-from ._lib import GEM_FEAT_DIM
-from .learned import OrderModel, cost_label, order_candidates
+from .learned import OrderModel, collect, rank_orders, relative_targets, schedule_work
 from .workloads import CAP, make_graph, make_queries
 
 
@@ -33,130 +36,108 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def collect_batched(ctx, queries, max_orders, cap, work_limit, group, seed=0):
-    """(features, costs, query index, candidate index, counts) for every
-    candidate order of every query, run through the batched path."""
-    entries, feats, qidx, cidx = [], [], [], []
-    for qi, q in enumerate(queries):
-        ctx.filter(q, want_lists=False)
-        ctx.build_edge_tables()
-        orders, f = order_candidates(ctx, q, max_orders, seed + qi)
-        for ci in range(len(orders)):
-            entries.append((q, orders[ci]))
-            feats.append(f[ci])
-            qidx.append(qi)
-            cidx.append(ci)
-    log(f"[train] {len(entries)} (query, order) pairs from {len(queries)} queries")
-    costs, counts = [], []
-    t = time.perf_counter()
-    for b in range(0, len(entries), group):
-        part = entries[b:b + group]
-        res = ctx.match_batch([e[0] for e in part], vertex_orders=[e[1] for e in part], cap=cap,
-                              work_limit=work_limit, group=group)
-        for r in res:
-            costs.append(cost_label(r["work"]))
-            counts.append(-1 if r["work_limited"] else r["count"])
-        log(f"[train]   ran {min(b + group, len(entries))}/{len(entries)} pairs, {time.perf_counter() - t:.1f}s")
-    return (np.array(feats, np.int32).reshape(-1, GEM_FEAT_DIM), np.array(costs, np.float32), np.array(qidx),
-            np.array(cidx), np.array(counts, np.int64))
-
-
-def evaluate(model, F, Y, Q, C, mu, sd, queries_sel, margin=0.0):
-    """Total work (2^label) over the selected queries of GEM's order, of the
-    model's choice (GEM's order unless a candidate is predicted cheaper by more
-    than `margin` log2 units) and of the best candidate."""
-    pred = model.predict(F) * sd + mu
-    gem = learned = best = 0.0
-    for q in queries_sel:
+def schedule_total(T, pred, queries, first, growth, width, limit, margin=0.0):
+    """Work of the restart schedule over `queries`: ranked by `pred` (None =
+    generation order), by the true costs ("best": each query's cheapest
+    candidate first) or as given."""
+    tot = 0.0
+    sel = np.isin(T["query"], list(queries))
+    Q, Cc, W = T["query"][sel], T["cand"][sel], T["work"][sel]
+    lim = np.where(T["limited"][sel], np.inf, W)
+    if isinstance(pred, str) and pred == "best":
+        rk = {int(q): [int(c) for c in Cc[Q == q][np.argsort(lim[Q == q], kind="stable")]] for q in np.unique(Q)}
+    elif pred is None:
+        rk = {int(q): sorted(int(c) for c in Cc[Q == q]) for q in np.unique(Q)}
+    else:
+        rk = rank_orders(pred[sel], Q, Cc, margin)
+    for q in np.unique(Q):
         m = Q == q
-        idx = np.flatnonzero(m)
-        g0 = idx[C[idx] == 0][0]
-        pick = idx[np.argmin(pred[idx])]
-        if not pred[pick] < pred[g0] - margin:
-            pick = g0
-        gem += 2.0 ** Y[g0]
-        learned += 2.0 ** Y[pick]
-        best += 2.0 ** Y[idx].min()
-    return {"gem_work": float(gem), "learned_work": float(learned), "best_work": float(best),
-            "gain_vs_gem": float(gem / learned) if learned else 0.0}
+        costs = dict(zip(Cc[m].tolist(), lim[m].tolist()))
+        tot += schedule_work(costs, rk[int(q)], first, growth, width)
+    return tot
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="rmat20_learn")
-    ap.add_argument("--queries", type=int, default=0, help="use only the first N queries")
-    ap.add_argument("--num-queries", type=int, default=0, help="queries generated for the workload (0 = its default)")
-    ap.add_argument("--eval-every", type=int, default=50, help="Adam steps between validation checks")
-    ap.add_argument("--val-frac", type=float, default=0.15,
-                    help="share of the queries that picks the step count (0: train all --iters steps on 80%%, "
-                         "evaluate on 20%%)")
-    ap.add_argument("--orders", type=int, default=16)
-    # the labels only have to rank a query's orders: a lower cap and work limit
-    # than the benchmark's keep the collection cheap
-    ap.add_argument("--cap", type=float, default=CAP / 100)
-    ap.add_argument("--work-limit", type=float, default=3e8)
-    ap.add_argument("--group", type=int, default=32)
+    ap.add_argument("--workload", default="rmat22_learn")
+    ap.add_argument("--num-queries", type=int, default=0, help="queries generated (0 = the workload's default)")
+    ap.add_argument("--orders", type=int, default=8, help="candidate orders per query")
+    ap.add_argument("--cap", type=float, default=CAP)
+    ap.add_argument("--work-limit", type=float, default=1e10, help="work after which a (query, order) run stops")
+    ap.add_argument("--group", type=int, default=64, help="(query, order) pairs per batched run")
     ap.add_argument("--cache", default=None, help=".npz of collected tuples: loaded if present, else written")
-    ap.add_argument("--iters", type=int, default=1000, help="most Adam steps")
+    ap.add_argument("--iters", type=int, default=1500, help="most Adam steps")
+    ap.add_argument("--eval-every", type=int, default=50)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--margin", type=float, default=1.0,
-                    help="log2 units by which a candidate must beat GEM's predicted cost to replace it")
-    ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "models", "order_mlp_rmat20.npz"))
+    ap.add_argument("--first-attempt", type=float, default=1e8, help="restart schedule the model is judged with")
+    ap.add_argument("--growth", type=int, default=4)
+    ap.add_argument("--width", type=int, default=1)
+    ap.add_argument("--margin", type=float, default=0.0,
+                    help="log2 units by which a candidate must beat GEM's predicted cost to be ranked first")
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "models", "order_mlp_rmat22.npz"))
     args = ap.parse_args()
 
     g = make_graph(args.workload)
     qs = make_queries(args.workload, g, args.num_queries or None)
-    if args.queries:
-        qs = qs[:args.queries]
     ctx = GemContext(0)
     ctx.load_data(g)
     if args.cache and os.path.exists(args.cache):
         z = np.load(args.cache, allow_pickle=False)
-        F, Y, Q, C, CNT = z["F"], z["Y"], z["Q"], z["C"], z["CNT"]
-        log(f"[train] loaded {len(Y)} tuples from {args.cache}")
+        T = {k: z[k] for k in z.files}
+        log(f"[train] loaded {len(T['work'])} tuples from {args.cache}")
     else:
-        F, Y, Q, C, CNT = collect_batched(ctx, qs, args.orders, int(args.cap), int(args.work_limit), args.group)
+        t = time.perf_counter()
+        T = collect(ctx, qs, args.orders, int(args.cap), int(args.work_limit), args.group, log=log)
+        log(f"[train] {len(T['work'])} tuples from {len(qs)} queries in {time.perf_counter() - t:.0f}s")
         if args.cache:
-            np.savez(args.cache, F=F, Y=Y, Q=Q, C=C, CNT=CNT)
+            np.savez(args.cache, **T)
+    lim = float(args.work_limit)
+    # counts agree across all finished orders of each query
+    consistent = all(len(set(T["count"][(T["query"] == q) & (T["count"] >= 0)].tolist())) <= 1
+                     for q in np.unique(T["query"]))
+    Y = relative_targets(np.minimum(T["work"], lim), T["query"])
     rng = np.random.default_rng(args.seed)
     perm = rng.permutation(len(qs))
-    nva = int(args.val_frac * len(qs))
-    ntr = int(0.7 * len(qs)) if nva else int(0.8 * len(qs))
-    tr_q, va_q, te_q = set(perm[:ntr].tolist()), perm[ntr:ntr + nva].tolist(), perm[ntr + nva:].tolist()
-    tr = np.array([q in tr_q for q in Q])
+    ntr, nva = int(0.7 * len(qs)), int(0.15 * len(qs))
+    tr_q, va_q, te_q = set(perm[:ntr].tolist()), set(perm[ntr:ntr + nva].tolist()), set(perm[ntr + nva:].tolist())
+    tr = np.isin(T["query"], list(tr_q))
     mu, sd = float(Y[tr].mean()), float(Y[tr].std() + 1e-6)
+    sched = (args.first_attempt, args.growth, args.width, lim)
     model = OrderModel(ctx, seed=args.seed)
     t = time.perf_counter()
     losses, best, best_step, step = [], None, 0, 0
     while step < args.iters:
         n = min(args.eval_every, args.iters - step)
-        losses += model.train(F[tr], (Y[tr] - mu) / sd, iters=n, lr=args.lr).tolist()
+        losses += model.train(T["feats"][tr], (Y[tr] - mu) / sd, iters=n, lr=args.lr).tolist()
         step += n
-        va = evaluate(model, F, Y, Q, C, mu, sd, va_q, args.margin)["learned_work"] if va_q else -step
+        va = schedule_total(T, model.predict(T["feats"]) * sd + mu, va_q, *sched, margin=args.margin)
         if best is None or va < best[0]:
             best = (va, model.get_params())
             best_step = step
     model.set_params(best[1])
-    log(f"[train] {step} Adam steps on {int(tr.sum())} tuples in {time.perf_counter() - t:.2f}s, "
-        f"loss {losses[0]:.4f} -> {losses[-1]:.4f}; kept step {best_step} (least validation work)")
-    ev = evaluate(model, F, Y, Q, C, mu, sd, te_q, args.margin)
-    ev_train = evaluate(model, F, Y, Q, C, mu, sd, sorted(tr_q), args.margin)
-    ev["no_margin"] = evaluate(model, F, Y, Q, C, mu, sd, te_q, 0.0)
-    if va_q:
-        ev["validation"] = evaluate(model, F, Y, Q, C, mu, sd, va_q, args.margin)
-    ev["step"] = best_step
-    log(f"[train] held-out: {json.dumps(ev)}")
-    losses = np.array(losses, np.float32)
-    # counts agree across all finished orders of each query
-    consistent = all(len(set(CNT[(Q == q) & (CNT >= 0)].tolist())) <= 1 for q in range(len(qs)))
+    log(f"[train] {step} Adam steps on {int(tr.sum())} tuples in {time.perf_counter() - t:.1f}s, loss "
+        f"{losses[0]:.4f} -> {losses[-1]:.4f}; kept step {best_step} (least validation schedule work)")
+    pred = model.predict(T["feats"]) * sd + mu
+
+    def evaluate(sel):
+        learned = schedule_total(T, pred, sel, *sched, margin=args.margin)
+        gen = schedule_total(T, None, sel, *sched)
+        bestw = schedule_total(T, "best", sel, *sched)
+        return {"queries": len(sel), "learned_work": learned, "generation_order_work": gen, "best_first_work": bestw,
+                "gain": gen / learned if learned else 0.0}
+
+    ev = {"heldout": evaluate(te_q), "validation": evaluate(va_q), "train": evaluate(tr_q), "step": best_step,
+          "schedule": {"first_attempt": args.first_attempt, "growth": args.growth, "width": args.width}}
+    log(f"[train] {json.dumps(ev)}")
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
-    np.savez(args.out, params=model.get_params(), mu=mu, sd=sd, in_dim=GEM_FEAT_DIM, hidden=model.hidden,
-             workload=args.workload, tuples=len(Y), queries=len(qs), losses=losses, margin=args.margin,
-             heldout=json.dumps(ev), train=json.dumps(ev_train), counts_consistent=consistent)
-    print(json.dumps({"tuples": int(len(Y)), "queries": len(qs), "heldout": ev, "train": ev_train,
-                      "loss": [float(losses[0]), float(losses[-1])], "counts_consistent": bool(consistent),
-                      "out": args.out}))
+    np.savez(args.out, params=model.get_params(), mu=mu, sd=sd, in_dim=model.in_dim, hidden=model.hidden,
+             workload=args.workload, tuples=len(Y), queries=len(qs), losses=np.array(losses, np.float32),
+             margin=args.margin, cap=args.cap, work_limit=lim, heldout=json.dumps(ev["heldout"]),
+             validation=json.dumps(ev["validation"]), train=json.dumps(ev["train"]), step=best_step,
+             schedule=json.dumps(ev["schedule"]), counts_consistent=consistent)
+    print(json.dumps({"tuples": int(len(Y)), "queries": len(qs), **ev, "loss": [losses[0], losses[-1]],
+                      "counts_consistent": bool(consistent), "out": args.out}))
     ctx.close()
 
 

@@ -8,8 +8,11 @@ the parity tests.
          (average degree >= 3) and 50 sparse, by rejection sampling.
   rmat22 R-MAT 2^22 vertices, average degree 16, 32 Zipf labels; 512
          random-walk queries with sizes uniform in [4, 32], mixed density.
-  rmat20_learn  the rmat20 graph with 256 queries of 8-24 vertices.
-  rmat22_learn  the rmat22 graph with 256 other queries of 4-32 vertices
+  rmat20_learn  the rmat20 graph with 256 queries of 8-24 vertices (the
+         learned-order workload; held out from its model's training).
+  rmat20_train  the rmat20 graph with other queries of 8-24 vertices
+         (training set of the order model used on rmat20 / rmat20_learn).
+  rmat22_learn  the rmat22 graph with other queries of 4-32 vertices
          (training set of the order model used on rmat22).
   ooc24  R-MAT 2^24 vertices, average degree 64, 64 Zipf labels; 32 sparse
          (average degree < 3) random-walk 12-vertex queries: the out-of-core
@@ -25,6 +28,7 @@ WORKLOADS = {
     "rmat20": dict(kind="rmat", scale=20, degree=16.0, labels=16, seed=1002),
     "rmat22": dict(kind="rmat", scale=22, degree=16.0, labels=32, seed=1003),
     "rmat20_learn": dict(kind="rmat", scale=20, degree=16.0, labels=16, seed=1002),
+    "rmat20_train": dict(kind="rmat", scale=20, degree=16.0, labels=16, seed=1002, qseed=2002),
     "rmat22_learn": dict(kind="rmat", scale=22, degree=16.0, labels=32, seed=1003, qseed=2003),
     "ooc24": dict(kind="rmat", scale=24, degree=64.0, labels=64, seed=1005),
 }

@@ -410,7 +410,9 @@ int gem_mlp_train(gem_mlp *m, const int32_t *feats, const float *targets, uint32
 /* For the current query (after gem_hip_filter and gem_hip_build_edge_tables):
  * up to max_orders candidate vertex orders (orders[max_orders * num_v], the
  * GEM-derived order first) and their int32 features
- * (feats[max_orders * GEM_FEAT_DIM]); *n receives the number produced. */
+ * (feats[max_orders * GEM_FEAT_DIM]); *n receives the number produced.  seed
+ * 0 seeds the random candidates with the query's hash, as the batched learned
+ * path does. */
 int gem_hip_order_candidates(gem_ctx *ctx, uint32_t max_orders, uint64_t seed, uint32_t *orders, int32_t *feats,
                              uint32_t *n);
 

@@ -139,31 +139,29 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
 
 
-def test_order_model_evaluation_rule():
-    """The trainer's evaluation: per query, the cheapest predicted candidate
-    replaces GEM's order (candidate 0) only when predicted cheaper by more than
-    the margin; work is 2^label."""
-    from gem_amd.train_order_model import evaluate
+def test_order_ranking_and_restart_schedule():
+    """The ranking rule the batched path applies (GEM's order, candidate 0,
+    stays first unless a candidate is predicted cheaper by more than the
+    margin), the restart schedule's work (attempt a runs the (a mod n)-th
+    ranked order with a limit of first * growth^(a // width)), and the
+    relative training labels, on hand-worked cases."""
+    from gem_amd.learned import rank_orders, relative_targets, schedule_work
 
-    class Fixed:
-        def __init__(self, p):
-            self.p = np.asarray(p, np.float32)
-
-        def predict(self, F):
-            return self.p
-
-    # query 0: candidates (gem, a, b), query 1: (gem, c)
     Q = np.array([0, 0, 0, 1, 1])
     C = np.array([0, 1, 2, 0, 1])
-    Y = np.array([10.0, 8.0, 12.0, 6.0, 5.0], np.float32)
-    F = np.zeros((5, 1), np.int32)
-    pred = [10.0, 7.5, 11.0, 6.0, 5.8]  # model output is de-normalised as pred * sd + mu
-    ev = evaluate(Fixed(pred), F, Y, Q, C, 0.0, 1.0, [0, 1], margin=1.0)
-    # query 0: 7.5 < 10 - 1 -> candidate 1 (2^8); query 1: 5.8 is not < 6 - 1 -> GEM's (2^6)
-    assert ev["gem_work"] == 2 ** 10 + 2 ** 6 and ev["learned_work"] == 2 ** 8 + 2 ** 6
-    assert ev["best_work"] == 2 ** 8 + 2 ** 5
-    ev0 = evaluate(Fixed(pred), F, Y, Q, C, 0.0, 1.0, [0, 1], margin=0.0)
-    assert ev0["learned_work"] == 2 ** 8 + 2 ** 5
+    pred = np.array([10.0, 7.5, 11.0, 6.0, 5.8])
+    # query 0: 7.5 < 10 - 1 -> candidate 1 first; query 1: 5.8 is not < 6 - 1 -> GEM's first
+    assert rank_orders(pred, Q, C, margin=1.0) == {0: [1, 0, 2], 1: [0, 1]}
+    assert rank_orders(pred, Q, C, margin=0.0) == {0: [1, 0, 2], 1: [1, 0]}
+    # costs: order 0 needs 50, order 1 needs 5, order 2 never ends
+    costs = {0: 50.0, 1: 5.0, 2: float("inf")}
+    assert schedule_work(costs, [1, 0, 2], 10) == 5.0                 # first attempt ends
+    assert schedule_work(costs, [2, 0, 1], 10, growth=4) == 10 + 40 + 5  # limits 10, 40, then order 1
+    assert schedule_work(costs, [0, 2, 1], 10, growth=4, width=3) == 10 + 10 + 5
+    assert schedule_work(costs, [2, 0], 10, growth=4) == 10 + 40 + 160 + 50  # 0 ends at limit 640
+    # labels: log2(1 + work) relative to the query's cheapest order
+    y = relative_targets(np.array([3.0, 15.0, 1.0, 7.0]), np.array([0, 0, 1, 1]))
+    assert np.allclose(y, [0.0, 2.0, 0.0, 2.0])
 
 
 def test_profiles_cover_the_walk_kernels():
