@@ -39,9 +39,11 @@ def log(*a):
 def schedule_total(T, pred, queries, first, growth, width, limit, margin=0.0):
     """Work of the restart schedule over `queries`: ranked by `pred` (None =
     generation order), by the true costs ("best": each query's cheapest
-    candidate first) or as given."""
+    candidate first) or as given.  Queries none of whose candidates ended
+    within the collection's work limit say nothing about the ranking and are
+    left out."""
     tot = 0.0
-    sel = np.isin(T["query"], list(queries))
+    sel = np.isin(T["query"], [q for q in queries if not T["limited"][T["query"] == q].all()])
     Q, Cc, W = T["query"][sel], T["cand"][sel], T["work"][sel]
     lim = np.where(T["limited"][sel], np.inf, W)
     if isinstance(pred, str) and pred == "best":
@@ -124,8 +126,9 @@ def main():
         learned = schedule_total(T, pred, sel, *sched, margin=args.margin)
         gen = schedule_total(T, None, sel, *sched)
         bestw = schedule_total(T, "best", sel, *sched)
-        return {"queries": len(sel), "learned_work": learned, "generation_order_work": gen, "best_first_work": bestw,
-                "gain": gen / learned if learned else 0.0}
+        unranked = sum(1 for q in sel if T["limited"][T["query"] == q].all())
+        return {"queries": len(sel), "queries_no_order_ended": unranked, "learned_work": learned,
+                "generation_order_work": gen, "best_first_work": bestw, "gain": gen / learned if learned else 0.0}
 
     ev = {"heldout": evaluate(te_q), "validation": evaluate(va_q), "train": evaluate(tr_q), "step": best_step,
           "schedule": {"first_attempt": args.first_attempt, "growth": args.growth, "width": args.width}}

@@ -118,8 +118,7 @@ def test_shipped_model_gains_on_heldout_workload(gpu_ctx):
     qs = make_queries("rmat20_learn", g)
     gpu_ctx.load_data(g)
     model = OrderModel.load(gpu_ctx, path)
-    if model.meta["workload"] != "rmat20_train":
-        pytest.skip("the shipped rmat20 model predates the rmat20_train queries")
+    assert model.meta["workload"] == "rmat20_train"
     kw = dict(max_orders=8, cap=10 ** 9, first_attempt=10 ** 8)
     base = match_batch_learned(gpu_ctx, qs, None, **kw)
     lrn = match_batch_learned(gpu_ctx, qs, model, **kw)
