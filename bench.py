@@ -125,9 +125,10 @@ def main():
                          "the query starts again (0 = no restarts)")
     ap.add_argument("--restart-growth", type=int, default=4)
     ap.add_argument("--restart-width", type=int, default=1)
-    ap.add_argument("--work-limit", type=float, default=0,
-                    help="total work over a batched query's attempts after which it is reported unfinished "
-                         "(0 = none)")
+    ap.add_argument("--work-limit", type=float, default=1e12,
+                    help="total work over a batched query's attempts after which it is reported unfinished (0 = "
+                         "none; the default is a safety net about a hundred times what the costliest query of the "
+                         "512-query batch needs)")
     ap.add_argument("--plain-work-limit", type=float, default=5e9,
                     help="rows the plain path's steps may read per query before it is reported unfinished")
     ap.add_argument("--group", type=int, default=None,

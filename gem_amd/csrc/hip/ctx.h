@@ -392,6 +392,13 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
 // Exclusive scan of n u64 values in place-free form: out[i] = sum(in[0..i)),
 // out[n] = total.  Returns nothing; caller reads out[n] if needed.
 void scan_u64(gem_ctx *c, const uint64_t *d_in, uint64_t *d_out, uint64_t n);
+// Independent in-place exclusive scans of u64 arrays p[0..n) (the total goes
+// to p[n] and to d_totals[i]); d_segs is the device copy of h_segs.
+struct ScanSeg {
+  uint64_t *p;
+  uint64_t n;
+};
+void scan_many_u64(gem_ctx *c, const ScanSeg *d_segs, const ScanSeg *h_segs, uint32_t nseg, uint64_t *d_totals);
 // Segmented exclusive scan of u32 counts: nseg segments of length len each,
 // out has len + 1 entries per segment (last = segment total).
 void scan_segments_u32(gem_ctx *c, const uint32_t *d_in, uint32_t *d_out, uint32_t nseg, uint64_t len);

@@ -88,6 +88,53 @@ void scan_u64(gem_ctx *c, const uint64_t *d_in, uint64_t *d_out, uint64_t n) {
   GEM_HIP_CHECK(hipMemcpyAsync(d_out + n, base.as<uint64_t>() + tiles, 8, hipMemcpyDeviceToDevice, c->stream));
 }
 
+// Many independent in-place scans in one launch: block b scans
+// segs[b].p[0..n) (u64, exclusive), stores the total at segs[b].p[n] and in
+// totals[b].  Segments longer than the library's one-block limit are left to
+// scan_u64 (their total is gathered by k_gather_totals).
+__global__ void __launch_bounds__(1024) k_scan_many_u64(const ScanSeg *__restrict__ segs, uint64_t small_max,
+                                                       uint64_t *__restrict__ totals) {
+  __shared__ uint64_t sm[16];
+  const ScanSeg sg = segs[blockIdx.x];
+  if (sg.n > small_max) return;
+  uint64_t carry = 0;
+  for (uint64_t b = 0; b < sg.n; b += blockDim.x) {
+    const uint64_t idx = b + threadIdx.x;
+    const uint64_t x = idx < sg.n ? sg.p[idx] : 0;
+    uint64_t ex;
+    const uint64_t tot = block_excl_scan_u64(x, &ex, sm);
+    if (idx < sg.n) sg.p[idx] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    sg.p[sg.n] = carry;
+    totals[blockIdx.x] = carry;
+  }
+}
+
+__global__ void k_gather_totals(const ScanSeg *__restrict__ segs, uint32_t nseg, uint64_t small_max,
+                                uint64_t *__restrict__ totals) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nseg && segs[b].n > small_max) totals[b] = segs[b].p[segs[b].n];
+}
+
+void scan_many_u64(gem_ctx *c, const ScanSeg *d_segs, const ScanSeg *h_segs, uint32_t nseg, uint64_t *d_totals) {
+  if (!nseg) return;
+  constexpr uint64_t SMALL = 4 * 1024 * 16;  // scan_u64's one-block limit
+  k_scan_many_u64<<<nseg, 1024, 0, c->stream>>>(d_segs, SMALL, d_totals);
+  GEM_LAUNCH_CHECK("k_scan_many_u64");
+  bool big = false;
+  for (uint32_t b = 0; b < nseg; ++b)
+    if (h_segs[b].n > SMALL) {
+      scan_u64(c, h_segs[b].p, h_segs[b].p, h_segs[b].n);
+      big = true;
+    }
+  if (big) {
+    k_gather_totals<<<(nseg + 255) / 256, 256, 0, c->stream>>>(d_segs, nseg, SMALL, d_totals);
+    GEM_LAUNCH_CHECK("k_gather_totals");
+  }
+}
+
 // One block per segment: exclusive scan of len u32 counts, out[len] = total.
 __global__ void __launch_bounds__(1024) k_scan_segments_u32(const uint32_t *in, uint32_t *out, uint64_t len) {
   __shared__ uint64_t sm[16];

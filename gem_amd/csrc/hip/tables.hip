@@ -366,15 +366,23 @@ void run_tables(gem_ctx *c, QueryDev &Qd) {
         ccnt.as<uint32_t>(), ndmax);
     GEM_LAUNCH_CHECK("k_theavy_bases");
   }
+  // row counts -> offsets for every table in one launch (the larger ones by
+  // the multi-block scan), then the table sizes and adjacency lengths in one
+  // copy back
   const size_t nt = desc_tab.size();
-  uint64_t *h = pinned<uint64_t>(c, nt + GEM_MAX_QV);
+  std::vector<ScanSeg> segs(nt);
   for (size_t k = 0; k < nt; ++k) {
     TableDev &T = Qd.tab[desc_tab[k]];
-    const uint64_t nrow = Qd.ncan[T.key_u];
-    scan_u64(c, T.off.as<uint64_t>(), T.off.as<uint64_t>(), nrow);
-    GEM_HIP_CHECK(hipMemcpyAsync(h + k, T.off.as<uint64_t>() + nrow, 8, hipMemcpyDeviceToHost, s));
+    segs[k] = ScanSeg{T.off.as<uint64_t>(), (uint64_t)Qd.ncan[T.key_u]};
   }
-  GEM_HIP_CHECK(hipMemcpyAsync(h + nt, adj_len.p, GEM_MAX_QV * 8, hipMemcpyDeviceToHost, s));
+  DevBuf dsegs, dtot;
+  dsegs.alloc(std::max<size_t>(nt, 1) * sizeof(ScanSeg), s);
+  dtot.alloc((nt + GEM_MAX_QV) * 8, s);
+  if (nt) GEM_HIP_CHECK(hipMemcpyAsync(dsegs.p, segs.data(), nt * sizeof(ScanSeg), hipMemcpyHostToDevice, s));
+  scan_many_u64(c, dsegs.as<ScanSeg>(), segs.data(), (uint32_t)nt, dtot.as<uint64_t>());
+  GEM_HIP_CHECK(hipMemcpyAsync(dtot.as<uint64_t>() + nt, adj_len.p, GEM_MAX_QV * 8, hipMemcpyDeviceToDevice, s));
+  uint64_t *h = pinned<uint64_t>(c, nt + GEM_MAX_QV);
+  GEM_HIP_CHECK(hipMemcpyAsync(h, dtot.p, (nt + GEM_MAX_QV) * 8, hipMemcpyDeviceToHost, s));
   GEM_HIP_CHECK(hipStreamSynchronize(s));
   // algorithmic bytes: per keyed u, its candidates (4 B), offset pairs (8 B)
   // and adjacency (4 B per slot) once for all its tables; per table the value

@@ -55,7 +55,9 @@ struct VQueryState {
   gemh::SQuery<DevBuf> s;
   // Restarts (learned batched path): the vertex orders in the order they are
   // tried and their candidate indices; ranked[cur] is the one running.
-  // Attempt a runs ranked[a % size] and may add
+  // Attempt a runs ranked[a] while a < size, then the order whose ended
+  // attempts counted the most embeddings per unit of work (ranked[a % size]
+  // while none counted any), and may add
   // first_attempt * growth^(a / width) work (first_attempt == 0: no restarts;
   // attempt max_attempts - 1, when max_attempts != 0, has no limit).
   // mnb_all / mnd_all: the most backward lists and equal-label columns of a
@@ -66,6 +68,7 @@ struct VQueryState {
   uint64_t first_attempt = 0;
   uint32_t growth = 4, width = 1;
   uint32_t max_attempts = 0;
+  std::vector<double> rate;  // per ranked order: most embeddings per unit of work in its ended attempts
   uint32_t mnb_all = 1, mnd_all = 0;
 };
 

@@ -44,6 +44,9 @@ namespace gemd {
 
 constexpr uint32_t VB = 256;  // rows per task block
 constexpr uint32_t VW = VB / 64;
+// survivors a wave stages in LDS before it reserves output rows for them (a
+// step adds at most 128)
+constexpr uint32_t VSTAGE = 512;
 
 __device__ __forceinline__ uint32_t vposition_in(const uint64_t *bm, const uint32_t *rank, uint32_t v) {
   const uint64_t word = ldg(bm + (v >> 6));
@@ -137,8 +140,8 @@ k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p, c
   const uint32_t nitems = *nitems_p;
   __shared__ uint32_t st[VW][65];
   __shared__ uint64_t drv[VW][64];  // address of each row's driving list
-  __shared__ uint8_t sv_row[VW][128];    // this step's survivors: row in the task
-  __shared__ uint32_t sv_val[VW][128];  // and new value
+  __shared__ uint8_t sv_row[VW][VSTAGE];    // staged survivors of the task: row in the task
+  __shared__ uint32_t sv_val[VW][VSTAGE];  // and new value
   extern __shared__ unsigned char vlds[];
   const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
   const uint32_t per_wave = 64 * ((max_nb > 0 ? max_nb - 1 : 0) * 12 + max_ndup * 4);
@@ -217,6 +220,25 @@ k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p, c
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     walked += wtotal;
     uint64_t survivors = 0;
+    uint32_t pend = 0;  // survivors staged in LDS, not yet written
+    // write the staged survivors: one reservation for all of them, then
+    // their rows as one run of consecutive words (word q is column q % (k+1)
+    // of survivor q / (k+1): k parent columns, then the value)
+    auto flush = [&]() {
+      uint64_t base = 0;
+      if (l == 0) base = atomicAdd(it.counter, (unsigned long long)pend);
+      base = readlane_u64(base, 0);
+      written += (uint64_t)pend * 4 * (k + 1);
+      const uint32_t rl = k + 1;
+      for (uint32_t q = l; q < pend * rl; q += 64) {
+        const uint32_t i = q / rl, c = q - i * rl;
+        if (base + i < it.out_cap)
+          stg(it.out + (base + i) * rl + c, c < k ? ldg(it.rows + (rbase + sv_row[w][i]) * k + c) : sv_val[w][i]);
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      pend = 0;
+    };
     // 128 items per step, two per lane (items s + l and s + 64 + l), so that
     // each lane has two loads in flight at every stage of the probes
     for (uint32_t s = 0; s < wtotal; s += 128) {
@@ -238,35 +260,25 @@ k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p, c
       if (FINAL) {
         survivors += nk;
       } else if (nk) {
-        uint64_t base = 0;
-        if (l == 0) base = atomicAdd(it.counter, (unsigned long long)nk);
-        base = readlane_u64(base, 0);
-        written += (uint64_t)nk * 4 * (k + 1);
+        // stage this step's survivors after the pending ones; the staging
+        // area is written out when the next step might not fit
         if (keep0) {
-          const uint32_t i = rank_below(m0);
+          const uint32_t i = pend + rank_below(m0);
           sv_row[w][i] = rw0;
           sv_val[w][i] = val0;
         }
         if (keep1) {
-          const uint32_t i = nk0 + rank_below(m1);
+          const uint32_t i = pend + nk0 + rank_below(m1);
           sv_row[w][i] = rw1;
           sv_val[w][i] = val1;
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        // the nk new rows are nk * (k + 1) consecutive output words, written
-        // by consecutive lanes: word q is column q % (k + 1) of survivor
-        // q / (k + 1) (k parent columns, then the value)
-        const uint32_t rl = k + 1;
-        for (uint32_t q = l; q < nk * rl; q += 64) {
-          const uint32_t i = q / rl, c = q - i * rl;
-          if (base + i < it.out_cap)
-            stg(it.out + (base + i) * rl + c, c < k ? ldg(it.rows + (rbase + sv_row[w][i]) * k + c) : sv_val[w][i]);
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        pend += nk;
+        if (pend > VSTAGE - 128) flush();
       }
     }
+    if (!FINAL && pend) flush();
     if (FINAL && l == 0 && survivors) atomicAdd(it.counter, (unsigned long long)survivors);
     // st / drv / LDS lists of this wave are rewritten by its next task
     __builtin_amdgcn_wave_barrier();
@@ -1051,7 +1063,20 @@ struct HipSchedBackend {
     if (!attempt_limit(V, 0)) return false;
     const uint32_t a = Q.attempts + 1;
     if (V.max_attempts && a >= V.max_attempts) return false;
-    V.cur = a % (uint32_t)V.ranked.size();
+    // the ended attempt's progress: embeddings counted per unit of work
+    const uint32_t K = (uint32_t)V.ranked.size();
+    if (V.rate.size() != K) V.rate.assign(K, 0.0);
+    const uint64_t spent = Q.work - Q.attempt_start;
+    if (spent) V.rate[V.cur] = std::max(V.rate[V.cur], (double)Q.count / (double)spent);
+    // every ranked order once, then the one that counted fastest (the ranking
+    // again while none has counted anything)
+    V.cur = a % K;
+    if (a >= K) {
+      uint32_t best = 0;
+      for (uint32_t j = 1; j < K; ++j)
+        if (V.rate[j] > V.rate[best]) best = j;
+      if (V.rate[best] > 0) V.cur = best;
+    }
     replan(c, V, V.ranked[V.cur]);
     if (tc) tc->bind(V, qi);
     if (V.mnb_all > slot.max_nb || V.mnd_all > slot.max_ndup)

@@ -8,7 +8,8 @@
 // Memory rule.  Pending level rows of every query ("live") plus the output
 // buffers allocated in an iteration are charged against `budget`.  Each
 // running query has a quota: budget / max_active while queries wait to start,
-// budget / (queries running) once none waits (quotas only grow then).  A
+// budget / (queries running) once none waits, but at most tail_growth times
+// the first quota (quotas only grow then).  A
 // query's next chunk may use (quota - its pending bytes) / (levels it still
 // has to write) bytes, so the levels after it find room too and a deep query
 // keeps descending in sizeable chunks instead of the budget filling up with
@@ -50,6 +51,7 @@ struct SchedConfig {
   // start as running ones end (continuous admission; a query's first level
   // is seeded when it starts)
   uint32_t max_active = 0;
+  uint32_t tail_growth = 8;  // most a quota grows once no query waits (see run_schedule)
 };
 
 struct SchedStats {
@@ -279,10 +281,17 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
     std::vector<uint64_t> fin_bounds;  // output bound of each measured final piece (0: unmeasured)
     std::vector<ChunkCand> cand;
     std::vector<uint32_t> cand_owner;
-    // quotas: fixed while queries wait to start, then the budget over the
-    // queries still running
-    const uint64_t quota = budget / std::max<uint64_t>(next < qs.size() && cfg.max_active ? cfg.max_active
-                                                                                          : act.size(), 1);
+    // quotas: budget / max_active while queries wait to start; then the
+    // budget over the queries still running, at most tail_growth times the
+    // first quota (the last queries take larger chunks, so they do not crawl
+    // through thousands of small iterations, but a capped query given the
+    // whole budget would walk its first levels so broadly that it can do
+    // many times the work it needs to reach the cap)
+    uint64_t shares = act.size();
+    if (cfg.max_active)
+      shares = next < qs.size() ? cfg.max_active
+                                : std::max<uint64_t>(shares, cfg.max_active / std::max<uint32_t>(cfg.tail_growth, 1));
+    const uint64_t quota = budget / std::max<uint64_t>(shares, 1);
     for (uint32_t qi : act) {
       SQuery<Buf> &S = *qs[qi];
       SLevel<Buf> &top = S.stack.back();

@@ -299,9 +299,11 @@ typedef struct gem_batch_opts {
                             of successive rounds of attempts (0 = 4)       */
   uint64_t first_attempt; /* learned path: work limit of the first round's
                             attempts (0 = no restarts).  Attempt a runs the
-                            (a mod n)-th ranked order and may do
+                            a-th ranked order (once all n have run: the one
+                            whose attempts counted the most embeddings per
+                            unit of work) and may do
                             first_attempt * growth^(a / width) work before
-                            the query starts again with the next one      */
+                            the query starts again                        */
   uint32_t restart_width; /* learned path: attempts per round (0 = 1)       */
   uint32_t max_attempts;  /* learned path: attempts per query (0 = no bound;
                             otherwise the last one has no limit)           */
