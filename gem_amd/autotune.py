@@ -55,6 +55,8 @@ def apply(ctx, cfg):
         ctx.set_heavy_split(cfg["heavy_t"], cfg.get("heavy_chunk", 1024))
     if "max_active" in cfg:
         ctx.set_max_active(cfg["max_active"])
+    if "graph" in cfg:
+        ctx.set_graph(bool(cfg["graph"]))
 
 
 def main():
