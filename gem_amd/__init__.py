@@ -4,8 +4,8 @@ The compute path is libgem_hip.so (hand-written HIP kernels behind a C ABI,
 include/gem_hip.h); this package is its Python mirror.
 """
 from ._lib import GEM_MAX_QE, GEM_MAX_QV, GemError, load
-from .graph import Graph, Query, gem_order
+from .graph import Graph, Query, edge_bits, gem_order, relaxed_queries
 from .matcher import NOT_FOUND, GemContext
 
-__all__ = ["GEM_MAX_QE", "GEM_MAX_QV", "GemError", "load", "Graph", "Query", "gem_order", "GemContext",
-           "NOT_FOUND"]
+__all__ = ["GEM_MAX_QE", "GEM_MAX_QV", "GemError", "load", "Graph", "Query", "gem_order", "relaxed_queries",
+           "edge_bits", "GemContext", "NOT_FOUND"]

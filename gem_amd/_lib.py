@@ -85,6 +85,7 @@ SIGNATURES = [
     ("gem_query_load", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p]),
     ("gem_query_save", C.c_int, [C.POINTER(QueryView), C.c_char_p]),
     ("gem_plan_gem_order", C.c_int, [C.POINTER(QueryView), u64p, u32p]),
+    ("gem_plan_relaxed_queries", C.c_int, [C.POINTER(QueryView), C.c_uint32, C.c_uint32, u64p, u32p]),
     ("gem_hip_create", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     ("gem_hip_destroy", None, [C.c_void_p]),
     ("gem_hip_last_error", C.c_char_p, [C.c_void_p]),
@@ -127,7 +128,12 @@ SIGNATURES = [
     ("gem_hip_order_candidates", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, u32p, C.POINTER(C.c_int32), u32p]),
     ("gem_hip_match_batch_learned", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, C.c_void_p,
                                               C.c_uint32, C.POINTER(BatchOpts), C.POINTER(JoinResult), u64p, u32p]),
+    ("gem_hip_match_relaxed", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, C.c_void_p, C.c_uint32,
+                                        C.POINTER(BatchOpts), C.c_uint32, u64p, C.POINTER(JoinResult), u32p]),
 ]
+
+GEM_RQ_WORDS = 4
+GEM_MAX_RQ = 1 << 20
 
 GEM_FEAT_DIM = 272
 

@@ -190,7 +190,6 @@ int gem_hip_filter(gem_ctx *c, const gem_query_view *q, uint32_t threshold, uint
                    uint64_t can_pitch) {
   return guard(c, [&]() {
     if (!h_num_can) throw Error{GEM_E_INVALID, "h_num_can is null"};
-    if (threshold != 0) throw Error{GEM_E_INVALID, "only threshold 0 is supported"};
     run_filter(c, q, threshold);
     const QueryDev &Qd = c->q;
     for (uint32_t u = 0; u < Qd.Q.nv; ++u) h_num_can[u] = Qd.ncan[u];
@@ -209,7 +208,6 @@ int gem_hip_filter(gem_ctx *c, const gem_query_view *q, uint32_t threshold, uint
 int gem_hip_filter_noalloc(gem_ctx *c, const gem_query_view *q, uint32_t threshold, uint32_t *d_num_can,
                            uint32_t *d_can, uint64_t can_pitch) {
   return guard(c, [&]() {
-    if (threshold != 0) throw Error{GEM_E_INVALID, "only threshold 0 is supported"};
     run_filter_noalloc(c, q, threshold, d_num_can, d_can, can_pitch);
   });
 }
@@ -332,6 +330,34 @@ int gem_hip_match_batch_learned(gem_ctx *c, const gem_query_view *queries, uint3
   return guard(c, [&]() {
     if (nq && (!queries || !results)) throw Error{GEM_E_INVALID, "null argument"};
     run_match_batch_learned(c, queries, nq, mlp, max_orders, opts, results, level_rows, chosen);
+  });
+}
+
+int gem_hip_match_relaxed(gem_ctx *c, const gem_query_view *q, uint32_t threshold, gem_mlp *mlp,
+                          uint32_t max_orders, const gem_batch_opts *opts, uint32_t max_rq, uint64_t *masks,
+                          gem_join_result *results, uint32_t *num_rq) {
+  return guard(c, [&]() {
+    if (!q || !masks || !results || !num_rq || !max_rq) throw Error{GEM_E_INVALID, "null argument"};
+    gemh::QueryInfo Q;
+    std::string why;
+    int st = gemh::analyse_query(q, &Q, &why);
+    if (st) throw Error{st, why};
+    std::vector<gemh::EdgeSet> sets;
+    st = gemh::relaxed_edge_sets(Q, threshold, max_rq, &sets);
+    if (st) throw Error{st, "more than max_rq relaxed queries"};
+    // Every relaxed query is a query of its own in one batched run: counts do
+    // not depend on the other relaxed queries, and the batch shares the
+    // persistent launches between them.
+    std::vector<std::vector<uint32_t>> edges(sets.size());
+    std::vector<gem_query_view> views(sets.size());
+    for (size_t i = 0; i < sets.size(); ++i) {
+      gemh::subquery_edges(q, sets[i], &edges[i]);
+      views[i] = gem_query_view{q->num_v, (uint32_t)(edges[i].size() / 2), q->labels, edges[i].data()};
+    }
+    run_match_batch_learned(c, views.data(), (uint32_t)views.size(), mlp, max_orders, opts, results, nullptr,
+                            nullptr);
+    gemh::pack_edge_sets(sets, masks);
+    *num_rq = (uint32_t)sets.size();
   });
 }
 

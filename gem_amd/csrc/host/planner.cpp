@@ -90,6 +90,69 @@ bool edges_connected(const QueryInfo &q, const EdgeSet &s) {
   return true;
 }
 
+bool edges_span(const QueryInfo &q, const EdgeSet &s) {
+  uint32_t parent[GEM_MAX_QV];
+  for (uint32_t v = 0; v < q.nv; ++v) parent[v] = v;
+  auto find = [&](uint32_t x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+  };
+  uint32_t components = q.nv;
+  for (uint32_t i = 0; i < q.ne; ++i) {
+    if (!s[i]) continue;
+    uint32_t a = find(q.edges[i].first), b = find(q.edges[i].second);
+    if (a != b) {
+      parent[a] = b;
+      --components;
+    }
+  }
+  return components <= 1;
+}
+
+// The reference reaches its edge sets breadth-first, one removal at a time,
+// keeping a set only while it spans; a superset of a spanning set spans, so
+// that reaches exactly the spanning sets with at most `threshold` edges left
+// out.  Here they are listed depth-first over ascending left-out tuples, a
+// branch ending as soon as its set stops spanning (every extension of it is a
+// subset and does not span either), then ordered by size.
+int relaxed_edge_sets(const QueryInfo &q, uint32_t threshold, uint64_t max_rq, std::vector<EdgeSet> *out) {
+  std::vector<std::pair<uint32_t, std::vector<uint32_t>>> found;  // (left out, tuple)
+  EdgeSet cur;
+  for (uint32_t i = 0; i < q.ne; ++i) cur.set(i);
+  std::vector<uint32_t> tuple;
+  bool over = false;
+  auto dfs = [&](auto &&self, uint32_t from) -> void {
+    if (over) return;
+    if (found.size() >= max_rq) {
+      over = true;
+      return;
+    }
+    found.push_back({(uint32_t)tuple.size(), tuple});
+    if (tuple.size() >= threshold) return;
+    for (uint32_t e = from; e < q.ne && !over; ++e) {
+      cur.reset(e);
+      if (edges_span(q, cur)) {
+        tuple.push_back(e);
+        self(self, e + 1);
+        tuple.pop_back();
+      }
+      cur.set(e);
+    }
+  };
+  dfs(dfs, 0);
+  if (over) return GEM_E_LIMIT;
+  std::stable_sort(found.begin(), found.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+  out->clear();
+  out->reserve(found.size());
+  for (auto &f : found) {
+    EdgeSet s;
+    for (uint32_t i = 0; i < q.ne; ++i) s.set(i);
+    for (uint32_t e : f.second) s.reset(e);
+    out->push_back(s);
+  }
+  return GEM_OK;
+}
+
 // Greedy (lexicographically smallest) removal sequence from the full edge set
 // down to `target`.
 static std::vector<uint32_t> removal_sequence(const QueryInfo &q, const EdgeSet &target) {
@@ -485,4 +548,18 @@ extern "C" int gem_plan_gem_order(const gem_query_view *q, const uint64_t *table
   int st = gemh::analyse_query(q, &Q, nullptr);
   if (st) return st;
   return gemh::gem_order(Q, table_sizes, order_out);
+}
+
+extern "C" int gem_plan_relaxed_queries(const gem_query_view *q, uint32_t threshold, uint32_t max_rq,
+                                        uint64_t *masks, uint32_t *num_rq) {
+  if (!num_rq || (max_rq && !masks)) return GEM_E_INVALID;
+  gemh::QueryInfo Q;
+  int st = gemh::analyse_query(q, &Q, nullptr);
+  if (st) return st;
+  std::vector<gemh::EdgeSet> sets;
+  st = gemh::relaxed_edge_sets(Q, threshold, max_rq ? max_rq : GEM_MAX_RQ, &sets);
+  if (st) return st;
+  *num_rq = (uint32_t)sets.size();
+  if (max_rq) gemh::pack_edge_sets(sets, masks);
+  return GEM_OK;
 }

@@ -30,6 +30,38 @@ int analyse_query(const gem_query_view *q, QueryInfo *out, std::string *why);
 // in-core/lib/graph/src/cpuGraph.cpp:74-194.)
 bool edges_connected(const QueryInfo &q, const EdgeSet &s);
 
+// Does the edge set `s` connect all q.nv query vertices?  (checkConnectivity,
+// in-core/lib/plan/src/relax.cpp:57-81: one union-find component.)
+bool edges_span(const QueryInfo &q, const EdgeSet &s);
+
+// The relaxed queries of q at `threshold` (relax(), in-core/lib/plan/src/
+// relax.cpp:84-142): every edge subset that leaves out at most `threshold`
+// edges and still connects all vertices, full set first, then by the number
+// left out and, within one number, by the ascending tuple of left-out edge
+// indices.  GEM_E_LIMIT once more than max_rq exist.
+int relaxed_edge_sets(const QueryInfo &q, uint32_t threshold, uint64_t max_rq, std::vector<EdgeSet> *out);
+
+// masks[i * GEM_RQ_WORDS + w] bit b = edge 64 w + b of sets[i].
+inline void pack_edge_sets(const std::vector<EdgeSet> &sets, uint64_t *masks) {
+  for (size_t i = 0; i < sets.size(); ++i)
+    for (uint32_t w = 0; w < GEM_RQ_WORDS; ++w) {
+      uint64_t m = 0;
+      for (uint32_t b = 0; b < 64; ++b)
+        if (sets[i][64 * w + b]) m |= 1ull << b;
+      masks[i * GEM_RQ_WORDS + w] = m;
+    }
+}
+
+// The query view of q restricted to the edges of s (edges in index order).
+inline void subquery_edges(const gem_query_view *q, const EdgeSet &s, std::vector<uint32_t> *edges) {
+  edges->clear();
+  for (uint32_t i = 0; i < q->num_e; ++i)
+    if (s[i]) {
+      edges->push_back(q->edges[2 * i]);
+      edges->push_back(q->edges[2 * i + 1]);
+    }
+}
+
 // GEM order; see gem_plan_gem_order in include/gem_hip.h.
 int gem_order(const QueryInfo &q, const uint64_t *table_sizes, uint32_t *order);
 

@@ -10,7 +10,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import GEM_MAX_QE, GEM_MAX_QV, GemError, QueryView, CsrView, load
+from ._lib import GEM_MAX_QE, GEM_MAX_QV, GEM_RQ_WORDS, GemError, QueryView, CsrView, load
 
 
 def _check(st, what):
@@ -153,6 +153,38 @@ class Query:
     def save(self, path):
         v = self.view()
         _check(load().gem_query_save(C.byref(v), str(path).encode()), f"gem_query_save({path})")
+
+    def subquery(self, edge_mask):
+        """The relaxed query keeping the edges whose bit is set in `edge_mask`
+        (an int, bit i = edge i), same vertices and labels."""
+        keep = [i for i in range(self.num_e) if (edge_mask >> i) & 1]
+        return Query(self.labels.copy(), self.edges[keep])
+
+
+def masks_to_ints(masks):
+    """Rows of GEM_RQ_WORDS uint64 words -> Python ints (bit i = edge i)."""
+    return [sum(int(w) << (64 * k) for k, w in enumerate(row)) for row in np.asarray(masks).reshape(-1, GEM_RQ_WORDS)]
+
+
+def relaxed_queries(query, threshold):
+    """Edge masks (ints, bit i = edge i) of the relaxed queries of `query` at
+    `threshold`, in the library's order (gem_plan_relaxed_queries)."""
+    v = query.view()
+    n = C.c_uint32()
+    _check(load().gem_plan_relaxed_queries(C.byref(v), threshold, 0, None, C.byref(n)), "gem_plan_relaxed_queries")
+    masks = np.zeros((max(n.value, 1), GEM_RQ_WORDS), np.uint64)
+    _check(load().gem_plan_relaxed_queries(C.byref(v), threshold, n.value,
+                                           masks.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(n)),
+           "gem_plan_relaxed_queries")
+    return masks_to_ints(masks[:n.value])
+
+
+def edge_bits(mask, num_e):
+    """The reference's printed edge-tag string: bit i of the mask is character
+    width-1-i, width max(32, num_e) (std::bitset<32> for queries of at most 32
+    edges)."""
+    width = max(32, num_e)
+    return "".join("1" if (mask >> i) & 1 else "0" for i in range(width - 1, -1, -1))
 
 
 def gem_order(query, edge_sizes):

@@ -8,7 +8,7 @@ import weakref
 
 import numpy as np
 
-from ._lib import (GEM_MAX_QV, BatchOpts, GemError, JoinResult, KernelStat, MemoryStats, PhaseTimes, QueryView, StepLog,
+from ._lib import (GEM_MAX_QV, GEM_RQ_WORDS, BatchOpts, GemError, JoinResult, KernelStat, MemoryStats, PhaseTimes, QueryView, StepLog,
                    StreamStats, load)
 
 NOT_FOUND = 0xFFFFFFFF
@@ -236,6 +236,28 @@ class GemContext:
                         "rows_processed": int(r.rows_processed), "work": int(r.work),
                         "level_rows": lr[i * GEM_MAX_QV:i * GEM_MAX_QV + q.num_v].copy()})
         return out
+
+    def match_relaxed(self, query, threshold, model=None, max_orders=8, cap=0, work_limit=0, level_budget=0,
+                      group=0, first_attempt=0, restart_growth=4, restart_width=1, max_attempts=0):
+        """Relaxed matching (gem_hip_match_relaxed): one entry per relaxed
+        query of `query` at `threshold`, in the library's order, as
+        dict(mask, count, saturated, work_limited, work, restarts); mask is an
+        int with bit i set when edge i is kept."""
+        from .graph import relaxed_queries, masks_to_ints
+        n = len(relaxed_queries(query, threshold))
+        v = query.view()
+        opts = BatchOpts(int(cap), int(work_limit), int(level_budget), int(group), int(restart_growth),
+                         int(first_attempt), int(restart_width), int(max_attempts))
+        masks = np.zeros((n, GEM_RQ_WORDS), np.uint64)
+        res = (JoinResult * n)()
+        got = C.c_uint32()
+        self._check(self._lib.gem_hip_match_relaxed(self._h, C.byref(v), int(threshold),
+                                                    model._h if model is not None else None, int(max_orders),
+                                                    C.byref(opts), n, _p64(masks), res, C.byref(got)),
+                    "gem_hip_match_relaxed")
+        return [{"mask": m, "count": int(r.count), "saturated": bool(r.saturated),
+                 "work_limited": bool(r.work_limited), "work": int(r.work), "restarts": int(r.restarts)}
+                for m, r in zip(masks_to_ints(masks[:got.value]), res)]
 
     def match(self, query, cap=0):
         """Whole plain pipeline with GEM order; returns (count, saturated, order)."""

@@ -119,6 +119,22 @@ int gem_query_save(const gem_query_view *q, const char *path);
 int gem_plan_gem_order(const gem_query_view *q, const uint64_t *table_sizes,
                        uint32_t *order_out);
 
+/* Relaxed queries (relax(), in-core/lib/plan/src/relax.cpp:84-142, called at
+ * in-core/src/main.cu:88-92): the edge subsets of q that leave out at most
+ * `threshold` edges and still connect every query vertex (the reference's
+ * checkConnectivity, relax.cpp:57-81).  Threshold 0 gives the full query
+ * only.  Order: the full set, then by the number of edges left out, then by
+ * the ascending tuple of left-out edge indices.  Edge set i is
+ * masks[i * GEM_RQ_WORDS .. +GEM_RQ_WORDS), bit b of word w = edge 64 w + b
+ * (the reference's ettype bitset, in-core/lib/utils/include/defs.h:23-39,
+ * holds 32 edges; this one holds GEM_MAX_QE).  max_rq = 0 only counts
+ * (*num_rq); more than max_rq sets (or GEM_MAX_RQ when counting) is
+ * GEM_E_LIMIT. */
+#define GEM_RQ_WORDS (GEM_MAX_QE / 64u)
+#define GEM_MAX_RQ (1u << 20)
+int gem_plan_relaxed_queries(const gem_query_view *q, uint32_t threshold, uint32_t max_rq,
+                             uint64_t *masks, uint32_t *num_rq);
+
 /* -------------------------------------------------------- device context */
 /* A context owns one HIP stream on `device`; all of its device work runs on
  * that stream.  Device memory the library frees is kept for reuse by the same
@@ -432,6 +448,19 @@ int gem_hip_order_candidates(gem_ctx *ctx, uint32_t max_orders, uint64_t seed, u
 int gem_hip_match_batch_learned(gem_ctx *ctx, const gem_query_view *queries, uint32_t nq, gem_mlp *mlp,
                                 uint32_t max_orders, const gem_batch_opts *opts, gem_join_result *results,
                                 uint64_t *level_rows, uint32_t *chosen);
+
+/* Relaxed matching of one query at `threshold` (the reference's main() at
+ * -t threshold, in-core/src/main.cu:88-236: relax(), the lattice join and
+ * the "Relaxed Query ET <bits>: <count> matches" lines).  The relaxed queries
+ * are gem_plan_relaxed_queries's edge sets, in its order (masks[max_rq *
+ * GEM_RQ_WORDS]); results[i] is the count of relaxed query i's embeddings
+ * (injective, label-preserving maps that keep the edges of set i), computed
+ * by gem_hip_match_batch_learned over all relaxed queries as one batch with
+ * mlp, max_orders and opts.  *num_rq receives the number of relaxed queries;
+ * more than max_rq is GEM_E_LIMIT. */
+int gem_hip_match_relaxed(gem_ctx *ctx, const gem_query_view *q, uint32_t threshold, gem_mlp *mlp,
+                          uint32_t max_orders, const gem_batch_opts *opts, uint32_t max_rq, uint64_t *masks,
+                          gem_join_result *results, uint32_t *num_rq);
 
 /* Whole plain pipeline for one query: filter -> index -> tables -> GEM order
  * -> join.  Equivalent to the reference's main() for one query at -t 0. */

@@ -192,3 +192,61 @@ def bruteforce(graph, query):
     st = _load().orc_bruteforce(*ga, *qa, C.byref(c))
     assert st == 0, st
     return int(c.value)
+
+
+def relax(query, threshold):
+    """Relaxed queries as the reference's relax() reaches them
+    (in-core/lib/plan/src/relax.cpp:84-142): breadth-first from the full edge
+    tag, a tag at distance >= threshold or with at most num_v - 1 edges is not
+    expanded, every present edge is flipped off in index order, and a new tag
+    is kept only if checkConnectivity (relax.cpp:57-81: union-find over all
+    num_v vertices, one component) holds.  Queries of at most one edge give the
+    empty set (relax.cpp:88-89).  Returns the set of edge masks (int, bit i =
+    edge i).  Pure Python: small queries only."""
+    nv, ne = query.num_v, query.num_e
+    edges = [(int(a), int(b)) for a, b in query.edges]
+
+    def connected(tag):
+        if nv <= 1:
+            return True
+        parent = list(range(nv))
+
+        def find(x):
+            while parent[x] != x:
+                parent[x] = parent[parent[x]]
+                x = parent[x]
+            return x
+
+        comps = nv
+        for i in range(ne):
+            if (tag >> i) & 1:
+                a, b = find(edges[i][0]), find(edges[i][1])
+                if a != b:
+                    parent[a] = b
+                    comps -= 1
+        return comps == 1
+
+    out = set()
+    if ne <= 1:
+        return out
+    full = (1 << ne) - 1
+    out.add(full)
+    queue = [(full, 0)]
+    head = 0
+    while head < len(queue):
+        tag, dist = queue[head]
+        head += 1
+        if dist >= threshold or bin(tag).count("1") <= nv - 1:
+            continue
+        for i in range(ne):
+            if not (tag >> i) & 1:
+                continue
+            nxt = tag & ~(1 << i)
+            if nxt in out:
+                continue
+            out.add(nxt)
+            if connected(nxt):
+                queue.append((nxt, dist + 1))
+            else:
+                out.discard(nxt)
+    return out

@@ -98,3 +98,25 @@ def test_plain_counts_match_oracle_fixtures(gpu_ctx, name):
         assert finished >= 1
     finally:
         gpu_ctx.set_work_limit(0)
+
+
+def test_streamed_counts_match_oracle_fixtures(gpu_ctx):
+    """The R-MAT 2^20 workload's 100 queries with their edge tables streamed
+    from pinned host memory through a 256 MiB device cache: every count equals
+    the oracle's fixture."""
+    fx, g, picked = _setup(gpu_ctx, "rmat20")
+    qs = [q for _, q in picked]
+    model = _model(gpu_ctx, "rmat20")
+    try:
+        gpu_ctx.set_table_stream(256 << 20, 1 << 20)
+        res = match_batch_learned(gpu_ctx, qs, model, max_orders=8, cap=fx["cap"], first_attempt=FIRST_ATTEMPT)
+        st = gpu_ctx.stream_stats()
+    finally:
+        gpu_ctx.set_table_stream(0)
+        if model is not None:
+            model.close()
+    print("stream stats", st, flush=True)
+    assert st["table_bytes"] > (256 << 20)
+    bad = [(e["index"], r["count"], r["saturated"], r["work_limited"]) for (e, _), r in zip(picked, res)
+           if r["work_limited"] or (r["count"], r["saturated"]) != (e["count"], e["saturated"])]
+    assert not bad, bad

@@ -54,6 +54,30 @@ def test_streamed_batch_equals_in_hbm(gpu_ctx, rmat_graph, cache, slot):
     assert st["h2d_bytes"] >= 4 * st["blocks_loaded"]
 
 
+def test_streamed_batch_equals_oracle(gpu_ctx, rmat_graph):
+    """Tables streamed through a 1 MiB cache of 32 KiB slots: counts and the
+    rows of every level equal the oracle's vertex-level walk along the same
+    vertex orders (GEM's, from the oracle's lattice)."""
+    import oracle
+    gpu_ctx.load_data(rmat_graph)
+    qs = _queries(rmat_graph, 12, [4, 5, 6, 7], 23001)
+    capv = 10 ** 8
+    vos = [oracle.vertex_order_from_edges(q, oracle.count(rmat_graph, q, cap=capv)["order"]) for q in qs]
+    try:
+        gpu_ctx.set_table_stream(1 << 20, 32 << 10)
+        got = gpu_ctx.match_batch(qs, vertex_orders=vos, cap=capv, level_budget=64 << 20)
+        st = gpu_ctx.stream_stats()
+    finally:
+        gpu_ctx.set_table_stream(0)
+    assert st["blocks_loaded"] > 0
+    for q, vo, r in zip(qs, vos, got):
+        ref = oracle.vcount(rmat_graph, q, vo, cap=capv)
+        print("streamed vs oracle", q.num_v, r["count"], ref["count"], flush=True)
+        assert (r["count"], r["saturated"]) == (ref["count"], ref["saturated"])
+        if not ref["saturated"]:
+            assert [int(x) for x in r["level_rows"]] == [int(x) for x in ref["level_rows"]]
+
+
 def test_streamed_learned_and_grouped(gpu_ctx, rmat_graph):
     gpu_ctx.load_data(rmat_graph)
     qs = _queries(rmat_graph, 12, [6, 8, 10], 22001)
