@@ -1,6 +1,7 @@
 // C ABI entry points of the device library (declared in include/gem_hip.h).
 #include <cstring>
 #include <new>
+#include <thread>
 
 #include "ctx.h"
 
@@ -26,6 +27,19 @@ int guard(gem_ctx *c, F &&f) {
     c->err = e.what();
     return GEM_E_INVALID;
   }
+}
+
+// memcpy of a large host array over up to 16 threads.
+void parallel_copy(void *dst, const void *src, size_t bytes) {
+  const size_t nt = std::min<size_t>(16, std::max<size_t>(1, bytes >> 26));
+  std::vector<std::thread> th;
+  const size_t chunk = (bytes + nt - 1) / nt;
+  for (size_t t = 0; t < nt; ++t) {
+    const size_t b = t * chunk, e = std::min(bytes, b + chunk);
+    if (b < e)
+      th.emplace_back([=]() { std::memcpy((char *)dst + b, (const char *)src + b, e - b); });
+  }
+  for (auto &x : th) x.join();
 }
 
 }  // namespace
@@ -101,6 +115,12 @@ int gem_hip_set_max_active(gem_ctx *c, uint32_t queries) {
   return GEM_OK;
 }
 
+int gem_hip_set_graph_residency(gem_ctx *c, int host) {
+  if (!c || host < 0 || host > 1) return GEM_E_INVALID;
+  c->graph_host = host;
+  return GEM_OK;
+}
+
 int gem_hip_set_graph(gem_ctx *c, int on) {
   if (!c) return GEM_E_INVALID;
   c->use_graph = on != 0;
@@ -169,9 +189,20 @@ int gem_hip_load_data_csr(gem_ctx *c, const gem_csr_view *g) {
     G.lab8 = G.label_freq.size() <= 255;
     hipStream_t s = c->stream;
     G.off.alloc(((uint64_t)G.n + 1) * 8, s);
-    G.nbr.alloc(std::max<uint64_t>(G.slots, 1) * 4, s);
     GEM_HIP_CHECK(hipMemcpyAsync(G.off.p, g->offsets, ((uint64_t)G.n + 1) * 8, hipMemcpyHostToDevice, s));
-    if (G.slots) GEM_HIP_CHECK(hipMemcpyAsync(G.nbr.p, g->neighbors, G.slots * 4, hipMemcpyHostToDevice, s));
+    if (c->graph_host) {
+      // Host-resident graph: the neighbour array (all but 12 B per vertex of
+      // the CSR) stays in host memory, pinned and mapped; the filter and table
+      // kernels read it over PCIe, non-coherent so that L2 may cache it.
+      G.nbr_host.alloc(std::max<uint64_t>(G.slots, 1) * 4, hipHostMallocMapped | hipHostMallocNonCoherent);
+      parallel_copy(G.nbr_host.p, g->neighbors, G.slots * 4);
+      void *dp = nullptr;
+      GEM_HIP_CHECK(hipHostGetDevicePointer(&dp, G.nbr_host.p, 0));
+      G.nbr_mapped = static_cast<const uint32_t *>(dp);
+    } else {
+      G.nbr.alloc(std::max<uint64_t>(G.slots, 1) * 4, s);
+      if (G.slots) GEM_HIP_CHECK(hipMemcpyAsync(G.nbr.p, g->neighbors, G.slots * 4, hipMemcpyHostToDevice, s));
+    }
     if (G.lab8) {
       std::vector<uint8_t> l8(dense.begin(), dense.end());
       G.lab.alloc(std::max<uint64_t>(G.n, 1), s);

@@ -104,35 +104,6 @@ struct DevBuf {
 // Data graph resident in HBM.  Labels are replaced by dense label ids
 // (first-seen order over the vertex array) held as u8 when the graph has at
 // most 255 distinct labels, else u32.
-struct GraphDev {
-  bool loaded = false;
-  uint32_t n = 0;
-  uint64_t slots = 0;
-  uint64_t max_deg = 0;
-  uint32_t n_hub = 0;  // vertices of degree > FILTER_HUB_DEG
-  DevBuf off;     // u64[n + 1]
-  DevBuf nbr;     // u32[slots]
-  DevBuf lab;     // u8[n] or u32[n] dense label ids
-  bool lab8 = true;
-  std::unordered_map<uint32_t, uint32_t> label_id;  // original -> dense
-  std::vector<uint64_t> label_freq;                 // dense id -> #vertices
-  std::vector<uint64_t> label_deg;                  // dense id -> sum of degrees
-};
-
-// The filter walks the adjacency of vertices of larger degree than this in
-// chunks of this many entries spread over waves.
-constexpr uint32_t FILTER_HUB_DEG = 4096;
-
-// Query description consumed by the filter kernel (device copy).
-struct FilterQuery {
-  uint32_t nv, nslots, T, W;
-  uint32_t qdeg[GEM_MAX_QV];
-  uint32_t group[GEM_MAX_QV];   // slot -> mask of query vertices with that label
-  uint32_t nreq[GEM_MAX_QV];
-  uint8_t req_slot[GEM_MAX_QV][GEM_MAX_QV];
-  uint8_t req_cnt[GEM_MAX_QV][GEM_MAX_QV];
-};
-
 // Pinned host allocation (hipHostMalloc), move-only.
 struct PinnedBuf {
   void *p = nullptr;
@@ -170,6 +141,40 @@ struct PinnedBuf {
   T *as() const {
     return reinterpret_cast<T *>(p);
   }
+};
+
+struct GraphDev {
+  bool loaded = false;
+  uint32_t n = 0;
+  uint64_t slots = 0;
+  uint64_t max_deg = 0;
+  uint32_t n_hub = 0;  // vertices of degree > FILTER_HUB_DEG
+  DevBuf off;     // u64[n + 1]
+  DevBuf nbr;     // u32[slots] in HBM, or
+  PinnedBuf nbr_host;  // u32[slots] in mapped pinned host memory (host-resident graph)
+  const uint32_t *nbr_mapped = nullptr;  // device address of nbr_host
+  DevBuf lab;     // u8[n] or u32[n] dense label ids
+  bool lab8 = true;
+  std::unordered_map<uint32_t, uint32_t> label_id;  // original -> dense
+  std::vector<uint64_t> label_freq;                 // dense id -> #vertices
+  std::vector<uint64_t> label_deg;                  // dense id -> sum of degrees
+  // Neighbour array as the kernels address it: HBM, or host memory read over
+  // PCIe by the filter and table kernels.
+  const uint32_t *nbr_dev() const { return nbr_mapped ? nbr_mapped : nbr.as<uint32_t>(); }
+};
+
+// The filter walks the adjacency of vertices of larger degree than this in
+// chunks of this many entries spread over waves.
+constexpr uint32_t FILTER_HUB_DEG = 4096;
+
+// Query description consumed by the filter kernel (device copy).
+struct FilterQuery {
+  uint32_t nv, nslots, T, W;
+  uint32_t qdeg[GEM_MAX_QV];
+  uint32_t group[GEM_MAX_QV];   // slot -> mask of query vertices with that label
+  uint32_t nreq[GEM_MAX_QV];
+  uint8_t req_slot[GEM_MAX_QV][GEM_MAX_QV];
+  uint8_t req_cnt[GEM_MAX_QV][GEM_MAX_QV];
 };
 
 struct TableDev {
@@ -262,6 +267,7 @@ struct gem_ctx {
   uint32_t persistent_blocks = 2048;        // grid of the batched path's persistent kernels
   uint32_t heavy_t = 1024, heavy_chunk = 1024;  // batched walks: long driving lists split over waves
   uint32_t max_active = 64;                 // batched path: queries running at once (0 = all)
+  int graph_host = 0;                       // next data graph: neighbours stay in pinned host memory
   bool use_graph = false;                   // replay the batched extend step as a hipGraph
   // batched path with streamed edge tables: values in pinned host memory,
   // a device cache of stream_cache_bytes in slots of stream_slot_bytes

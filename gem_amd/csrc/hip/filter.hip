@@ -370,11 +370,11 @@ static void filter_launch(gem_ctx *c, uint32_t nv, uint32_t W, const FilterBufs 
   {
     ProfLaunch pl(c, K_FILTER);
     if (G.lab8)
-      k_filter<uint8_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+      k_filter<uint8_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr_dev(),
                                                              G.lab.as<uint8_t>(), G.n, B.fq, B.slot_of, B.bitmap,
                                                              B.tile_count, B.heavy_list, B.heavy_n);
     else
-      k_filter<uint32_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+      k_filter<uint32_t><<<blocks, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr_dev(),
                                                               G.lab.as<uint32_t>(), G.n, B.fq, B.slot_of, B.bitmap,
                                                               B.tile_count, B.heavy_list, B.heavy_n);
     GEM_LAUNCH_CHECK("k_filter");
@@ -385,12 +385,12 @@ static void filter_launch(gem_ctx *c, uint32_t nv, uint32_t W, const FilterBufs 
     ProfLaunch pl(c, K_FILTER_HEAVY);
     const uint32_t hb = 1024;  // persistent: 4096 waves walk the heavy list
     if (G.lab8)
-      k_filter_heavy<uint8_t><<<hb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+      k_filter_heavy<uint8_t><<<hb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr_dev(),
                                                                G.lab.as<uint8_t>(), B.fq, B.slot_of, bm, B.tile_count,
                                                                B.heavy_list, B.heavy_n, B.hub_list, B.hub_n,
                                                                B.hub_cap);
     else
-      k_filter_heavy<uint32_t><<<hb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+      k_filter_heavy<uint32_t><<<hb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr_dev(),
                                                                 G.lab.as<uint32_t>(), B.fq, B.slot_of, bm,
                                                                 B.tile_count, B.heavy_list, B.heavy_n, B.hub_list,
                                                                 B.hub_n, B.hub_cap);
@@ -400,14 +400,14 @@ static void filter_launch(gem_ctx *c, uint32_t nv, uint32_t W, const FilterBufs 
                                                        2048);
       const uint32_t db = (G.n_hub + FILTER_WAVES - 1) / FILTER_WAVES;
       if (G.lab8) {
-        k_filter_hub<uint8_t><<<tb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+        k_filter_hub<uint8_t><<<tb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr_dev(),
                                                                G.lab.as<uint8_t>(), B.fq, B.slot_of, B.hub_list,
                                                                B.hub_n, B.hub_cap, maxc, B.hub_cnt);
         k_filter_hub_decide<uint8_t><<<db, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.lab.as<uint8_t>(), B.fq,
                                                                       B.slot_of, B.hub_list, B.hub_n, B.hub_cap,
                                                                       B.hub_cnt, bm, B.tile_count);
       } else {
-        k_filter_hub<uint32_t><<<tb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr.as<uint32_t>(),
+        k_filter_hub<uint32_t><<<tb, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.nbr_dev(),
                                                                 G.lab.as<uint32_t>(), B.fq, B.slot_of, B.hub_list,
                                                                 B.hub_n, B.hub_cap, maxc, B.hub_cnt);
         k_filter_hub_decide<uint32_t><<<db, FILTER_WAVES * 64, 0, s>>>(G.off.as<uint64_t>(), G.lab.as<uint32_t>(),

@@ -314,12 +314,12 @@ void run_tables(gem_ctx *c, QueryDev &Qd) {
     if (!blocks) return;
     if (write)
       k_tables<true><<<blocks, TAB_WAVES * 64, 0, s>>>(
-          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
+          G.off.as<uint64_t>(), G.nbr_dev(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
           Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
           heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>());
     else
       k_tables<false><<<blocks, TAB_WAVES * 64, 0, s>>>(
-          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
+          G.off.as<uint64_t>(), G.nbr_dev(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
           Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
           heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>());
     GEM_LAUNCH_CHECK(write ? "k_tables<write>" : "k_tables<count>");
@@ -329,12 +329,12 @@ void run_tables(gem_ctx *c, QueryDev &Qd) {
     const uint32_t hb = (uint32_t)std::min<uint64_t>((nchunks + TAB_WAVES - 1) / TAB_WAVES, 2048);
     if (write)
       k_tables_heavy<true><<<hb, TAB_WAVES * 64, 0, s>>>(
-          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
+          G.off.as<uint64_t>(), G.nbr_dev(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
           Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
           heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>(), cpre.as<uint64_t>(), ccnt.as<uint32_t>(), ndmax);
     else
       k_tables_heavy<false><<<hb, TAB_WAVES * 64, 0, s>>>(
-          G.off.as<uint64_t>(), G.nbr.as<uint32_t>(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
+          G.off.as<uint64_t>(), G.nbr_dev(), Qd.can.as<uint32_t>(), Qd.bitmap.as<uint64_t>(),
           Qd.rank.as<uint32_t>(), job, ddesc.as<TableDesc>(), adj_len.as<unsigned long long>(),
           heavy_list.as<uint64_t>(), heavy_n.as<uint32_t>(), cpre.as<uint64_t>(), ccnt.as<uint32_t>(), ndmax);
     GEM_LAUNCH_CHECK(write ? "k_tables_heavy<write>" : "k_tables_heavy<count>");

@@ -161,6 +161,11 @@ class GemContext:
         `chunk`-entry pieces spread over waves (load balance only)."""
         self._check(self._lib.gem_hip_set_heavy_split(self._h, int(threshold), int(chunk)), "gem_hip_set_heavy_split")
 
+    def set_graph_residency(self, host):
+        """host=True: the next load_data keeps the neighbour array in pinned
+        host memory (read over PCIe by the filter and table kernels)."""
+        self._check(self._lib.gem_hip_set_graph_residency(self._h, int(bool(host))), "gem_hip_set_graph_residency")
+
     def set_max_active(self, queries):
         """Queries of a batched run holding join levels at once (0 = all)."""
         self._check(self._lib.gem_hip_set_max_active(self._h, int(queries)), "gem_hip_set_max_active")

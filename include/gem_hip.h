@@ -2,8 +2,8 @@
  * gem_hip.h — C ABI of the MI355X (gfx950) subgraph-matching library.
  *
  * The library enumerates/counts the embeddings of a labelled query graph in a
- * labelled data graph (injective, label- and edge-preserving maps; threshold 0)
- * with GEM's edge-centric pipeline:
+ * labelled data graph (injective, label- and edge-preserving maps), and of its
+ * relaxed queries at a threshold T > 0, with GEM's edge-centric pipeline:
  *
  *   data CSR -> candidate filter -> position index -> edge-candidate tables
  *            -> GEM-ordered join (expand / select) -> uint64 count
@@ -23,7 +23,9 @@
  *   in-core/lib/structure/src/unifiedTrie.cu:939-1018
  *                                               find_best_edge_candidate (GEM order)
  *   in-core/lib/join/include/join_trie.cuh:31-32 join
+ *   in-core/lib/plan/include/relax.h            relax (relaxed queries, T > 0)
  *   in-core/src/main.cu:229-236                 per-query result readout
+ *   out-of-core/src/main.cu:334-410             per-query child processes (CLI)
  */
 #ifndef GEM_HIP_H
 #define GEM_HIP_H
@@ -162,6 +164,15 @@ typedef struct gem_csr_view {
   const uint32_t *labels;    /* [num_v], original labels */
 } gem_csr_view;
 int gem_hip_load_data_csr(gem_ctx *ctx, const gem_csr_view *g);
+/* Where the next gem_hip_load_data_csr keeps the neighbour array (the
+ * out-of-core reference keeps its data graph on the host, out-of-core/src/
+ * main.cu:445-447, and filters and builds tables on the CPU,
+ * out-of-core/lib/filter/src/filter.cu:61-295 and
+ * out-of-core/lib/structure/src/unifiedTrie.cu:1224-1427): host = 0, in HBM
+ * (default); host = 1, in pinned, mapped host memory that the filter and
+ * table kernels read over PCIe, for graphs larger than HBM.  Offsets and
+ * labels (12 B per vertex) stay in HBM.  Results do not depend on it. */
+int gem_hip_set_graph_residency(gem_ctx *ctx, int host);
 
 /* Candidate filter for a new query (replaces filterSSM,
  * in-core/lib/filter/src/filter.cu:34-167 and the CPU form
