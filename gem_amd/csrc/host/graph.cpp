@@ -221,8 +221,105 @@ int read_text(const char *path, TextGraph &t) {
 }
 }  // namespace
 
+// Binary CSR cache: a 64-byte header ("GEMCSR01", n, slots, label count
+// and the FNV-1a 64 digest of the three arrays), then offsets[n + 1] (u64),
+// neighbors[slots] (u32) and labels[n] (u32), little-endian as in memory.
+namespace {
+constexpr char BIN_MAGIC[8] = {'G', 'E', 'M', 'C', 'S', 'R', '0', '1'};
+struct BinHeader {
+  char magic[8];
+  uint64_t n, slots, reserved, digest;
+  uint64_t pad[3];
+};
+static_assert(sizeof(BinHeader) == 64, "header is 64 bytes");
+
+uint64_t fnv1a(uint64_t h, const void *p, size_t bytes) {
+  const unsigned char *b = static_cast<const unsigned char *>(p);
+  for (size_t i = 0; i < bytes; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+  return h;
+}
+
+// Digest over fixed 1 MiB pieces, combined in order: the same value however
+// many threads hash the pieces.
+uint64_t csr_digest(const gem_graph &g) {
+  struct Span {
+    const void *p;
+    size_t bytes;
+  };
+  const Span arrays[3] = {{g.off.data(), g.off.size() * 8}, {g.nbr.data(), g.nbr.size() * 4},
+                          {g.lab.data(), g.lab.size() * 4}};
+  std::vector<Span> pieces;
+  constexpr size_t PIECE = 1u << 20;
+  for (const Span &a : arrays)
+    for (size_t o = 0; o < a.bytes; o += PIECE)
+      pieces.push_back({static_cast<const char *>(a.p) + o, std::min(PIECE, a.bytes - o)});
+  std::vector<uint64_t> h(pieces.size());
+  gemh::parallel_chunks(pieces.size(), [&](uint64_t b, uint64_t e) {
+    for (uint64_t i = b; i < e; ++i) h[i] = fnv1a(0xcbf29ce484222325ull, pieces[i].p, pieces[i].bytes);
+  });
+  return fnv1a(0xcbf29ce484222325ull, h.data(), h.size() * 8);
+}
+
+bool read_all(FILE *f, void *p, size_t bytes) {
+  char *c = static_cast<char *>(p);
+  while (bytes) {
+    const size_t k = std::fread(c, 1, std::min<size_t>(bytes, 1u << 30), f);
+    if (k == 0) return false;
+    c += k;
+    bytes -= k;
+  }
+  return true;
+}
+
+int load_binary(FILE *f, gem_graph **out) {
+  BinHeader h;
+  if (!read_all(f, &h, sizeof h) || std::memcmp(h.magic, BIN_MAGIC, 8) != 0) return GEM_E_INVALID;
+  if (h.n > 0xFFFFFFFFull || h.slots > (1ull << 40)) return GEM_E_LIMIT;
+  auto *g = new gem_graph();
+  g->n = (uint32_t)h.n;
+  g->off.resize(h.n + 1);
+  g->nbr.resize(h.slots);
+  g->lab.resize(h.n);
+  const bool ok = read_all(f, g->off.data(), g->off.size() * 8) && read_all(f, g->nbr.data(), g->nbr.size() * 4) &&
+                  read_all(f, g->lab.data(), g->lab.size() * 4) && std::fgetc(f) == EOF;
+  if (!ok || g->off[0] != 0 || g->off[h.n] != h.slots || csr_digest(*g) != h.digest) {
+    delete g;
+    return GEM_E_INVALID;
+  }
+  // the digest guards the bytes; these checks guard the indexing
+  bool bad = false;
+  for (uint32_t v = 0; v < g->n && !bad; ++v) bad = g->off[v + 1] < g->off[v];
+  std::vector<uint8_t> out_of_range(64, 0);
+  gemh::parallel_chunks(g->nbr.size(), [&](uint64_t b, uint64_t e) {
+    uint8_t any = 0;
+    for (uint64_t i = b; i < e; ++i) any |= g->nbr[i] >= g->n;
+    if (any) __atomic_store_n(&out_of_range[0], 1, __ATOMIC_RELAXED);
+  });
+  if (bad || out_of_range[0]) {
+    delete g;
+    return GEM_E_INVALID;
+  }
+  *out = g;
+  return GEM_OK;
+}
+}  // namespace
+
 int gem_graph_load(const char *path, gem_graph **out) {
   if (!out) return GEM_E_INVALID;
+  if (path) {
+    // a binary CSR cache is recognised by its magic
+    if (FILE *f = std::fopen(path, "rb")) {
+      char m[8] = {0};
+      const bool bin = std::fread(m, 1, 8, f) == 8 && std::memcmp(m, BIN_MAGIC, 8) == 0;
+      int st = GEM_OK;
+      if (bin) {
+        std::rewind(f);
+        st = load_binary(f, out);
+      }
+      std::fclose(f);
+      if (bin) return st;
+    }
+  }
   TextGraph t;
   int st = read_text(path, t);
   if (st) return st;
@@ -266,6 +363,25 @@ int gem_graph_save(const gem_graph *g, const char *path) {
   std::fclose(f);
   return bad ? GEM_E_IO : GEM_OK;
 }
+
+int gem_graph_save_binary(const gem_graph *g, const char *path) {
+  if (!g || !path) return GEM_E_INVALID;
+  BinHeader h{};
+  std::memcpy(h.magic, BIN_MAGIC, 8);
+  h.n = g->n;
+  h.slots = g->off[g->n];
+  h.digest = csr_digest(*g);
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return GEM_E_IO;
+  bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+  ok = ok && std::fwrite(g->off.data(), 8, g->off.size(), f) == g->off.size();
+  ok = ok && std::fwrite(g->nbr.data(), 4, g->nbr.size(), f) == g->nbr.size();
+  ok = ok && std::fwrite(g->lab.data(), 4, g->lab.size(), f) == g->lab.size();
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? GEM_OK : GEM_E_IO;
+}
+
+uint64_t gem_graph_digest(const gem_graph *g) { return g ? csr_digest(*g) : 0; }
 
 uint32_t gem_graph_num_vertices(const gem_graph *g) { return g ? g->n : 0; }
 uint64_t gem_graph_num_slots(const gem_graph *g) { return g ? g->off[g->n] : 0; }

@@ -90,6 +90,14 @@ class Graph:
     def save(self, path):
         _check(load().gem_graph_save(self._h, str(path).encode()), f"gem_graph_save({path})")
 
+    def save_binary(self, path):
+        """Binary CSR cache; Graph.load recognises it."""
+        _check(load().gem_graph_save_binary(self._h, str(path).encode()), f"gem_graph_save_binary({path})")
+
+    def digest(self):
+        """FNV-1a 64 digest of offsets, neighbours and labels."""
+        return int(load().gem_graph_digest(self._h))
+
     def csr_view(self):
         return CsrView(self.num_vertices, self.num_slots,
                        self.offsets.ctypes.data_as(C.POINTER(C.c_uint64)),

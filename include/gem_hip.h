@@ -80,6 +80,15 @@ int gem_graph_from_csr(uint32_t n, const uint64_t *offsets,
  * (in-core/lib/io/src/io.cu:45-220).  Declared degrees must match. */
 int gem_graph_load(const char *path, gem_graph **out);
 int gem_graph_save(const gem_graph *g, const char *path);
+/* Binary CSR cache (no reference counterpart: the reference parses the text
+ * format on every run, in-core/lib/io/src/io.cu:45-220, which takes minutes
+ * for graphs of billions of edges).  A 64-byte header ("GEMCSR01", n, slots,
+ * a digest of the arrays), then offsets, neighbors and labels as in memory.
+ * gem_graph_load recognises such a file by its magic and checks the digest,
+ * the offsets and the neighbour range (GEM_E_INVALID otherwise). */
+int gem_graph_save_binary(const gem_graph *g, const char *path);
+/* FNV-1a 64 digest of the CSR arrays (over 1 MiB pieces, combined in order). */
+uint64_t gem_graph_digest(const gem_graph *g);
 uint32_t gem_graph_num_vertices(const gem_graph *g);
 uint64_t gem_graph_num_slots(const gem_graph *g); /* = 2 |E| */
 const uint64_t *gem_graph_offsets(const gem_graph *g);

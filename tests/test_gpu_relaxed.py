@@ -98,7 +98,8 @@ def test_filter_at_threshold_equals_oracle(gpu_ctx, er_graph, T):
 def test_cli_relaxed_lines_and_isolation(tmp_path, er_graph):
     """The front end at -t 1 prints one line per relaxed query with the
     oracle's count; in a directory, a query file it cannot load fails in its
-    own child and the other queries still run."""
+    own child and the other queries still run; a binary CSR cache loads in
+    place of the text graph."""
     assert os.path.exists(CLI)
     er_graph.save(tmp_path / "d.graph")
     qd = tmp_path / "q"
@@ -119,3 +120,10 @@ def test_cli_relaxed_lines_and_isolation(tmp_path, er_graph):
         for m in relaxed_queries(q, 1):
             want.append((edge_bits(m, q.num_e), str(oracle.count(er_graph, q.subquery(m), cap=CAP)["count"])))
     assert got == want
+    # the same data graph from its binary CSR cache, one query at -t 0
+    er_graph.save_binary(tmp_path / "d.bin")
+    r = subprocess.run([CLI, "-q", str(qd / "Q_0.in"), "-d", str(tmp_path / "d.bin"), "-t", "0", "--cap", str(CAP)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    got = re.findall(r"Relaxed Query ET ([01]+): (\d+) matches", r.stdout)
+    assert got == [(edge_bits((1 << qs[0].num_e) - 1, qs[0].num_e), str(oracle.count(er_graph, qs[0], cap=CAP)["count"]))]

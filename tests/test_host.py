@@ -193,3 +193,37 @@ def test_benchmark_workload_is_the_described_batch():
     learn = make_queries("rmat22_learn", g)
     key = {(tuple(q.labels.tolist()), tuple(map(tuple, q.edges.tolist()))) for q in qs}
     assert not any((tuple(q.labels.tolist()), tuple(map(tuple, q.edges.tolist()))) in key for q in learn)
+
+
+def _fnv1a(data, h=0xcbf29ce484222325):
+    for b in data:
+        h = ((h ^ b) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_binary_csr_cache(tmp_path):
+    """Round trip through the binary CSR cache, its digest (FNV-1a 64 of each
+    1 MiB piece of offsets, neighbours and labels, then of the piece digests),
+    and rejection of a flipped byte and of a truncated file."""
+    g = Graph.erdos_renyi(3000, 9000, 5, 8)
+    p = tmp_path / "g.bin"
+    g.save_binary(p)
+    h = Graph.load(p)
+    assert np.array_equal(g.offsets, h.offsets) and np.array_equal(g.neighbors, h.neighbors)
+    assert np.array_equal(g.labels, h.labels)
+    pieces = []
+    for arr in (g.offsets.astype(np.uint64), g.neighbors.astype(np.uint32), g.labels.astype(np.uint32)):
+        b = arr.tobytes()
+        pieces += [b[i:i + (1 << 20)] for i in range(0, len(b), 1 << 20)]
+    want = _fnv1a(b"".join(_fnv1a(x).to_bytes(8, "little") for x in pieces))
+    assert g.digest() == want == h.digest()
+    raw = bytearray(p.read_bytes())
+    assert raw[:8] == b"GEMCSR01" and int.from_bytes(raw[32:40], "little") == want
+    bad = bytearray(raw)
+    bad[64 + 8 * 3001 + 17] ^= 0x40  # a neighbour byte
+    (tmp_path / "bad.bin").write_bytes(bad)
+    with pytest.raises(GemError):
+        Graph.load(tmp_path / "bad.bin")
+    (tmp_path / "short.bin").write_bytes(raw[:-4])
+    with pytest.raises(GemError):
+        Graph.load(tmp_path / "short.bin")
