@@ -97,6 +97,7 @@ SIGNATURES = [
     ("gem_hip_set_heavy_split", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     ("gem_hip_set_graph", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_set_graph_residency", C.c_int, [C.c_void_p, C.c_int]),
+    ("gem_hip_set_relaxed_sharing", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_set_max_active", C.c_int, [C.c_void_p, C.c_uint32]),
     ("gem_hip_set_table_stream", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     ("gem_hip_stream_stats", C.c_int, [C.c_void_p, C.POINTER(StreamStats)]),

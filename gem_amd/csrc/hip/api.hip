@@ -115,6 +115,12 @@ int gem_hip_set_max_active(gem_ctx *c, uint32_t queries) {
   return GEM_OK;
 }
 
+int gem_hip_set_relaxed_sharing(gem_ctx *c, int on) {
+  if (!c) return GEM_E_INVALID;
+  c->relaxed_share = on != 0;
+  return GEM_OK;
+}
+
 int gem_hip_set_graph_residency(gem_ctx *c, int host) {
   if (!c || host < 0 || host > 1) return GEM_E_INVALID;
   c->graph_host = host;
@@ -376,17 +382,7 @@ int gem_hip_match_relaxed(gem_ctx *c, const gem_query_view *q, uint32_t threshol
     std::vector<gemh::EdgeSet> sets;
     st = gemh::relaxed_edge_sets(Q, threshold, max_rq, &sets);
     if (st) throw Error{st, "more than max_rq relaxed queries"};
-    // Every relaxed query is a query of its own in one batched run: counts do
-    // not depend on the other relaxed queries, and the batch shares the
-    // persistent launches between them.
-    std::vector<std::vector<uint32_t>> edges(sets.size());
-    std::vector<gem_query_view> views(sets.size());
-    for (size_t i = 0; i < sets.size(); ++i) {
-      gemh::subquery_edges(q, sets[i], &edges[i]);
-      views[i] = gem_query_view{q->num_v, (uint32_t)(edges[i].size() / 2), q->labels, edges[i].data()};
-    }
-    run_match_batch_learned(c, views.data(), (uint32_t)views.size(), mlp, max_orders, opts, results, nullptr,
-                            nullptr);
+    run_match_relaxed(c, q, threshold, sets, mlp, max_orders, opts, results);
     gemh::pack_edge_sets(sets, masks);
     *num_rq = (uint32_t)sets.size();
   });

@@ -268,6 +268,7 @@ struct gem_ctx {
   uint32_t heavy_t = 1024, heavy_chunk = 1024;  // batched walks: long driving lists split over waves
   uint32_t max_active = 64;                 // batched path: queries running at once (0 = all)
   int graph_host = 0;                       // next data graph: neighbours stay in pinned host memory
+  bool relaxed_share = true;                // relaxed queries share one filter and one table build
   bool use_graph = false;                   // replay the batched extend step as a hipGraph
   // batched path with streamed edge tables: values in pinned host memory,
   // a device cache of stream_cache_bytes in slots of stream_slot_bytes
@@ -393,6 +394,9 @@ void run_order_candidates(gem_ctx *c, uint32_t max_orders, uint64_t seed, uint32
 void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, gem_mlp *mlp, uint32_t max_orders,
                              const gem_batch_opts *opts, gem_join_result *results, uint64_t *level_rows,
                              uint32_t *chosen);
+void run_match_relaxed(gem_ctx *c, const gem_query_view *q, uint32_t threshold,
+                       const std::vector<gemh::EdgeSet> &sets, gem_mlp *mlp, uint32_t max_orders,
+                       const gem_batch_opts *opts, gem_join_result *results);
 
 // ------------------------------------------------------------------ scans
 // Exclusive scan of n u64 values in place-free form: out[i] = sum(in[0..i)),

@@ -47,6 +47,11 @@ struct HeavyRow {
 
 struct VQueryState {
   const QueryDev *qd = nullptr;
+  // A relaxed query sharing its parent's candidates and tables: planned on its
+  // own edges (Qp), whose edge i is edge (*edge_map)[i] of qd's query; the
+  // plan's table ids are qd's.  Null: the query is qd->Q itself.
+  const gemh::QueryInfo *Qp = nullptr;
+  const std::vector<uint32_t> *edge_map = nullptr;
   gemh::VertexPlan plan;
   DevBuf levels;  // VLevel[nv]
   std::vector<VLevel> hlv;  // host copy of `levels`

@@ -577,8 +577,14 @@ void build_vlevels(gem_ctx *c, VQueryState &S) {
 }
 
 void replan(gem_ctx *c, VQueryState &S, const std::vector<uint32_t> &vorder) {
-  if (gemh::make_vertex_plan(S.qd->Q, vorder, &S.plan))
+  if (gemh::make_vertex_plan(S.Qp ? *S.Qp : S.qd->Q, vorder, &S.plan))
     throw Error{GEM_E_INVALID, "invalid vertex order (leaf first, or a vertex without an earlier neighbour)"};
+  if (S.edge_map)
+    for (size_t l = 1; l < S.plan.levels.size(); ++l) {
+      gemh::VLevelPlan &lp = S.plan.levels[l];
+      for (uint32_t j = 0; j < lp.nb; ++j)
+        lp.directed[j] = 2 * (*S.edge_map)[lp.directed[j] >> 1] + (lp.directed[j] & 1);
+    }
   build_vlevels(c, S);
   S.mnb_all = std::max(S.mnb_all, *std::max_element(S.max_nb.begin(), S.max_nb.end()));
   S.mnd_all = std::max(S.mnd_all, *std::max_element(S.max_ndup.begin(), S.max_ndup.end()));

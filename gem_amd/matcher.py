@@ -161,6 +161,11 @@ class GemContext:
         `chunk`-entry pieces spread over waves (load balance only)."""
         self._check(self._lib.gem_hip_set_heavy_split(self._h, int(threshold), int(chunk)), "gem_hip_set_heavy_split")
 
+    def set_relaxed_sharing(self, on=True):
+        """Relaxed queries share one filter at the threshold and one table per
+        query edge (default), or are filtered and tabled one by one."""
+        self._check(self._lib.gem_hip_set_relaxed_sharing(self._h, int(bool(on))), "gem_hip_set_relaxed_sharing")
+
     def set_graph_residency(self, host):
         """host=True: the next load_data keeps the neighbour array in pinned
         host memory (read over PCIe by the filter and table kernels)."""

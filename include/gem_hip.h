@@ -470,17 +470,25 @@ int gem_hip_match_batch_learned(gem_ctx *ctx, const gem_query_view *queries, uin
                                 uint64_t *level_rows, uint32_t *chosen);
 
 /* Relaxed matching of one query at `threshold` (the reference's main() at
- * -t threshold, in-core/src/main.cu:88-236: relax(), the lattice join and
- * the "Relaxed Query ET <bits>: <count> matches" lines).  The relaxed queries
- * are gem_plan_relaxed_queries's edge sets, in its order (masks[max_rq *
+ * -t threshold, in-core/src/main.cu:88-236: relax(), the lattice join of
+ * in-core/lib/join/src/join_trie.cu:1130-1267 and the "Relaxed Query ET
+ * <bits>: <count> matches" lines).  The relaxed queries are
+ * gem_plan_relaxed_queries's edge sets, in its order (masks[max_rq *
  * GEM_RQ_WORDS]); results[i] is the count of relaxed query i's embeddings
- * (injective, label-preserving maps that keep the edges of set i), computed
- * by gem_hip_match_batch_learned over all relaxed queries as one batch with
- * mlp, max_orders and opts.  *num_rq receives the number of relaxed queries;
- * more than max_rq is GEM_E_LIMIT. */
+ * (injective, label-preserving maps that keep the edges of set i).  They run
+ * as one batch of the batched learned path (mlp, max_orders and opts as in
+ * gem_hip_match_batch_learned).  By default the batch shares what the
+ * reference's lattice shares below the relaxed queries: one filter of the
+ * query at `threshold` (its candidates admit every relaxed query's
+ * embeddings) and one edge table per query edge, read by every relaxed query
+ * that keeps the edge; gem_hip_set_relaxed_sharing(ctx, 0), or streamed
+ * tables, filters and tables each relaxed query on its own edges instead.
+ * Counts are the same either way.  *num_rq receives the number of relaxed
+ * queries; more than max_rq is GEM_E_LIMIT. */
 int gem_hip_match_relaxed(gem_ctx *ctx, const gem_query_view *q, uint32_t threshold, gem_mlp *mlp,
                           uint32_t max_orders, const gem_batch_opts *opts, uint32_t max_rq, uint64_t *masks,
                           gem_join_result *results, uint32_t *num_rq);
+int gem_hip_set_relaxed_sharing(gem_ctx *ctx, int on);
 
 /* Whole plain pipeline for one query: filter -> index -> tables -> GEM order
  * -> join.  Equivalent to the reference's main() for one query at -t 0. */

@@ -36,9 +36,19 @@ def _queries(g, n, sizes, seed):
     return out
 
 
-@pytest.mark.parametrize("T", [1, 2])
-def test_relaxed_counts_equal_oracle(gpu_ctx, er_graph, T):
+@pytest.mark.parametrize("T,share", [(1, True), (2, True), (1, False), (2, False)])
+def test_relaxed_counts_equal_oracle(gpu_ctx, er_graph, T, share):
+    """Shared (one filter at T, one table per edge) and one-by-one relaxed
+    queries both give the oracle's count of every edge subset."""
     gpu_ctx.load_data(er_graph)
+    gpu_ctx.set_relaxed_sharing(share)
+    try:
+        _check_relaxed(gpu_ctx, er_graph, T)
+    finally:
+        gpu_ctx.set_relaxed_sharing(True)
+
+
+def _check_relaxed(gpu_ctx, er_graph, T):
     for q in _queries(er_graph, 4, [5, 6], 300 + T):
         print("relaxed", T, q.edges.tolist(), flush=True)
         res = gpu_ctx.match_relaxed(q, T, cap=CAP)
@@ -67,6 +77,21 @@ def test_relaxed_triangle_on_a_cycle(gpu_ctx):
     tri = Query(np.zeros(3), [(0, 1), (0, 2), (1, 2)])
     res = gpu_ctx.match_relaxed(tri, 1)
     assert [(r["mask"], r["count"]) for r in res] == [(0b111, 0), (0b110, 2 * n), (0b101, 2 * n), (0b011, 2 * n)]
+
+
+def test_relaxed_streamed_tables(gpu_ctx, er_graph):
+    """With streamed tables every relaxed query is filtered and tabled on its
+    own edges; counts equal the shared form's."""
+    gpu_ctx.load_data(er_graph)
+    q = _queries(er_graph, 1, [6], 950)[0]
+    shared = gpu_ctx.match_relaxed(q, 2, cap=CAP)
+    gpu_ctx.set_table_stream(64 << 20, 64 << 10)
+    try:
+        streamed = gpu_ctx.match_relaxed(q, 2, cap=CAP)
+        assert gpu_ctx.stream_stats()["blocks_loaded"] > 0
+    finally:
+        gpu_ctx.set_table_stream(0)
+    assert [(r["mask"], r["count"]) for r in shared] == [(r["mask"], r["count"]) for r in streamed]
 
 
 def test_relaxed_with_model_and_restarts(gpu_ctx, er_graph):
