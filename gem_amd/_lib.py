@@ -98,6 +98,9 @@ SIGNATURES = [
     ("gem_hip_set_graph", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_set_graph_residency", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_set_relaxed_sharing", C.c_int, [C.c_void_p, C.c_int]),
+    ("gem_export_rows", C.c_int, [u32p, C.c_uint64, C.c_uint32, C.c_char_p, C.c_uint64, C.c_void_p]),
+    ("gem_hip_export_embeddings", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_char_p, C.c_uint64, C.c_uint64,
+                                            C.c_void_p]),
     ("gem_hip_set_max_active", C.c_int, [C.c_void_p, C.c_uint32]),
     ("gem_hip_set_table_stream", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     ("gem_hip_stream_stats", C.c_int, [C.c_void_p, C.POINTER(StreamStats)]),
@@ -135,6 +138,10 @@ SIGNATURES = [
     ("gem_hip_match_relaxed", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, C.c_void_p, C.c_uint32,
                                         C.POINTER(BatchOpts), C.c_uint32, u64p, C.POINTER(JoinResult), u32p]),
 ]
+
+class ExportResult(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("digest", C.c_uint64), ("runs", C.c_uint32), ("num_v", C.c_uint32)]
+
 
 GEM_RQ_WORDS = 4
 GEM_MAX_RQ = 1 << 20

@@ -1,7 +1,7 @@
 // Command-line front end with the reference's flags (in-core/src/main.cu:25-44,
 // out-of-core/src/main.cu:412-460):
 //   gem -q <query file | directory of Q_<id>.in> -d <data graph> [-g gpu]
-//       [-t threshold (0..10, default 2)] [--cap N] [--no-fork]
+//       [-t threshold (0..10, default 2)] [--cap N] [--no-fork] [--export FILE]
 // It loads the data graph in the reference text format once, on the host, and
 // runs each query through the C ABI: at threshold 0 the plain pipeline (phase
 // lines and the result line of in-core/src/main.cu:147-236), above it every
@@ -13,6 +13,8 @@
 // child opens it, uploads the data graph, matches and exits, and a child that
 // fails or dies is reported ("[batch-fail] ...") while the batch continues.
 // --no-fork runs the directory in this process with one upload instead.
+// --export (one query file, threshold 0) also writes every embedding to FILE
+// in canonical order (include/gem_hip.h, gem_hip_export_embeddings).
 #include <sys/stat.h>
 #include <sys/wait.h>
 #include <unistd.h>
@@ -35,7 +37,7 @@ namespace {
 void usage() {
   std::fprintf(stderr,
                "usage: gem -q <query file|dir> -d <data graph> [-g gpu] [-t threshold 0..10 (=2)] [--cap N] "
-               "[--no-fork]\n");
+               "[--no-fork] [--export FILE]\n");
 }
 
 bool is_dir(const std::string &p) {
@@ -118,6 +120,8 @@ long long us_since(std::chrono::steady_clock::time_point t0) {
       .count();
 }
 
+std::string g_export;  // --export FILE
+
 int run_query(gem_ctx *ctx, const std::string &path, uint32_t threshold, uint64_t cap) {
   uint32_t nv = 0, ne = 0;
   std::vector<uint32_t> lab(GEM_MAX_QV), edges(2 * GEM_MAX_QE);
@@ -147,6 +151,16 @@ int run_query(gem_ctx *ctx, const std::string &path, uint32_t threshold, uint64_
     for (uint32_t i = 0; i < ne; ++i) all[i / 64] |= 1ull << (i % 64);
     std::printf("\n=== Final Results ===\nRelaxed Query ET %s: %llu matches%s\n\n", edge_bits(all.data(), ne).c_str(),
                 (unsigned long long)r.count, r.saturated ? " (capped)" : "");
+    if (!g_export.empty()) {
+      gem_export_result er{};
+      st = gem_hip_export_embeddings(ctx, &q, g_export.c_str(), 0, 0, &er);
+      if (st) {
+        std::fprintf(stderr, "%s: export: %s (status %d)\n", path.c_str(), gem_hip_last_error(ctx), st);
+        return st;
+      }
+      std::printf("Embeddings written: %llu to %s (digest %016llx)\n", (unsigned long long)er.rows,
+                  g_export.c_str(), (unsigned long long)er.digest);
+    }
     return 0;
   }
   uint32_t nrq = 0;
@@ -278,6 +292,7 @@ int main(int argc, char **argv) {
     else if (a == "--cap" && (v = next())) cap = std::strtoull(v, nullptr, 10);
     else if (a == "--eviction" && (v = next())) { /* accepted; tables are streamed, not evicted */ }
     else if (a == "--no-fork") fork_each = false;
+    else if (a == "--export" && (v = next())) g_export = v;
     else {
       usage();
       return 2;
@@ -293,6 +308,10 @@ int main(int argc, char **argv) {
   }
   std::vector<Job> jobs;
   const bool batch = is_dir(qpath);
+  if (!g_export.empty() && (batch || threshold != 0)) {
+    std::fprintf(stderr, "--export takes one query file at --threshold 0\n");
+    return 2;
+  }
   if (batch) {
     std::string err;
     if (!directory_jobs(qpath, &jobs, &err)) {

@@ -3,6 +3,7 @@
 #include <new>
 #include <thread>
 
+#include "../host/emb_export.h"
 #include "ctx.h"
 
 using namespace gemd;
@@ -27,6 +28,22 @@ int guard(gem_ctx *c, F &&f) {
     c->err = e.what();
     return GEM_E_INVALID;
   }
+}
+
+// Filter, tables and GEM's edge order of q (the plain pipeline before the join).
+std::vector<uint32_t> plain_prepare(gem_ctx *c, const gem_query_view *q) {
+  run_filter(c, q, 0);
+  run_tables(c);
+  const gemh::QueryInfo &Q = c->q.Q;
+  std::vector<uint64_t> sz(Q.ne);
+  for (uint32_t e = 0; e < Q.ne; ++e) {
+    const TableDev &a = c->q.tab[2 * e], &b = c->q.tab[2 * e + 1];
+    sz[e] = a.present ? a.size : b.size;
+  }
+  std::vector<uint32_t> order(Q.ne);
+  const int st = gemh::gem_order(Q, sz.data(), order.data());
+  if (st) throw Error{st, "GEM ordering failed"};
+  return order;
 }
 
 // memcpy of a large host array over up to 16 threads.
@@ -397,17 +414,8 @@ int gem_hip_phase_times(const gem_ctx *c, gem_phase_times *out) {
 int gem_hip_match(gem_ctx *c, const gem_query_view *q, uint64_t cap, gem_join_result *out, uint32_t *order_out) {
   return guard(c, [&]() {
     if (!out) throw Error{GEM_E_INVALID, "null result"};
-    run_filter(c, q, 0);
-    run_tables(c);
+    const std::vector<uint32_t> order = plain_prepare(c, q);
     const gemh::QueryInfo &Q = c->q.Q;
-    std::vector<uint64_t> sz(Q.ne);
-    for (uint32_t e = 0; e < Q.ne; ++e) {
-      const TableDev &a = c->q.tab[2 * e], &b = c->q.tab[2 * e + 1];
-      sz[e] = a.present ? a.size : b.size;
-    }
-    std::vector<uint32_t> order(Q.ne);
-    int st = gemh::gem_order(Q, sz.data(), order.data());
-    if (st) throw Error{st, "GEM ordering failed"};
     if (order_out) std::copy(order.begin(), order.end(), order_out);
     JoinOut jo;
     run_join(c, order.data(), cap, &jo, nullptr, nullptr, 0);
@@ -419,6 +427,32 @@ int gem_hip_match(gem_ctx *c, const gem_query_view *q, uint64_t cap, gem_join_re
     out->work_limited = jo.work_limited ? 1 : 0;
     out->restarts = 0;
     out->work = jo.rows_processed;
+  });
+}
+
+int gem_hip_export_embeddings(gem_ctx *c, const gem_query_view *q, const char *path, uint64_t max_rows,
+                              uint64_t run_bytes, gem_export_result *out) {
+  return guard(c, [&]() {
+    if (!q || !path) throw Error{GEM_E_INVALID, "null argument"};
+    const std::vector<uint32_t> order = plain_prepare(c, q);
+    struct FileSink : EmbSink {
+      gemh::EmbWriter w;
+      FileSink(const char *p, uint32_t nv, uint64_t rb, uint64_t mx) : w(p, nv, rb, mx) {}
+      bool put(const uint32_t *r, uint64_t n, uint32_t) override { return w.put(r, n); }
+    } sink(path, c->q.Q.nv, run_bytes, max_rows);
+    JoinOut jo;
+    run_join_sink(c, order.data(), 0, &jo, nullptr, &sink);
+    if (jo.work_limited) {
+      sink.w.discard();
+      throw Error{GEM_E_LIMIT, "work limit reached before every embedding was enumerated"};
+    }
+    if (jo.sink_stopped) {
+      const int st = sink.w.status();
+      sink.w.discard();
+      throw Error{st ? st : GEM_E_IO, st == GEM_E_LIMIT ? "more than max_rows embeddings" : "writing embeddings failed"};
+    }
+    const int st = sink.w.finish(out);
+    if (st) throw Error{st, "writing embeddings failed"};
   });
 }
 

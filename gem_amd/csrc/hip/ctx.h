@@ -382,7 +382,19 @@ struct JoinOut {
   uint64_t emb_written = 0;
   uint64_t rows_processed = 0;
   bool work_limited = false;
+  bool sink_stopped = false;  // the sink refused rows and the join stopped
 };
+
+// Receives a join's final rows as vertex ids (n rows of nv ids, column u =
+// query vertex u) in host memory; returning false stops the join.
+struct EmbSink {
+  virtual ~EmbSink() = default;
+  virtual bool put(const uint32_t *rows, uint64_t n, uint32_t nv) = 0;
+};
+// The plain join with its final rows going to `sink` (count cap off); null
+// sink: count only, capped at cap.
+void run_join_sink(gem_ctx *c, const uint32_t *order, uint64_t cap, JoinOut *out, gem_step_log *log,
+                   EmbSink *sink);
 void run_join(gem_ctx *c, const uint32_t *order, uint64_t cap, JoinOut *out, gem_step_log *log, uint32_t *emb_out,
               uint64_t emb_capacity);
 

@@ -21,6 +21,7 @@
 // level whose output would exceed the level budget is split into block ranges
 // and each range is finished depth-first, so memory stays bounded.
 #include <algorithm>
+#include <cstring>
 #include <functional>
 
 #include "ctx.h"
@@ -263,9 +264,8 @@ struct JStep {
 
 class JoinRunner {
  public:
-  JoinRunner(gem_ctx *c, const gemh::JoinPlan &plan, uint64_t cap, gem_step_log *log, uint32_t *emb_out,
-             uint64_t emb_cap)
-      : c_(c), Qd_(c->q), cap_(cap), log_(log), emb_out_(emb_out), emb_cap_(emb_cap) {
+  JoinRunner(gem_ctx *c, const gemh::JoinPlan &plan, uint64_t cap, gem_step_log *log, EmbSink *sink)
+      : c_(c), Qd_(c->q), cap_(cap), log_(log), sink_(sink) {
     // step 0: the first table seeds 2-column rows from the 1-column identity
     JStep s0{};
     s0.select = false;
@@ -311,6 +311,7 @@ class JoinRunner {
     out->count = saturated_ ? cap_ : final_total_;
     out->saturated = saturated_;
     out->emb_written = emb_rows_;
+    out->sink_stopped = sink_stopped_;
     out->rows_processed = processed_;
     out->work_limited = work_limited_;
     if (work_limited_) {
@@ -334,8 +335,7 @@ class JoinRunner {
   QueryDev &Qd_;
   uint64_t cap_;
   gem_step_log *log_;
-  uint32_t *emb_out_;
-  uint64_t emb_cap_;
+  EmbSink *sink_;
   std::vector<JStep> steps_;
   std::vector<uint32_t> col_u_;
   std::vector<uint64_t> rows_;
@@ -349,6 +349,8 @@ class JoinRunner {
   uint64_t work_limit_ = 0;
   bool work_limited_ = false;
   uint64_t emb_rows_ = 0;
+  bool sink_stopped_ = false;
+  std::vector<uint32_t> host_rows_;
   hipEvent_t e0_{}, e1_{};
 
   StepArgs args(const JStep &st, const uint32_t *rows, uint64_t n, uint32_t k) {
@@ -431,13 +433,9 @@ class JoinRunner {
   }
 
   void emit_rows(const uint32_t *rows, uint64_t n, uint32_t k) {
-    // rows are final (k == nv columns); store as vertex ids if they fit
+    // rows are final (k == nv columns): vertex ids, in query-vertex order, to the sink
     const uint32_t nv = Qd_.Q.nv;
-    if (!emb_out_ || n == 0) return;
-    if (emb_rows_ + n > emb_cap_) {
-      emb_rows_ = emb_cap_ + 1;  // marks overflow
-      return;
-    }
+    if (!sink_ || n == 0) return;
     hipStream_t s = c_->stream;
     std::vector<const uint32_t *> cc(k);
     for (uint32_t c = 0; c < k; ++c) cc[c] = Qd_.can.as<uint32_t>() + Qd_.can_off[col_u_[c]];
@@ -451,9 +449,14 @@ class JoinRunner {
     k_rows_to_ids<<<(uint32_t)((tot + 255) / 256), 256, 0, s>>>(rows, n, k, dcc.as<const uint32_t *>(),
                                                                  dcu.as<uint32_t>(), nv, dout.as<uint32_t>());
     GEM_LAUNCH_CHECK("k_rows_to_ids");
-    GEM_HIP_CHECK(hipMemcpyAsync(emb_out_ + emb_rows_ * nv, dout.p, n * nv * 4, hipMemcpyDeviceToHost, s));
+    host_rows_.resize(n * nv);
+    GEM_HIP_CHECK(hipMemcpyAsync(host_rows_.data(), dout.p, n * nv * 4, hipMemcpyDeviceToHost, s));
     GEM_HIP_CHECK(hipStreamSynchronize(s));
     emb_rows_ += n;
+    if (!sink_->put(host_rows_.data(), n, nv)) {
+      sink_stopped_ = true;
+      stop_ = true;
+    }
   }
 
   // Apply step si to `n` rows of width k.
@@ -467,7 +470,7 @@ class JoinRunner {
     }
     const JStep &st = steps_[si];
     const bool last = si + 1 == steps_.size();
-    if (last && !emb_out_) {
+    if (last && !sink_) {
       final_count(si, rows, n, k);
       return;
     }
@@ -568,15 +571,33 @@ class JoinRunner {
   }
 };
 
-void run_join(gem_ctx *c, const uint32_t *order, uint64_t cap, JoinOut *out, gem_step_log *log, uint32_t *emb_out,
-              uint64_t emb_capacity) {
+// Keeps the first `cap` rows in a host array; later rows are dropped and
+// mark the array overflowed (the join goes on and counts them).
+struct ArraySink : EmbSink {
+  uint32_t *out;
+  uint64_t cap, rows = 0;
+  bool overflow = false;
+  ArraySink(uint32_t *o, uint64_t c) : out(o), cap(c) {}
+  bool put(const uint32_t *r, uint64_t n, uint32_t nv) override {
+    if (overflow || rows + n > cap) {
+      overflow = true;
+      return true;
+    }
+    std::memcpy(out + rows * nv, r, n * nv * 4);
+    rows += n;
+    return true;
+  }
+};
+
+void run_join_sink(gem_ctx *c, const uint32_t *order, uint64_t cap, JoinOut *out, gem_step_log *log,
+                   EmbSink *sink) {
   QueryDev &Qd = c->q;
   if (!Qd.tabled) throw Error{GEM_E_STATE, "edge tables have not been built"};
   gemh::JoinPlan plan;
   if (gemh::make_join_plan(Qd.Q, order, &plan)) throw Error{GEM_E_INVALID, "edge order is not a connected order"};
   hipStream_t s = c->stream;
   GEM_HIP_CHECK(hipEventRecord(c->ev[4], s));
-  JoinRunner jr(c, plan, emb_out ? 0 : cap, log, emb_out, emb_capacity);
+  JoinRunner jr(c, plan, sink ? 0 : cap, log, sink);
   jr.first_edge_ = plan.first_edge;
   jr.run(out);
   GEM_HIP_CHECK(hipEventRecord(c->ev[5], s));
@@ -585,7 +606,17 @@ void run_join(gem_ctx *c, const uint32_t *order, uint64_t cap, JoinOut *out, gem
   GEM_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
   c->times.join_ms = ms;
   prof_flush(c);
-  if (emb_out && out->emb_written > emb_capacity) out->emb_written = 0;
+}
+
+void run_join(gem_ctx *c, const uint32_t *order, uint64_t cap, JoinOut *out, gem_step_log *log, uint32_t *emb_out,
+              uint64_t emb_capacity) {
+  if (!emb_out) {
+    run_join_sink(c, order, cap, out, log, nullptr);
+    return;
+  }
+  ArraySink sink(emb_out, emb_capacity);
+  run_join_sink(c, order, cap, out, log, &sink);
+  out->emb_written = sink.overflow ? 0 : sink.rows;
 }
 
 }  // namespace gemd

@@ -269,6 +269,16 @@ class GemContext:
                  "work_limited": bool(r.work_limited), "work": int(r.work), "restarts": int(r.restarts)}
                 for m, r in zip(masks_to_ints(masks[:got.value]), res)]
 
+    def export_embeddings(self, query, path, max_rows=0, run_bytes=0):
+        """Every embedding of `query` (plain pipeline) in a canonical file
+        (gem_amd.export); dict(rows, digest, runs, num_v)."""
+        from ._lib import ExportResult
+        r = ExportResult()
+        v = query.view()
+        self._check(self._lib.gem_hip_export_embeddings(self._h, C.byref(v), str(path).encode(), int(max_rows),
+                                                        int(run_bytes), C.byref(r)), "gem_hip_export_embeddings")
+        return {"rows": int(r.rows), "digest": int(r.digest), "runs": int(r.runs), "num_v": int(r.num_v)}
+
     def match(self, query, cap=0):
         """Whole plain pipeline with GEM order; returns (count, saturated, order)."""
         r = self.match_full(query, cap)

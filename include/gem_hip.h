@@ -490,6 +490,31 @@ int gem_hip_match_relaxed(gem_ctx *ctx, const gem_query_view *q, uint32_t thresh
                           gem_join_result *results, uint32_t *num_rq);
 int gem_hip_set_relaxed_sharing(gem_ctx *ctx, int on);
 
+/* ------------------------------------------------------ embedding export */
+/* No reference counterpart: the reference only counts (in-core/lib/join/src/
+ * join_trie.cu:1204 never materialises final rows).  An embedding file holds
+ * rows of num_v vertex ids (column u = image of query vertex u) in canonical
+ * order, ascending lexicographic over the columns: a 32-byte header
+ * ("GEMEMB01", u32 num_v, u32 0, u64 rows, u64 digest), then the rows, u32
+ * little-endian.  digest = FNV-1a 64 of the row bytes in that order.  Rows are
+ * sorted in runs of run_bytes of host memory (0 = 256 MiB); runs beyond the
+ * first go to temporary files "<path>.run<i>" and are merged into `path`. */
+typedef struct gem_export_result {
+  uint64_t rows;    /* embeddings written                        */
+  uint64_t digest;  /* FNV-1a 64 of the canonical row bytes       */
+  uint32_t runs;    /* sorted runs merged                         */
+  uint32_t num_v;
+} gem_export_result;
+/* Canonical file of n host rows given in any order (host only). */
+int gem_export_rows(const uint32_t *rows, uint64_t n, uint32_t num_v, const char *path, uint64_t run_bytes,
+                    gem_export_result *out);
+/* Every embedding of q (the plain pipeline: filter, tables, GEM order, join)
+ * streamed from the GPU into a canonical file.  The join runs without a count
+ * cap; more than max_rows embeddings (0 = no bound) stops it with GEM_E_LIMIT
+ * and leaves no file, as does the work limit of gem_hip_set_work_limit. */
+int gem_hip_export_embeddings(gem_ctx *ctx, const gem_query_view *q, const char *path, uint64_t max_rows,
+                              uint64_t run_bytes, gem_export_result *out);
+
 /* Whole plain pipeline for one query: filter -> index -> tables -> GEM order
  * -> join.  Equivalent to the reference's main() for one query at -t 0. */
 int gem_hip_match(gem_ctx *ctx, const gem_query_view *q, uint64_t cap,

@@ -147,8 +147,12 @@ def test_cli_relaxed_lines_and_isolation(tmp_path, er_graph):
     assert got == want
     # the same data graph from its binary CSR cache, one query at -t 0
     er_graph.save_binary(tmp_path / "d.bin")
-    r = subprocess.run([CLI, "-q", str(qd / "Q_0.in"), "-d", str(tmp_path / "d.bin"), "-t", "0", "--cap", str(CAP)],
-                       capture_output=True, text=True, timeout=300)
+    r = subprocess.run([CLI, "-q", str(qd / "Q_0.in"), "-d", str(tmp_path / "d.bin"), "-t", "0", "--cap", str(CAP),
+                        "--export", str(tmp_path / "q0.emb")], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     got = re.findall(r"Relaxed Query ET ([01]+): (\d+) matches", r.stdout)
-    assert got == [(edge_bits((1 << qs[0].num_e) - 1, qs[0].num_e), str(oracle.count(er_graph, qs[0], cap=CAP)["count"]))]
+    n0 = oracle.count(er_graph, qs[0], cap=CAP)["count"]
+    assert got == [(edge_bits((1 << qs[0].num_e) - 1, qs[0].num_e), str(n0))]
+    from gem_amd.export import read_embeddings
+    rows, head = read_embeddings(tmp_path / "q0.emb")
+    assert head["rows"] == rows.shape[0] == n0 and f"Embeddings written: {n0} " in r.stdout
