@@ -918,6 +918,7 @@ struct HipSchedBackend {
   ExtSlot &slot;
   TableCache *tc;
   bool verbose;
+  uint64_t cap_;
   using clk = std::chrono::steady_clock;
   clk::time_point last_print = clk::now();
   double t_len = 0, t_ext = 0;
@@ -1061,6 +1062,30 @@ struct HipSchedBackend {
 
   void admit(uint32_t qi) { seed_query(c, *qs[qi]); }
 
+  // The attempt of query qi went past its limit with its levels still
+  // pending.  It goes on when it has counted at least a quarter of the cap and,
+  // at the rate it counted so far, needs less work to pass the cap than 4/5 of
+  // the next attempt's limit (a new attempt starts from nothing, and an
+  // attempt whose cost is near its limit would otherwise end or restart by
+  // chance: the order rows come out of a level in varies from run to run).
+  // Its limit becomes what it has spent plus 1.5 times that estimate.
+  bool prolong(uint32_t qi) {
+    VQueryState &V = *qs[qi];
+    gemh::SQuery<DevBuf> &Q = V.s;
+    if (!cap_ || Q.count < cap_ / 4 || Q.count == 0) return false;
+    const uint64_t spent = Q.work - Q.attempt_start;
+    const double need = (double)spent * ((double)(cap_ + 1) / (double)Q.count - 1.0);
+    const uint64_t next = attempt_limit(V, Q.attempts + 1);
+    if (next && need >= 0.8 * (double)next) return false;
+    Q.attempt_limit = spent + (uint64_t)(1.5 * need) + 1;
+    if (verbose) {
+      std::fprintf(stderr, "[vjoin] query %u: attempt %u goes on (%.0f%% of the cap counted), limit %.3g\n", qi,
+                   Q.attempts, 100.0 * (double)Q.count / (double)cap_, (double)Q.attempt_limit);
+      std::fflush(stderr);
+    }
+    return true;
+  }
+
   // The attempt of query qi went past its limit (its levels are dropped):
   // the next ranked order, with the next attempt's limit.
   bool restart(uint32_t qi) {
@@ -1099,8 +1124,27 @@ struct HipSchedBackend {
     return true;
   }
 
+  // wall time and iterations by the number of queries active (tail: at most 4)
+  clk::time_point last_iter = clk::now();
+  size_t last_active = 0;
+  double t_tail = 0, t_body = 0;
+  uint64_t it_tail = 0, it_body = 0;
+
   void on_iteration(uint64_t iter, size_t active, uint64_t live, uint64_t budget) {
-    if (!verbose || clk::now() - last_print < std::chrono::seconds(2)) return;
+    const auto now = clk::now();
+    if (last_active) {
+      const double dt = std::chrono::duration<double>(now - last_iter).count();
+      if (last_active <= 4) {
+        t_tail += dt;
+        ++it_tail;
+      } else {
+        t_body += dt;
+        ++it_body;
+      }
+    }
+    last_iter = now;
+    last_active = active;
+    if (!verbose || now - last_print < std::chrono::seconds(2)) return;
     last_print = clk::now();
     std::fprintf(stderr, "[vjoin] iteration %llu: %zu of %zu queries active, pending levels %.2f of %.2f GiB\n",
                  (unsigned long long)iter, active, qs.size(), live / 1073741824.0, budget / 1073741824.0);
@@ -1159,7 +1203,7 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   }
   const char *verbose_env = std::getenv("GEM_VERBOSE");
   const bool verbose = verbose_env && verbose_env[0] == '1';
-  HipSchedBackend be{c_, qs, slot, tc.get(), verbose};
+  HipSchedBackend be{c_, qs, slot, tc.get(), verbose, cap_};
   gemh::SchedConfig cfg;
   cfg.cap = cap_;
   cfg.work_limit = work_limit_;
@@ -1174,13 +1218,15 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
     std::fprintf(stderr,
                  "[vjoin] run of %zu queries: %llu iterations in %.2f s: %llu length passes %.2f s, extend steps "
                  "%.2f s (%llu 256-row tasks); budget %.2f GiB, peak charged %.2f GiB, %llu forced iterations, "
-                 "%llu restarts, %llu queries started (at most %u at once)\n",
+                 "%llu restarts, %llu queries started (at most %u at once); %llu iterations with more than 4 "
+                 "queries active %.2f s, %llu with at most 4 %.2f s\n",
                  qs.size(), (unsigned long long)stats_.iterations,
                  std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),
                  (unsigned long long)stats_.length_passes, be.t_len, be.t_ext, (unsigned long long)be.ext_tasks,
                  total_budget / 1073741824.0, stats_.peak_charged / 1073741824.0,
                  (unsigned long long)stats_.forced, (unsigned long long)stats_.restarts,
-                 (unsigned long long)stats_.admitted, cfg.max_active);
+                 (unsigned long long)stats_.admitted, cfg.max_active, (unsigned long long)be.it_body, be.t_body,
+                 (unsigned long long)be.it_tail, be.t_tail);
     std::fflush(stderr);
   }
 }

@@ -26,7 +26,9 @@
 // (attempt_limit).  When an attempt goes past it, its pending levels are
 // dropped and the backend may start the query again (another vertex order, a
 // larger limit); the final count does not depend on the order, so whichever
-// attempt ends gives the query's result.
+// attempt ends gives the query's result.  Before that, the backend may prolong
+// the attempt instead (raise its limit), when what it counted so far says it
+// is near its end.
 #pragma once
 
 #include <algorithm>
@@ -196,6 +198,9 @@ inline int plan_chunks(const std::vector<ChunkCand> &cand, uint64_t left, uint64
 //   // again for a new attempt (count, level_rows, attempt fields) and return
 //   // true, or return false to end it unfinished
 //   bool restart(uint32_t q);
+//   // query q's attempt went past its limit, its levels still pending: raise
+//   // its attempt_limit and return true to let it go on, or return false
+//   bool prolong(uint32_t q);
 //   // query q starts: seed its first level (or mark it done); only called
 //   // for queries whose stack is empty and that are not done
 //   void admit(uint32_t q);
@@ -435,7 +440,7 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
         S.work_limited = true;
         S.done = true;
         S.stack.clear();
-      } else if (S.attempt_limit && S.work - S.attempt_start > S.attempt_limit) {
+      } else if (S.attempt_limit && S.work - S.attempt_start > S.attempt_limit && !be.prolong(qi)) {
         S.stack.clear();
         if (be.restart(qi)) {
           ++st->restarts;

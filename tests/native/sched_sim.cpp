@@ -133,6 +133,7 @@ struct SimBackend {
   }
   void on_iteration(uint64_t, size_t, uint64_t, uint64_t) {}
   bool restart(uint32_t) { return false; }
+  bool prolong(uint32_t) { return false; }
   void admit(uint32_t) {}
 };
 
