@@ -12,6 +12,12 @@
 
 namespace gemd {
 
+static uint64_t pool_retain_bytes() {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
+  return total_b / 2;
+}
+
 // Vertex order of first appearance along GEM's edge order.
 static std::vector<uint32_t> gem_vertex_order(const QueryDev &Qd) {
   const gemh::QueryInfo &Q = Qd.Q;
@@ -116,9 +122,10 @@ void run_match_batch(gem_ctx *c, const gem_query_view *qs, uint32_t nq, const ui
   GEM_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
   c->times.join_ms = ms;
   prof_flush(c);
-  // the run's level buffers and tables are gone: empty segments go back to
-  // the driver, so that repeated batches start from the same pool state
-  pool_trim();
+  // the run's level buffers and tables are gone: empty segments beyond half
+  // of the device memory go back to the driver (the rest stays for the next
+  // run, whose first allocations would otherwise wait for the driver)
+  pool_trim(pool_retain_bytes());
 }
 
 void run_order_candidates(gem_ctx *c, uint32_t max_orders, uint64_t seed, uint32_t *orders, int32_t *feats,
@@ -288,9 +295,10 @@ static void end_run(gem_ctx *c, double sum_filter, double sum_tables) {
   c->times.filter_ms = sum_filter * 1e3;
   c->times.tables_ms = sum_tables * 1e3;
   prof_flush(c);
-  // the run's level buffers and tables are gone: empty segments go back to
-  // the driver, so that repeated batches start from the same pool state
-  pool_trim();
+  // the run's level buffers and tables are gone: empty segments beyond half
+  // of the device memory go back to the driver (the rest stays for the next
+  // run, whose first allocations would otherwise wait for the driver)
+  pool_trim(pool_retain_bytes());
 }
 
 void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, gem_mlp *mlp, uint32_t max_orders,

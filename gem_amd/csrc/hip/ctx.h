@@ -58,7 +58,7 @@ void pool_release(hipStream_t s);
 size_t pool_reusable_bytes();
 // Return empty segments and parked small blocks to the driver (waits for the
 // device); the bytes released.
-uint64_t pool_trim();
+uint64_t pool_trim(uint64_t retain = 0);
 // In-use / peak / held bytes of the allocator (optionally restarting the peak).
 gemh::PoolStats pool_stats(bool reset_peak);
 

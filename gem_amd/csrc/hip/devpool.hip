@@ -7,9 +7,10 @@
 // sized by a bound, so hardly two requests are alike.  Driver allocations
 // (hipMalloc / hipMallocAsync) cost tens of microseconds each and far more for
 // large sizes, so memory is taken from the driver in 8 GiB segments and small
-// size classes and reused.  Empty segments and parked small blocks go back to
-// the driver at the end of every batched run (pool_trim), when the driver
-// refuses an allocation, and when a context is destroyed.
+// size classes and reused.  Parked small blocks, and empty segments beyond
+// half of the device memory, go back to the driver at the end of every
+// batched run (pool_trim); everything empty goes back when the driver refuses
+// an allocation, and when a context is destroyed.
 #include <mutex>
 
 #include "../host/pool_core.h"
@@ -64,9 +65,9 @@ size_t pool_reusable_bytes() {
   return st.empty_segments + st.cached_small;
 }
 
-uint64_t pool_trim() {
+uint64_t pool_trim(uint64_t retain) {
   std::lock_guard<std::mutex> lk(g_mu);
-  return core().trim();
+  return core().trim(retain);
 }
 
 gemh::PoolStats pool_stats(bool reset_peak) {

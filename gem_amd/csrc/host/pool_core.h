@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <iterator>
 #include <map>
 #include <string>
 #include <utility>
@@ -143,26 +144,32 @@ class PoolCore {
   }
 
   // Return parked small blocks and empty segments to the backing allocator
-  // (after waiting for all work).  Returns the bytes released.
-  uint64_t trim() {
+  // (after waiting for all work), as long as the memory held exceeds
+  // `retain` bytes: segments first, then small blocks.  Returns the bytes
+  // released.
+  uint64_t trim(uint64_t retain = 0) {
     if (small_.empty() && !has_empty_segment()) return 0;
+    if (st_.held <= retain) return 0;
     if (b_.sync) b_.sync(b_.user);
     uint64_t n = 0;
-    for (auto &kv : small_)
-      for (void *p : kv.second) {
-        b_.release(p, b_.user);
-        n += kv.first.second;
-      }
-    small_.clear();
-    st_.cached_small = 0;
     for (auto it = segs_.begin(); it != segs_.end();) {
-      if (it->second.used == 0) {
+      if (it->second.used == 0 && st_.held - n > retain) {
         b_.release(it->second.base, b_.user);
         n += it->second.size;
         it = segs_.erase(it);
       } else {
         ++it;
       }
+    }
+    for (auto it = small_.begin(); it != small_.end() && st_.held - n > retain;) {
+      std::vector<void *> &v = it->second;
+      while (!v.empty() && st_.held - n > retain) {
+        b_.release(v.back(), b_.user);
+        v.pop_back();
+        n += it->first.second;
+        st_.cached_small -= it->first.second;
+      }
+      it = v.empty() ? small_.erase(it) : std::next(it);
     }
     st_.held -= n;
     return n;

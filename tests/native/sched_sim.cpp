@@ -275,8 +275,17 @@ int pool_cases() {
                 ", \"classes_ok\": %s}",
                 round ? ", " : "", in_use_full, expect, after.in_use, classes_ok ? "true" : "false");
   }
+  // a trim that retains 10 GiB keeps one 8 GiB segment and some small
+  // blocks, and holds at most that much; a second one keeps it all (nothing
+  // to give back down to 10 GiB); a full trim returns the rest
+  const uint64_t held_full = pool.stats().held;
+  const uint64_t rel_retain = pool.trim(10ull << 30);
+  const uint64_t held_retain = pool.stats().held;
+  const uint64_t rel_again = pool.trim(10ull << 30);
   pool.trim();
-  std::printf("], \"held_after_trim\": %" PRIu64 "}\n", pool.stats().held);
+  std::printf("], \"held_full\": %" PRIu64 ", \"released_retain\": %" PRIu64 ", \"held_retain\": %" PRIu64
+              ", \"released_again\": %" PRIu64 ", \"held_after_trim\": %" PRIu64 "}\n",
+              held_full, rel_retain, held_retain, rel_again, pool.stats().held);
   return 0;
 }
 

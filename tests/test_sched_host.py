@@ -59,6 +59,10 @@ def test_pool_class_boundary(sim):
         assert rnd["classes_ok"]
         assert rnd["in_use_full"] == rnd["expect"]
         assert rnd["in_use_after"] == 0
+    # retaining trims: held drops to at most the retained amount, by whole
+    # segments and blocks, and a second such trim releases nothing
+    assert r["held_full"] > (10 << 30) >= r["held_retain"] > 0
+    assert r["released_retain"] == r["held_full"] - r["held_retain"] and r["released_again"] == 0
     assert r["held_after_trim"] == 0
 
 
