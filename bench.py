@@ -159,6 +159,7 @@ def main():
                     help="stream edge tables through a device cache of this size (e.g. 8G); default: 8G for ooc24, "
                          "tables in HBM otherwise")
     ap.add_argument("--stream-slot", default="1M", help="cache slot size for streamed tables")
+    ap.add_argument("--no-racing", action="store_true", help="no twins racing the batch's stragglers")
     ap.add_argument("--graph-host", choices=["auto", "on", "off"], default="auto",
                     help="keep the data graph's neighbour array in pinned host memory (auto: on for ooc24)")
     args = ap.parse_args()
@@ -210,6 +211,7 @@ def main():
     if stream_cache:
         ctx.set_table_stream(stream_cache, parse_bytes(args.stream_slot))
         log(f"[bench] edge tables streamed from host memory through a {args.stream_cache} device cache")
+    ctx.set_racing(not args.no_racing)
 
     # ---- batched path with the learned order (the measured value)
     from gem_amd.learned import OrderModel, match_batch_learned
@@ -410,7 +412,9 @@ def main():
                        "learned order (MLP-ranked candidate orders)" if model is not None
                        else "candidate orders in generation order"),
                    "cap": cap, "restarts": {"first_attempt_work": first, "growth": args.restart_growth,
-                                            "width": args.restart_width}, "work_limit": wl,
+                                            "width": args.restart_width,
+                                            "racing_twins": not args.no_racing and not stream_cache},
+                   "work_limit": wl,
                    "tuning": tuning or None},
         "embeddings_per_s": round(eps, 1),
         "pass_seconds": [round(x, 3) for x in pass_s],

@@ -132,6 +132,12 @@ int gem_hip_set_max_active(gem_ctx *c, uint32_t queries) {
   return GEM_OK;
 }
 
+int gem_hip_set_racing(gem_ctx *c, int on) {
+  if (!c) return GEM_E_INVALID;
+  c->racing = on != 0;
+  return GEM_OK;
+}
+
 int gem_hip_set_relaxed_sharing(gem_ctx *c, int on) {
   if (!c) return GEM_E_INVALID;
   c->relaxed_share = on != 0;

@@ -268,12 +268,55 @@ static double run_items(gem_ctx *c, const std::vector<BatchItem> &items, gem_mlp
     V.cur = 0;
     ptrs.push_back(&V);
   }
+  // Racing: each query with restarts gets a twin on the same candidates and
+  // tables; the twins start, in the same order, once every query of the batch
+  // has started, when a running query ends and leaves room (the tail, where
+  // the few queries left would leave most of the device idle), and whichever
+  // of a pair ends first with a count cancels the other.  Counts do not
+  // depend on the order, so either gives the query's result.
+  std::vector<VQueryState> twins;
+  std::vector<int64_t> twin_of(n, -1);
+  if (c->racing && !streamed && o.first_attempt) {
+    twins.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+      VQueryState &P = st[i];
+      if (P.ranked.size() < 2) continue;
+      VQueryState &T = twins[i];
+      T.qd = P.qd;
+      T.Qp = P.Qp;
+      T.edge_map = P.edge_map;
+      T.first_attempt = P.first_attempt;
+      T.growth = P.growth;
+      T.width = P.width;
+      T.max_attempts = P.max_attempts;
+      T.ranked = P.ranked;
+      T.ranked_idx = P.ranked_idx;
+      T.mnb_all = P.mnb_all;
+      T.mnd_all = P.mnd_all;
+      T.twin = true;
+      replan(c, T, T.ranked[1]);
+      twin_of[i] = (int64_t)ptrs.size();
+      T.s.partner = (int64_t)i;
+      P.s.partner = twin_of[i];
+      ptrs.push_back(&T);
+    }
+  }
   const double t_orders = std::chrono::duration<double>(clk::now() - t0).count();
   VBatchRunner runner(c, o.cap, o.work_limit, o.level_budget, streamed);
+  if (!twins.empty()) runner.set_primaries((uint32_t)n);
   runner.run(ptrs);
   for (size_t i = 0; i < n; ++i) {
-    fill_result(st[i], o.cap, &results[i], level_rows ? level_rows + i * GEM_MAX_QV : nullptr);
-    if (chosen) chosen[i] = st[i].ranked_idx[st[i].cur];
+    const VQueryState *W = &st[i];
+    uint64_t twin_work = 0;
+    if (twin_of[i] >= 0) {
+      const VQueryState &T = twins[i];
+      twin_work = T.s.work;
+      // the twin's result when it ended with a count and the query did not
+      if (T.s.done && !T.s.cancelled && !T.s.work_limited && (st[i].s.cancelled || st[i].s.work_limited)) W = &T;
+    }
+    fill_result(*W, o.cap, &results[i], level_rows ? level_rows + i * GEM_MAX_QV : nullptr);
+    results[i].work = st[i].s.work + twin_work;
+    if (chosen) chosen[i] = W->ranked_idx[W->cur];
   }
   return t_orders;
 }

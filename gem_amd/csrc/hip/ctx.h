@@ -269,6 +269,7 @@ struct gem_ctx {
   uint32_t max_active = 64;                 // batched path: queries running at once (0 = all)
   int graph_host = 0;                       // next data graph: neighbours stay in pinned host memory
   bool relaxed_share = true;                // relaxed queries share one filter and one table build
+  bool racing = true;                       // batched learned path: twins race stragglers
   bool use_graph = false;                   // replay the batched extend step as a hipGraph
   // batched path with streamed edge tables: values in pinned host memory,
   // a device cache of stream_cache_bytes in slots of stream_slot_bytes

@@ -75,6 +75,12 @@ struct VQueryState {
   uint32_t max_attempts = 0;
   std::vector<double> rate;  // per ranked order: most embeddings per unit of work in its ended attempts
   uint32_t mnb_all = 1, mnd_all = 0;
+  // A racing twin (SQuery::partner set, index past the batch): attempt a runs
+  // ranked[(a + shift) % size] while a < size; shift is set when it starts so
+  // that its first attempt takes the order after its partner's current one,
+  // at its partner's current limit.
+  bool twin = false;
+  uint32_t shift = 0;
 };
 
 void build_vlevels(gem_ctx *c, VQueryState &S);
@@ -89,6 +95,9 @@ class VBatchRunner {
   // and are read through a device block cache of the context's size
   VBatchRunner(gem_ctx *c, uint64_t cap, uint64_t work_limit, uint64_t budget_bytes, bool streamed = false)
       : c_(c), cap_(cap), work_limit_(work_limit), budget_bytes_(budget_bytes), streamed_(streamed) {}
+  // queries [0, n) are the batch, later ones race a partner among them
+  // (SQuery::partner, VQueryState::twin); 0 = no racing
+  void set_primaries(uint32_t n) { primaries_ = n; }
   void run(std::vector<VQueryState *> &qs);
   const gemh::SchedStats &stats() const { return stats_; }
 
@@ -96,6 +105,7 @@ class VBatchRunner {
   gem_ctx *c_;
   uint64_t cap_, work_limit_, budget_bytes_;
   bool streamed_;
+  uint32_t primaries_ = 0;
   gemh::SchedStats stats_;
 };
 

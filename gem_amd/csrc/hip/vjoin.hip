@@ -1060,7 +1060,30 @@ struct HipSchedBackend {
     t_ext += std::chrono::duration<double>(clk::now() - t0).count();
   }
 
-  void admit(uint32_t qi) { seed_query(c, *qs[qi]); }
+  void admit(uint32_t qi) {
+    VQueryState &V = *qs[qi];
+    if (V.twin && V.s.partner >= 0) {
+      // a racing twin starts at its partner's current limit, on the order
+      // after the one its partner runs
+      const VQueryState &P = *qs[(size_t)V.s.partner];
+      const uint32_t K = (uint32_t)V.ranked.size();
+      const uint32_t a = P.s.attempts;
+      V.shift = (P.cur + 1 + K - a % K) % K;
+      V.cur = (a + V.shift) % K;
+      replan(c, V, V.ranked[V.cur]);
+      if (V.mnb_all > slot.max_nb || V.mnd_all > slot.max_ndup)
+        throw Error{GEM_E_STATE, "racing order needs more launch LDS than the run was sized for"};
+      V.s.attempts = a;
+      V.s.attempt_start = V.s.work;
+      V.s.attempt_limit = attempt_limit(V, a);
+      if (verbose) {
+        std::fprintf(stderr, "[vjoin] query %u races query %lld: order %u, limit %.3g\n", qi,
+                     (long long)V.s.partner, V.ranked_idx[V.cur], (double)V.s.attempt_limit);
+        std::fflush(stderr);
+      }
+    }
+    seed_query(c, V);
+  }
 
   // The attempt of query qi went past its limit with its levels still
   // pending.  It goes on when it has counted at least a quarter of the cap and,
@@ -1101,7 +1124,7 @@ struct HipSchedBackend {
     if (spent) V.rate[V.cur] = std::max(V.rate[V.cur], (double)Q.count / (double)spent);
     // every ranked order once, then the one that counted fastest (the ranking
     // again while none has counted anything)
-    V.cur = a % K;
+    V.cur = (a + V.shift) % K;
     if (a >= K) {
       uint32_t best = 0;
       for (uint32_t j = 1; j < K; ++j)
@@ -1194,7 +1217,9 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   std::vector<gemh::SQuery<DevBuf> *> sq;
   for (VQueryState *S : qs) {
     gemh::SQuery<DevBuf> &Q = S->s;
+    const int64_t partner = Q.partner;
     Q = gemh::SQuery<DevBuf>{};
+    Q.partner = partner;
     Q.nv = S->qd->Q.nv;
     Q.nb = S->max_nb;
     Q.level_rows.assign(Q.nv, 0);
@@ -1209,6 +1234,7 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   cfg.work_limit = work_limit_;
   cfg.budget = total_budget;
   cfg.max_active = c_->max_active;
+  cfg.primaries = primaries_;
   const auto t0 = std::chrono::steady_clock::now();
   stats_ = gemh::SchedStats{};
   gemh::run_schedule(be, sq, cfg, &stats_);

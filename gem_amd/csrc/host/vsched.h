@@ -54,6 +54,10 @@ struct SchedConfig {
   // is seeded when it starts)
   uint32_t max_active = 0;
   uint32_t tail_growth = 8;  // most a quota grows once no query waits (see run_schedule)
+  // queries [0, primaries) are the batch; later ones (0 = none) race a
+  // partner among them, start only once every primary has started, and do
+  // not count as waiting for the quotas
+  uint32_t primaries = 0;
 };
 
 struct SchedStats {
@@ -99,6 +103,10 @@ struct SQuery {
   // measured final-level pieces of the current attempt: output bounds and
   // embeddings counted (their ratio sizes the next capped piece)
   uint64_t fin_bound = 0, fin_got = 0;
+  // Racing: another query of the schedule computing the same count (another
+  // vertex order).  Whichever ends first with a count cancels the other.
+  int64_t partner = -1;
+  bool cancelled = false;
 };
 
 // One unit of a launch: rows [row_begin, row_end) of a pending level of
@@ -221,6 +229,16 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
   };
   std::vector<uint8_t> started(qs.size(), 0);
   size_t next = 0;
+  const size_t nprim = cfg.primaries ? std::min<size_t>(cfg.primaries, qs.size()) : qs.size();
+  // a query that ends with a count (or saturated) cancels its partner
+  auto cancel_partner = [&](SQuery<Buf> &S) {
+    if (S.partner < 0) return;
+    SQuery<Buf> &P = *qs[(size_t)S.partner];
+    if (P.done) return;
+    P.done = true;
+    P.cancelled = true;
+    P.stack.clear();
+  };
   for (;;) {
     // ---- admission (index order)
     {
@@ -241,6 +259,7 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
     std::vector<uint32_t> act;
     for (uint32_t i = 0; i < qs.size(); ++i)
       if (started[i] && !qs[i]->done && !qs[i]->stack.empty()) act.push_back(i);
+
     if (act.empty()) break;
     ++st->iterations;
     const uint64_t live = live_bytes();
@@ -294,8 +313,8 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
     // many times the work it needs to reach the cap)
     uint64_t shares = act.size();
     if (cfg.max_active)
-      shares = next < qs.size() ? cfg.max_active
-                                : std::max<uint64_t>(shares, cfg.max_active / std::max<uint32_t>(cfg.tail_growth, 1));
+      shares = next < nprim ? cfg.max_active
+                            : std::max<uint64_t>(shares, cfg.max_active / std::max<uint32_t>(cfg.tail_growth, 1));
     const uint64_t quota = budget / std::max<uint64_t>(shares, 1);
     for (uint32_t qi : act) {
       SQuery<Buf> &S = *qs[qi];
@@ -432,10 +451,12 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
     }
     for (uint32_t qi : act) {
       SQuery<Buf> &S = *qs[qi];
+      if (S.done) continue;  // cancelled by its partner this iteration
       if (cfg.cap && S.count > cfg.cap) {
         S.saturated = true;
         S.done = true;
         S.stack.clear();
+        cancel_partner(S);
       } else if (cfg.work_limit && S.work > cfg.work_limit) {
         S.work_limited = true;
         S.done = true;
@@ -450,6 +471,7 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
         }
       } else if (S.stack.empty()) {
         S.done = true;
+        cancel_partner(S);
       }
     }
   }

@@ -161,6 +161,10 @@ class GemContext:
         `chunk`-entry pieces spread over waves (load balance only)."""
         self._check(self._lib.gem_hip_set_heavy_split(self._h, int(threshold), int(chunk)), "gem_hip_set_heavy_split")
 
+    def set_racing(self, on=True):
+        """Twins race the batched learned path's stragglers (default on)."""
+        self._check(self._lib.gem_hip_set_racing(self._h, int(bool(on))), "gem_hip_set_racing")
+
     def set_relaxed_sharing(self, on=True):
         """Relaxed queries share one filter at the threshold and one table per
         query edge (default), or are filtered and tabled one by one."""

@@ -376,6 +376,13 @@ int gem_hip_set_graph(gem_ctx *ctx, int on);
  * the budget over the running queries after that.  Counts do not depend on
  * it.  Set by the autotuner. */
 int gem_hip_set_max_active(gem_ctx *ctx, uint32_t queries);
+/* on != 0 (default): in the learned path with restarts, every query with at
+ * least two candidate orders gets a twin that runs another order; twins start
+ * once every query of the batch has started, as running queries end (the
+ * batch's tail), at their query's current attempt limit, and whichever of a
+ * pair ends first with a count cancels the other.  Results are the same; a
+ * query's `work` includes its twin's.  Not used with streamed tables. */
+int gem_hip_set_racing(gem_ctx *ctx, int on);
 
 /* Out-of-core edge tables for the batched paths (replaces the reference's
  * out-of-core MemoryManager / LRUEvictPolicy, out-of-core/lib/structure/src/

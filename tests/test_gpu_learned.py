@@ -120,8 +120,12 @@ def test_shipped_model_gains_on_heldout_workload(gpu_ctx):
     model = OrderModel.load(gpu_ctx, path)
     assert model.meta["workload"] == "rmat20_train"
     kw = dict(max_orders=8, cap=10 ** 9, first_attempt=10 ** 8)
-    base = match_batch_learned(gpu_ctx, qs, None, **kw)
-    lrn = match_batch_learned(gpu_ctx, qs, model, **kw)
+    gpu_ctx.set_racing(False)  # the ranking's own effect: no twins racing stragglers
+    try:
+        base = match_batch_learned(gpu_ctx, qs, None, **kw)
+        lrn = match_batch_learned(gpu_ctx, qs, model, **kw)
+    finally:
+        gpu_ctx.set_racing(True)
     assert not any(r["work_limited"] for r in base + lrn)
     assert [(r["count"], r["saturated"]) for r in base] == [(r["count"], r["saturated"]) for r in lrn]
     wb, wl = sum(r["work"] for r in base), sum(r["work"] for r in lrn)

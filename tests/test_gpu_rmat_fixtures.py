@@ -5,8 +5,9 @@ saturated flag (cap 1e9) of every query of the R-MAT 2^20 / 100-query
 workload, every 16th query of the R-MAT 2^22 / 512-query batch and every 8th
 query of the 8-24-vertex learned-order workload on the 2^20 graph.  Here the
 same seeded generators rebuild the graphs and queries (checked against the
-recorded digests), and the batched path (model-ranked orders with restarts,
-as the benchmark runs it), the batched path without the model, and the plain
+recorded digests), and the batched path (model-ranked orders with restarts
+and racing twins, as the benchmark runs it; without the twins; without the
+model), and the plain
 path on a sample must reproduce every recorded result.  Queries the oracle
 did not finish are listed in the fixture and skipped here.
 """
@@ -77,6 +78,24 @@ def test_batched_counts_match_oracle_fixtures(gpu_ctx, name, model_name):
         assert not bad, bad
         if m is not None:
             m.close()
+
+
+def test_racing_off_matches_oracle_fixtures(gpu_ctx):
+    """Without twins racing the stragglers (and with them, above) the R-MAT
+    2^20 workload's counts equal the fixture."""
+    fx, g, picked = _setup(gpu_ctx, "rmat20")
+    qs = [q for _, q in picked]
+    model = _model(gpu_ctx, "rmat20")
+    gpu_ctx.set_racing(False)
+    try:
+        res = match_batch_learned(gpu_ctx, qs, model, max_orders=8, cap=fx["cap"], first_attempt=10 ** 7)
+    finally:
+        gpu_ctx.set_racing(True)
+        if model is not None:
+            model.close()
+    bad = [(e["index"], r["count"], r["saturated"]) for (e, _), r in zip(picked, res)
+           if r["work_limited"] or (r["count"], r["saturated"]) != (e["count"], e["saturated"])]
+    assert not bad, bad
 
 
 @pytest.mark.parametrize("name", ["rmat20", "rmat22"])
