@@ -95,45 +95,74 @@ k_tables(const uint64_t *__restrict__ goff, const uint32_t *__restrict__ gnbr, c
   if (!WRITE && valid && full_len == 0)
     for (uint32_t j = 0; j < nd; ++j) desc[t0 + j].off[i] = 0;
   const uint32_t W = job.W;
-  uint32_t carry_reg = 0;  // lane j: hits of the row carried into the next step, table j
-  for (uint32_t t = 0; t < total; t += 64) {
+  // lane j < nd holds table j: value vertex, row offsets, values
+  uint32_t my_vu = 0;
+  uint64_t my_off = 0, my_val = 0;
+  if (l < nd) {
+    const TableDesc d = desc[t0 + l];
+    my_vu = d.val_u;
+    my_off = reinterpret_cast<uint64_t>(d.off);
+    my_val = reinterpret_cast<uint64_t>(d.val);
+  }
+  uint32_t carry_reg = 0;  // lane j: hits of the row carried into the next group, table j
+  // One 64-item group of the flattened adjacency: the lane's row, its value,
+  // and the lane range of its row inside the group.
+  struct Grp {
+    bool iv, row_done;
+    uint32_t row, wv, last, row0, seg_hi;
+    uint64_t seg_mask, below_mask;
+  };
+  auto group = [&](uint32_t t) {
+    Grp g;
     const uint32_t item = t + l;
-    const bool iv = item < total;
-    const uint32_t row = iv ? seg_of(st[w], item) : 63;
-    const uint32_t rs = st[w][row], re = st[w][row + 1];
-    const uint32_t wv = iv ? gnbr[ob[w][row] + (item - rs)] : 0;
-    const uint32_t last = min(63u, total - 1 - t);
-    const uint32_t row0 = readlane_u32(row, 0);
+    g.iv = item < total;
+    g.row = g.iv ? seg_of(st[w], item) : 63;
+    const uint32_t rs = st[w][g.row], re = st[w][g.row + 1];
+    g.wv = g.iv ? gnbr[ob[w][g.row] + (item - rs)] : 0;
+    g.last = min(63u, total - 1 - t);
+    g.row0 = readlane_u32(g.row, 0);
     const uint32_t seg_lo = rs > t ? rs - t : 0;
-    const uint32_t seg_hi = (re < t + 64 ? re - t : 64) - 1;
-    const bool row_done = re <= t + 64;
-    const uint64_t seg_mask = iv ? lane_range(seg_lo, seg_hi) : 0;
-    const uint64_t below_mask = iv ? (lane_range(seg_lo, l) & ~(1ull << l)) : 0;
+    g.seg_hi = (re < t + 64 ? re - t : 64) - 1;
+    g.row_done = re <= t + 64;
+    g.seg_mask = g.iv ? lane_range(seg_lo, g.seg_hi) : 0;
+    g.below_mask = g.iv ? (lane_range(seg_lo, l) & ~(1ull << l)) : 0;
+    return g;
+  };
+  // table j over one group, `carry` = the row's hits before the group; returns
+  // the hits carried into the next group
+  auto apply = [&](const Grp &g, uint32_t j, uint32_t vu, uint64_t word, uint32_t carry) {
+    const bool hit = (word >> (g.wv & 63)) & 1ull;
+    const uint64_t m = __ballot(hit);
+    const uint32_t before = (g.row == g.row0) ? carry : 0;
+    const uint32_t tot_row = before + (uint32_t)__popcll(m & g.seg_mask);
+    uint64_t *off = reinterpret_cast<uint64_t *>(readlane_u64(my_off, j));
+    if (WRITE) {
+      if (hit) {
+        uint32_t *val = reinterpret_cast<uint32_t *>(readlane_u64(my_val, j));
+        const uint32_t r = before + (uint32_t)__popcll(m & g.below_mask);
+        const uint32_t p = rank[(uint64_t)vu * (W + 1) + (g.wv >> 6)] +
+                           (uint32_t)__popcll(word & ((1ull << (g.wv & 63)) - 1));
+        val[off[row_base + g.row] + r] = p;
+      }
+    } else {
+      if (g.iv && l == g.seg_hi && g.row_done) off[row_base + g.row] = tot_row;
+    }
+    return readlane_u32(g.row_done ? 0u : tot_row, g.last);
+  };
+  // two groups per step: both groups' loads are in flight before either
+  // group's ballots
+  for (uint32_t t = 0; t < total; t += 128) {
+    const bool two = t + 64 < total;
+    const Grp A = group(t);
+    const Grp B = two ? group(t + 64) : A;
     for (uint32_t j = 0; j < nd; ++j) {
-      const TableDesc d = desc[t0 + j];
-      const uint32_t vu = d.val_u;
-      bool hit = false;
-      uint64_t word = 0;
-      if (iv) {
-        word = bitmap[(uint64_t)vu * W + (wv >> 6)];
-        hit = (word >> (wv & 63)) & 1ull;
-      }
-      const uint64_t m = __ballot(hit);
-      const uint32_t carry = readlane_u32(carry_reg, j);
-      const uint32_t before = (row == row0) ? carry : 0;
-      const uint32_t tot_row = before + (uint32_t)__popcll(m & seg_mask);
-      if (WRITE) {
-        if (hit) {
-          const uint32_t r = before + (uint32_t)__popcll(m & below_mask);
-          const uint32_t p = rank[(uint64_t)vu * (W + 1) + (wv >> 6)] +
-                             (uint32_t)__popcll(word & ((1ull << (wv & 63)) - 1));
-          d.val[d.off[row_base + row] + r] = p;
-        }
-      } else {
-        if (iv && l == seg_hi && row_done) d.off[row_base + row] = tot_row;
-      }
-      const uint32_t carry_next = readlane_u32(row_done ? 0u : tot_row, last);
-      if (l == j) carry_reg = carry_next;
+      const uint32_t vu = readlane_u32(my_vu, j);
+      const uint64_t *bm = bitmap + (uint64_t)vu * W;
+      const uint64_t wa = A.iv ? bm[A.wv >> 6] : 0;
+      const uint64_t wb = two && B.iv ? bm[B.wv >> 6] : 0;
+      uint32_t carry = apply(A, j, vu, wa, readlane_u32(carry_reg, j));
+      if (two) carry = apply(B, j, vu, wb, carry);
+      if (l == j) carry_reg = carry;
     }
   }
 }
@@ -202,27 +231,40 @@ k_tables_heavy(const uint64_t *__restrict__ goff, const uint32_t *__restrict__ g
     const uint64_t t0 = (c - cpre[lo]) * TCH, t1 = t0 + TCH < len ? t0 + TCH : len;
     if (!WRITE && l == 0) atomicAdd(adj_len + ju, (unsigned long long)(t1 - t0));
     const uint32_t nd = job.nd[ju], tb = job.tab0[ju];
-    uint32_t run_reg = 0;
-    if (WRITE && l < nd) run_reg = ccnt[c * ndmax + l];
-    for (uint64_t t = t0; t < t1; t += 64) {
-      const bool iv = t + l < t1;
-      const uint32_t wv = iv ? gnbr[o0 + t + l] : 0;
+    // lane j < nd holds table j: its value vertex, (write) the address of the
+    // row's first value, and its running count in the row
+    uint32_t my_vu = 0, run_reg = 0;
+    uint64_t my_dst = 0;
+    if (l < nd) {
+      const TableDesc d = desc[tb + l];
+      my_vu = d.val_u;
+      if (WRITE) {
+        my_dst = reinterpret_cast<uint64_t>(d.val + d.off[i]);
+        run_reg = ccnt[c * ndmax + l];
+      }
+    }
+    // two 64-entry groups per step: both groups' loads are in flight before
+    // either group's ballots
+    for (uint64_t t = t0; t < t1; t += 128) {
+      const bool iv0 = t + l < t1, iv1 = t + 64 + l < t1;
+      const uint32_t wv0 = iv0 ? gnbr[o0 + t + l] : 0;
+      const uint32_t wv1 = iv1 ? gnbr[o0 + t + 64 + l] : 0;
       for (uint32_t j = 0; j < nd; ++j) {
-        const TableDesc d = desc[tb + j];
-        bool hit = false;
-        uint64_t word = 0;
-        if (iv) {
-          word = bitmap[(uint64_t)d.val_u * W + (wv >> 6)];
-          hit = (word >> (wv & 63)) & 1ull;
-        }
-        const uint64_t m = __ballot(hit);
+        const uint32_t vu = readlane_u32(my_vu, j);
+        const uint64_t *bm = bitmap + (uint64_t)vu * W;
+        const uint64_t word0 = iv0 ? bm[wv0 >> 6] : 0, word1 = iv1 ? bm[wv1 >> 6] : 0;
+        const bool hit0 = (word0 >> (wv0 & 63)) & 1ull, hit1 = (word1 >> (wv1 & 63)) & 1ull;
+        const uint64_t m0 = __ballot(hit0), m1 = __ballot(hit1);
         const uint32_t run = readlane_u32(run_reg, j);
-        if (WRITE && hit) {
-          const uint32_t p = rank[(uint64_t)d.val_u * (W + 1) + (wv >> 6)] +
-                             (uint32_t)__popcll(word & ((1ull << (wv & 63)) - 1));
-          d.val[d.off[i] + run + rank_below(m)] = p;
+        if (WRITE && (hit0 || hit1)) {
+          uint32_t *dst = reinterpret_cast<uint32_t *>(readlane_u64(my_dst, j));
+          const uint32_t *rk = rank + (uint64_t)vu * (W + 1);
+          if (hit0) dst[run + rank_below(m0)] = rk[wv0 >> 6] + (uint32_t)__popcll(word0 & ((1ull << (wv0 & 63)) - 1));
+          if (hit1)
+            dst[run + (uint32_t)__popcll(m0) + rank_below(m1)] =
+                rk[wv1 >> 6] + (uint32_t)__popcll(word1 & ((1ull << (wv1 & 63)) - 1));
         }
-        if (l == j) run_reg = run + (uint32_t)__popcll(m);
+        if (l == j) run_reg = run + (uint32_t)__popcll(m0) + (uint32_t)__popcll(m1);
       }
     }
     if (!WRITE && l < nd) ccnt[c * ndmax + l] = run_reg;
