@@ -71,7 +71,19 @@ k_filter(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr, con
   uint32_t pass = 0;
   // light vertices: own lane, private LDS column
   if (cmask && deg <= LIGHT_DEG) {
-    for (uint64_t j = 0; j < deg; ++j) {
+    // four neighbours at a time: four independent neighbour -> label -> slot
+    // chains in flight instead of one
+    uint64_t j = 0;
+    for (; j + 4 <= deg; j += 4) {
+      const uint32_t a0 = nbr[o0 + j], a1 = nbr[o0 + j + 1], a2 = nbr[o0 + j + 2], a3 = nbr[o0 + j + 3];
+      const uint32_t b0 = lab[a0], b1 = lab[a1], b2 = lab[a2], b3 = lab[a3];
+      const uint32_t s0 = slot_of[b0], s1 = slot_of[b1], s2 = slot_of[b2], s3 = slot_of[b3];
+      if (s0 != NO_SLOT) h[s0][l] += 1;
+      if (s1 != NO_SLOT) h[s1][l] += 1;
+      if (s2 != NO_SLOT) h[s2][l] += 1;
+      if (s3 != NO_SLOT) h[s3][l] += 1;
+    }
+    for (; j < deg; ++j) {
       uint32_t s = slot_of[lab[nbr[o0 + j]]];
       if (s != NO_SLOT) h[s][l] += 1;
     }
@@ -159,11 +171,14 @@ k_filter_heavy(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nb
     }
     const uint32_t g = fq.group[slot_of[lab[v]]];
     uint32_t acc = 0;
-    for (uint64_t j = 0; j < deg; j += 64) {
-      uint32_t s = NO_SLOT;
-      if (j + l < deg) s = slot_of[lab[nbr[o0 + j + l]]];
+    for (uint64_t j = 0; j < deg; j += 128) {
+      // two 64-entry groups: both neighbour -> label -> slot chains in flight
+      const bool i0 = j + l < deg, i1 = j + 64 + l < deg;
+      const uint32_t a0 = i0 ? nbr[o0 + j + l] : 0, a1 = i1 ? nbr[o0 + j + 64 + l] : 0;
+      const uint32_t b0 = i0 ? (uint32_t)lab[a0] : 0, b1 = i1 ? (uint32_t)lab[a1] : 0;
+      const uint32_t s0 = i0 ? slot_of[b0] : NO_SLOT, s1 = i1 ? slot_of[b1] : NO_SLOT;
       for (uint32_t q = 0; q < nslots; ++q) {
-        const uint32_t c = (uint32_t)__popcll(__ballot(s == q));
+        const uint32_t c = (uint32_t)__popcll(__ballot(s0 == q)) + (uint32_t)__popcll(__ballot(s1 == q));
         if (l == q) acc += c;
       }
     }
@@ -197,11 +212,13 @@ k_filter_hub(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nbr,
     if (j0 >= deg) continue;
     const uint64_t j1 = j0 + FILTER_HUB_DEG < deg ? j0 + FILTER_HUB_DEG : deg;
     uint32_t acc = 0;
-    for (uint64_t j = j0; j < j1; j += 64) {
-      uint32_t s = NO_SLOT;
-      if (j + l < j1) s = slot_of[lab[nbr[o0 + j + l]]];
+    for (uint64_t j = j0; j < j1; j += 128) {
+      const bool i0 = j + l < j1, i1 = j + 64 + l < j1;
+      const uint32_t a0 = i0 ? nbr[o0 + j + l] : 0, a1 = i1 ? nbr[o0 + j + 64 + l] : 0;
+      const uint32_t b0 = i0 ? (uint32_t)lab[a0] : 0, b1 = i1 ? (uint32_t)lab[a1] : 0;
+      const uint32_t s0 = i0 ? slot_of[b0] : NO_SLOT, s1 = i1 ? slot_of[b1] : NO_SLOT;
       for (uint32_t q = 0; q < nslots; ++q) {
-        const uint32_t c = (uint32_t)__popcll(__ballot(s == q));
+        const uint32_t c = (uint32_t)__popcll(__ballot(s0 == q)) + (uint32_t)__popcll(__ballot(s1 == q));
         if (l == q) acc += c;
       }
     }
