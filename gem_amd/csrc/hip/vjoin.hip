@@ -239,43 +239,57 @@ k_vext(const VItem *__restrict__ items, const uint32_t *__restrict__ nitems_p, c
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       pend = 0;
     };
-    // 128 items per step, two per lane (items s + l and s + 64 + l), so that
-    // each lane has two loads in flight at every stage of the probes
-    for (uint32_t s = 0; s < wtotal; s += 128) {
-      uint32_t val0 = 0, val1 = 0, rw0 = 0, rw1 = 0;
-      bool keep0 = s + l < wtotal, keep1 = s + 64 + l < wtotal;
-      if (keep0) rw0 = seg_of(st[w], s + l);
-      if (keep1) rw1 = seg_of(st[w], s + 64 + l);
-      if (keep0) val0 = ldg(reinterpret_cast<const uint32_t *>(drv[w][rw0]) + (s + l - st[w][rw0]));
-      if (keep1) val1 = ldg(reinterpret_cast<const uint32_t *>(drv[w][rw1]) + (s + 64 + l - st[w][rw1]));
-      for (uint32_t c = 0; c < ndup; ++c) {
-        if (dpos[c * 64 + rw0] == val0) keep0 = false;
-        if (dpos[c * 64 + rw1] == val1) keep1 = false;
+    // 256 items per step, four per lane (items s + 64 g + l), so that each
+    // lane has four loads in flight at every stage of the probes
+    for (uint32_t s = 0; s < wtotal; s += 256) {
+      uint32_t val[4], rw[4];
+      bool keep[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const uint32_t item = s + 64u * g + l;
+        keep[g] = item < wtotal;
+        rw[g] = keep[g] ? seg_of(st[w], item) : 0;
+        val[g] = keep[g] ? ldg(reinterpret_cast<const uint32_t *>(drv[w][rw[g]]) + (item - st[w][rw[g]])) : 0;
       }
-      for (uint32_t o = 0; o + 1 < nb; ++o)
-        contains_pair(reinterpret_cast<const uint32_t *>(obase[o * 64 + rw0]), olen[o * 64 + rw0], val0, keep0,
-                      reinterpret_cast<const uint32_t *>(obase[o * 64 + rw1]), olen[o * 64 + rw1], val1, keep1);
-      const uint64_t m0 = __ballot(keep0), m1 = __ballot(keep1);
-      const uint32_t nk0 = (uint32_t)__popcll(m0), nk = nk0 + (uint32_t)__popcll(m1);
+      for (uint32_t c = 0; c < ndup; ++c) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+          if (dpos[c * 64 + rw[g]] == val[g]) keep[g] = false;
+      }
+      for (uint32_t o = 0; o + 1 < nb; ++o) {
+        const uint32_t *lists[4];
+        uint32_t lens[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          lists[g] = reinterpret_cast<const uint32_t *>(obase[o * 64 + rw[g]]);
+          lens[g] = olen[o * 64 + rw[g]];
+        }
+        contains_quad(lists, lens, val, keep);
+      }
+      uint64_t m[4];
+      uint32_t nk = 0, before[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        m[g] = __ballot(keep[g]);
+        before[g] = nk;
+        nk += (uint32_t)__popcll(m[g]);
+      }
       if (FINAL) {
         survivors += nk;
       } else if (nk) {
         // stage this step's survivors after the pending ones; the staging
         // area is written out when the next step might not fit
-        if (keep0) {
-          const uint32_t i = pend + rank_below(m0);
-          sv_row[w][i] = rw0;
-          sv_val[w][i] = val0;
-        }
-        if (keep1) {
-          const uint32_t i = pend + nk0 + rank_below(m1);
-          sv_row[w][i] = rw1;
-          sv_val[w][i] = val1;
-        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+          if (keep[g]) {
+            const uint32_t i = pend + before[g] + rank_below(m[g]);
+            sv_row[w][i] = rw[g];
+            sv_val[w][i] = val[g];
+          }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         pend += nk;
-        if (pend > VSTAGE - 128) flush();
+        if (pend > VSTAGE - 256) flush();
       }
     }
     if (!FINAL && pend) flush();

@@ -175,6 +175,41 @@ __device__ __forceinline__ void contains_pair(const uint32_t *a0, uint32_t n0, u
   k1 = k1 && f1;
 }
 
+// Four membership tests at once (contains_pair for four values): v[g] in
+// sorted a[g][0..n[g]); k[g] enters as "still wanted" and leaves as the
+// result.  The four searches advance in lockstep, so up to four loads of one
+// lane are in flight at every stage.
+__device__ __forceinline__ void contains_quad(const uint32_t *const a[4], const uint32_t n_in[4],
+                                              const uint32_t v[4], bool k[4]) {
+  uint32_t lo[4], n[4];
+  bool f[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    lo[g] = 0;
+    n[g] = k[g] ? n_in[g] : 0;
+    f[g] = false;
+  }
+  while (n[0] | n[1] | n[2] | n[3]) {
+    uint32_t x[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) x[g] = n[g] ? ldg(a[g] + lo[g] + (n[g] >> 1)) : 0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      if (!n[g]) continue;
+      const uint32_t h = n[g] >> 1;
+      if (x[g] < v[g]) {
+        lo[g] += h + 1;
+        n[g] -= h + 1;
+      } else {
+        f[g] |= x[g] == v[g];
+        n[g] = h;
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < 4; ++g) k[g] = k[g] && f[g];
+}
+
 // Map a hardware workgroup id to a logical one so that consecutive logical
 // blocks run on the same XCD (workgroups are dispatched round-robin over the
 // 8 XCDs); neighbouring row blocks then share one L2.
