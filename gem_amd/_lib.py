@@ -99,6 +99,8 @@ SIGNATURES = [
     ("gem_hip_set_graph_residency", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_set_relaxed_sharing", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_set_racing", C.c_int, [C.c_void_p, C.c_int]),
+    ("gem_hip_set_level_swap", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("gem_hip_swap_stats", C.c_int, [C.c_void_p, C.c_void_p]),
     ("gem_export_rows", C.c_int, [u32p, C.c_uint64, C.c_uint32, C.c_char_p, C.c_uint64, C.c_void_p]),
     ("gem_hip_export_embeddings", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_char_p, C.c_uint64, C.c_uint64,
                                             C.c_void_p]),
@@ -139,6 +141,11 @@ SIGNATURES = [
     ("gem_hip_match_relaxed", C.c_int, [C.c_void_p, C.POINTER(QueryView), C.c_uint32, C.c_void_p, C.c_uint32,
                                         C.POINTER(BatchOpts), C.c_uint32, u64p, C.POINTER(JoinResult), u32p]),
 ]
+
+class SwapStats(C.Structure):
+    _fields_ = [("levels_out", C.c_uint64), ("levels_in", C.c_uint64), ("bytes_out", C.c_uint64),
+                ("bytes_in", C.c_uint64), ("peak_host_bytes", C.c_uint64)]
+
 
 class ExportResult(C.Structure):
     _fields_ = [("rows", C.c_uint64), ("digest", C.c_uint64), ("runs", C.c_uint32), ("num_v", C.c_uint32)]

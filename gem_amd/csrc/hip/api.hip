@@ -132,6 +132,19 @@ int gem_hip_set_max_active(gem_ctx *c, uint32_t queries) {
   return GEM_OK;
 }
 
+int gem_hip_set_level_swap(gem_ctx *c, uint64_t host_bytes) {
+  if (!c) return GEM_E_INVALID;
+  c->swap_bytes = host_bytes;
+  return GEM_OK;
+}
+
+int gem_hip_swap_stats(const gem_ctx *c, gem_swap_stats *out) {
+  if (!c || !out) return GEM_E_INVALID;
+  const gemh::SchedStats &st = c->last_sched;
+  *out = gem_swap_stats{st.swaps_out, st.swaps_in, st.swap_bytes_out, st.swap_bytes_in, st.peak_host};
+  return GEM_OK;
+}
+
 int gem_hip_set_racing(gem_ctx *c, int on) {
   if (!c) return GEM_E_INVALID;
   c->racing = on != 0;

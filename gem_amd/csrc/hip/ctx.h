@@ -270,6 +270,7 @@ struct gem_ctx {
   int graph_host = 0;                       // next data graph: neighbours stay in pinned host memory
   bool relaxed_share = true;                // relaxed queries share one filter and one table build
   bool racing = true;                       // batched learned path: twins race stragglers
+  uint64_t swap_bytes = 0;                  // batched paths: host memory pending levels may move to
   bool use_graph = false;                   // replay the batched extend step as a hipGraph
   // batched path with streamed edge tables: values in pinned host memory,
   // a device cache of stream_cache_bytes in slots of stream_slot_bytes

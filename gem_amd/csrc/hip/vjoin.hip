@@ -1074,6 +1074,28 @@ struct HipSchedBackend {
     t_ext += std::chrono::duration<double>(clk::now() - t0).count();
   }
 
+  // Swapping: a level's rows to pinned host memory and back, both copies on
+  // the context stream (the device buffer goes back to the pool at once: its
+  // next user is ordered after the copy).  A host buffer is freed after the
+  // stream has drained, so no copy still reads or writes it.
+  std::shared_ptr<void> swap_out(const DevBuf &b, uint64_t bytes) {
+    void *h = nullptr;
+    GEM_HIP_CHECK(hipHostMalloc(&h, std::max<uint64_t>(bytes, 1), hipHostMallocDefault));
+    hipStream_t s = c->stream;
+    std::shared_ptr<void> hp(h, [s](void *p) {
+      (void)hipStreamSynchronize(s);
+      (void)hipHostFree(p);
+    });
+    GEM_HIP_CHECK(hipMemcpyAsync(h, b.p, bytes, hipMemcpyDeviceToHost, s));
+    return hp;
+  }
+  std::shared_ptr<DevBuf> swap_in(const std::shared_ptr<void> &h, uint64_t bytes) {
+    auto b = std::make_shared<DevBuf>();
+    b->alloc(bytes, c->stream);
+    GEM_HIP_CHECK(hipMemcpyAsync(b->p, h.get(), bytes, hipMemcpyHostToDevice, c->stream));
+    return b;
+  }
+
   void admit(uint32_t qi) {
     VQueryState &V = *qs[qi];
     if (V.twin && V.s.partner >= 0) {
@@ -1249,6 +1271,7 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
   cfg.budget = total_budget;
   cfg.max_active = c_->max_active;
   cfg.primaries = primaries_;
+  cfg.swap_bytes = c_->swap_bytes;
   const auto t0 = std::chrono::steady_clock::now();
   stats_ = gemh::SchedStats{};
   gemh::run_schedule(be, sq, cfg, &stats_);

@@ -29,6 +29,15 @@
 // attempt ends gives the query's result.  Before that, the backend may prolong
 // the attempt instead (raise its limit), when what it counted so far says it
 // is near its end.
+//
+// Swapping (swap_bytes != 0).  When the device-resident pending rows exceed
+// the budget after an extend step, the least recently used pending levels
+// that are not on top of a stack move to host memory (up to swap_bytes in
+// all), and a level comes back to the device when it is on top of its stack
+// at the start of an iteration.  Depth-first order makes the least recently
+// used levels the ones needed last: the shallow levels under a query's
+// current path.  The rows are the same wherever they are, so counts do not
+// depend on it.
 #pragma once
 
 #include <algorithm>
@@ -58,6 +67,7 @@ struct SchedConfig {
   // partner among them, start only once every primary has started, and do
   // not count as waiting for the quotas
   uint32_t primaries = 0;
+  uint64_t swap_bytes = 0;  // host memory pending levels may move to (0 = none)
 };
 
 struct SchedStats {
@@ -71,6 +81,9 @@ struct SchedStats {
   uint64_t extend_items = 0;    // items handed to extend steps
   uint64_t restarts = 0;        // attempts ended at their limit and started again
   uint64_t admitted = 0;        // queries started
+  uint64_t swaps_out = 0, swaps_in = 0;            // levels moved to host memory / back
+  uint64_t swap_bytes_out = 0, swap_bytes_in = 0;  // their bytes
+  uint64_t peak_host = 0;                          // most bytes held in host memory at once
 };
 
 template <class Buf>
@@ -82,6 +95,11 @@ struct SLevel {
   std::vector<uint64_t> unit_u;   // prefix of per-unit output bounds (nunits + 1) once measured
   uint64_t cursor = 0;            // next unit
   bool measured = false;
+  // swapping: the rows in host memory (rows null meanwhile), their bytes, and
+  // the iteration that last pushed or read this level
+  std::shared_ptr<void> host;
+  uint64_t host_bytes = 0;
+  uint64_t last_use = 0;
   uint64_t nunits() const { return (nrows + SCHED_UNIT - 1) / SCHED_UNIT; }
 };
 
@@ -209,6 +227,10 @@ inline int plan_chunks(const std::vector<ChunkCand> &cand, uint64_t left, uint64
 //   // query q's attempt went past its limit, its levels still pending: raise
 //   // its attempt_limit and return true to let it go on, or return false
 //   bool prolong(uint32_t q);
+//   // swapping only: copy `bytes` of a device buffer to new host memory, and
+//   // back to a new device buffer (copies ordered on the backend's stream)
+//   std::shared_ptr<void> swap_out(const Buf &rows, uint64_t bytes);
+//   std::shared_ptr<Buf> swap_in(const std::shared_ptr<void> &host, uint64_t bytes);
 //   // query q starts: seed its first level (or mark it done); only called
 //   // for queries whose stack is empty and that are not done
 //   void admit(uint32_t q);
@@ -229,6 +251,7 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
   };
   std::vector<uint8_t> started(qs.size(), 0);
   size_t next = 0;
+  uint64_t host_used = 0;  // swapped levels' bytes
   const size_t nprim = cfg.primaries ? std::min<size_t>(cfg.primaries, qs.size()) : qs.size();
   // a query that ends with a count (or saturated) cancels its partner
   auto cancel_partner = [&](SQuery<Buf> &S) {
@@ -262,6 +285,20 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
 
     if (act.empty()) break;
     ++st->iterations;
+    // swapping: every running query's top level is on the device
+    if (cfg.swap_bytes)
+      for (uint32_t qi : act) {
+        SLevel<Buf> &top = qs[qi]->stack.back();
+        top.last_use = st->iterations;
+        if (top.host) {
+          top.rows = be.swap_in(top.host, top.host_bytes);
+          host_used -= top.host_bytes;
+          ++st->swaps_in;
+          st->swap_bytes_in += top.host_bytes;
+          top.host.reset();
+          top.host_bytes = 0;
+        }
+      }
     const uint64_t live = live_bytes();
     be.on_iteration(st->iterations, act.size(), live, budget);
 
@@ -437,6 +474,7 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
         L.rows = std::move(rows);
         L.nrows = got;
         L.k = it.k + 1;
+        L.last_use = st->iterations;
         S.stack.push_back(std::move(L));
       }
       for (size_t t = 0; t < fin.size(); ++t) {
@@ -473,6 +511,37 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
         S.done = true;
         cancel_partner(S);
       }
+    }
+    // swapping: device-resident pending rows above the budget go to host
+    // memory, least recently used first, never a level on top of its stack
+    if (cfg.swap_bytes) {
+      uint64_t dev = live_bytes();
+      if (dev > budget) {
+        std::vector<SLevel<Buf> *> cands;
+        for (SQuery<Buf> *S : qs)
+          for (size_t i = 0; i + 1 < S->stack.size(); ++i)
+            if (S->stack[i].rows) cands.push_back(&S->stack[i]);
+        std::stable_sort(cands.begin(), cands.end(),
+                         [](const SLevel<Buf> *a, const SLevel<Buf> *b) { return a->last_use < b->last_use; });
+        for (SLevel<Buf> *L : cands) {
+          if (dev <= budget) break;
+          const uint64_t b = BE::buf_bytes(*L->rows);
+          if (host_used + b > cfg.swap_bytes) continue;
+          L->host = be.swap_out(*L->rows, b);
+          L->host_bytes = b;
+          L->rows.reset();
+          host_used += b;
+          dev -= b;
+          ++st->swaps_out;
+          st->swap_bytes_out += b;
+        }
+      }
+      // a cleared stack (saturated, cancelled, restarted) has released its host copies
+      uint64_t h = 0;
+      for (SQuery<Buf> *S : qs)
+        for (const SLevel<Buf> &L : S->stack) h += L.host ? L.host_bytes : 0;
+      host_used = h;
+      st->peak_host = std::max(st->peak_host, host_used);
     }
   }
 }

@@ -161,6 +161,19 @@ class GemContext:
         `chunk`-entry pieces spread over waves (load balance only)."""
         self._check(self._lib.gem_hip_set_heavy_split(self._h, int(threshold), int(chunk)), "gem_hip_set_heavy_split")
 
+    def set_level_swap(self, host_bytes):
+        """Pending join levels may move to pinned host memory, up to host_bytes
+        (least recently used first), when the device-resident ones exceed the
+        level budget (0 = never)."""
+        self._check(self._lib.gem_hip_set_level_swap(self._h, int(host_bytes)), "gem_hip_set_level_swap")
+
+    def swap_stats(self):
+        """Swapping statistics of the last batched run."""
+        from ._lib import SwapStats
+        st = SwapStats()
+        self._check(self._lib.gem_hip_swap_stats(self._h, C.byref(st)), "gem_hip_swap_stats")
+        return {k: int(getattr(st, k)) for k, _ in SwapStats._fields_}
+
     def set_racing(self, on=True):
         """Twins race the batched learned path's stragglers (default on)."""
         self._check(self._lib.gem_hip_set_racing(self._h, int(bool(on))), "gem_hip_set_racing")

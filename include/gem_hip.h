@@ -383,6 +383,23 @@ int gem_hip_set_max_active(gem_ctx *ctx, uint32_t queries);
  * pair ends first with a count cancels the other.  Results are the same; a
  * query's `work` includes its twin's.  Not used with streamed tables. */
 int gem_hip_set_racing(gem_ctx *ctx, int on);
+/* Swapping of pending join levels (the reference's out-of-core LRU swap of
+ * intermediate results, out-of-core/lib/structure/src/memory_manager.cu:748-822
+ * with LRUEvictPolicy, policy.cpp:106-172, hooked into the join at
+ * out-of-core/lib/join/src/join_trie.cu:1088-1108).  host_bytes != 0: in the
+ * batched paths, when the device-resident pending level rows exceed the level
+ * budget after an extend step, the least recently used pending levels that are
+ * not on top of a query's depth-first stack move to pinned host memory (at
+ * most host_bytes in all), and come back when they are on top again.  Counts
+ * do not depend on it.  0 (default): levels stay on the device. */
+int gem_hip_set_level_swap(gem_ctx *ctx, uint64_t host_bytes);
+typedef struct gem_swap_stats {
+  uint64_t levels_out, levels_in; /* levels moved to host memory / back */
+  uint64_t bytes_out, bytes_in;   /* their bytes                        */
+  uint64_t peak_host_bytes;       /* most bytes in host memory at once  */
+} gem_swap_stats;
+/* Statistics of the last batched run. */
+int gem_hip_swap_stats(const gem_ctx *ctx, gem_swap_stats *out);
 
 /* Out-of-core edge tables for the batched paths (replaces the reference's
  * out-of-core MemoryManager / LRUEvictPolicy, out-of-core/lib/structure/src/

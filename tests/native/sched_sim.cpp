@@ -8,7 +8,7 @@
 // memory is touched).  The program runs the scheduler over many queries for
 // several passes on one pool and prints one JSON object per pass.
 //
-//   sched_sim run <queries> <budget_bytes> <passes> <seed> [max_active]
+//   sched_sim run <queries> <budget_bytes> <passes> <seed> [max_active] [swap_bytes]
 //   sched_sim plan      (fixed plan_chunks cases, one JSON line)
 #include <cinttypes>
 #include <cstdio>
@@ -99,6 +99,10 @@ struct SimBackend {
     prefix->resize(items.size());
     for (size_t t = 0; t < items.size(); ++t) {
       const auto &x = items[t];
+      if (!x.rows) {
+        std::fprintf(stderr, "a level in host memory was measured\n");
+        std::exit(5);
+      }
       const uint64_t nrows = x.row_end - x.row_begin;
       const uint64_t nu = (nrows + gemh::SCHED_UNIT - 1) / gemh::SCHED_UNIT;
       std::vector<uint64_t> &P = (*prefix)[t];
@@ -115,6 +119,10 @@ struct SimBackend {
     got_fin->assign(fin.size(), 0);
     for (size_t t = 0; t < wr.size(); ++t) {
       const auto &x = wr[t];
+      if (!x.rows) {
+        std::fprintf(stderr, "a level in host memory was extended\n");
+        std::exit(5);
+      }
       const uint64_t nrows = level_rows_of(x);
       for (uint64_t u = x.row_begin / gemh::SCHED_UNIT; u * gemh::SCHED_UNIT < x.row_end; ++u)
         (*got_wr)[t] += survivors(x.rows->id, x.level, u, nrows, false);
@@ -135,9 +143,34 @@ struct SimBackend {
   bool restart(uint32_t) { return false; }
   bool prolong(uint32_t) { return false; }
   void admit(uint32_t) {}
+  // swapping: a host copy keeps the rows' identity (buffer id, rows)
+  struct HostRows {
+    uint64_t id, nrows, bytes;
+  };
+  uint64_t host_live = 0, host_peak = 0;
+  std::shared_ptr<void> swap_out(const SimBuf &b, uint64_t bytes) {
+    host_live += bytes;
+    host_peak = std::max(host_peak, host_live);
+    SimBackend *self = this;
+    return std::shared_ptr<void>(new HostRows{b.id, b.nrows, bytes}, [self](void *p) {
+      self->host_live -= static_cast<HostRows *>(p)->bytes;
+      delete static_cast<HostRows *>(p);
+    });
+  }
+  std::shared_ptr<SimBuf> swap_in(const std::shared_ptr<void> &h, uint64_t bytes) {
+    const HostRows &r = *static_cast<const HostRows *>(h.get());
+    if (r.bytes != bytes) {
+      std::fprintf(stderr, "swapped level came back with another size\n");
+      std::exit(4);
+    }
+    auto b = alloc(bytes);
+    b->id = r.id;
+    b->nrows = r.nrows;
+    return b;
+  }
 };
 
-int run(int nq, uint64_t budget, int passes, uint64_t seed, uint32_t max_active) {
+int run(int nq, uint64_t budget, int passes, uint64_t seed, uint32_t max_active, uint64_t swap_bytes) {
   FakeDevice dev;
   gemh::PoolCore pool(gemh::PoolBacking{fake_alloc, fake_release, fake_sync, &dev});
   for (int pass = 0; pass < passes; ++pass) {
@@ -170,6 +203,7 @@ int run(int nq, uint64_t budget, int passes, uint64_t seed, uint32_t max_active)
     cfg.budget = budget;
     cfg.max_chunk = 1ull << 30;
     cfg.max_active = max_active;
+    cfg.swap_bytes = swap_bytes;
     gemh::SchedStats st;
     const gemh::PoolStats before = pool.stats();
     pool.reset_peak();
@@ -193,10 +227,13 @@ int run(int nq, uint64_t budget, int passes, uint64_t seed, uint32_t max_active)
         ", \"pool_peak\": %" PRIu64 ", \"pool_in_use_after\": %" PRIu64 ", \"pool_held_after\": %" PRIu64
         ", \"trimmed\": %" PRIu64 ", \"held_after_trim\": %" PRIu64 ", \"saturated\": %" PRIu64
         ", \"work_limited\": %" PRIu64 ", \"finished\": %" PRIu64 ", \"pending\": %" PRIu64
-        ", \"length_items\": %" PRIu64 ", \"extend_calls\": %" PRIu64 "}\n",
+        ", \"length_items\": %" PRIu64 ", \"extend_calls\": %" PRIu64 ", \"swaps_out\": %" PRIu64
+        ", \"swaps_in\": %" PRIu64 ", \"swap_bytes_out\": %" PRIu64 ", \"peak_host\": %" PRIu64
+        ", \"backend_host_peak\": %" PRIu64 ", \"host_left\": %" PRIu64 "}\n",
         pass, nq, max_nv, budget, force_chunk, st.iterations, st.forced, st.waits, st.peak_charged,
         st.max_overshoot, st.max_unit_bytes, before.in_use, after.peak_in_use, after.in_use, after.held, trimmed,
-        trimmed_st.held, saturated, limited, finished, pending, be.length_items, be.extend_calls);
+        trimmed_st.held, saturated, limited, finished, pending, be.length_items, be.extend_calls, st.swaps_out,
+        st.swaps_in, st.swap_bytes_out, st.peak_host, be.host_peak, be.host_live);
     std::fflush(stdout);
   }
   return 0;
@@ -296,7 +333,8 @@ int main(int argc, char **argv) {
   if (argc >= 2 && !std::strcmp(argv[1], "pool")) return pool_cases();
   if (argc >= 6 && !std::strcmp(argv[1], "run"))
     return run(std::atoi(argv[2]), std::strtoull(argv[3], nullptr, 10), std::atoi(argv[4]),
-               std::strtoull(argv[5], nullptr, 10), argc >= 7 ? (uint32_t)std::atoi(argv[6]) : 0u);
+               std::strtoull(argv[5], nullptr, 10), argc >= 7 ? (uint32_t)std::atoi(argv[6]) : 0u,
+               argc >= 8 ? std::strtoull(argv[7], nullptr, 10) : 0ull);
   std::fprintf(stderr, "usage: sched_sim run <queries> <budget> <passes> <seed> [max_active] | sched_sim plan\n");
   return 2;
 }

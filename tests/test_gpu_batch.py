@@ -107,6 +107,29 @@ def test_graph_replays_match_eager(gpu_ctx, rmat_graph):
         gpu_ctx.set_graph(False)
 
 
+def test_level_swap_keeps_counts_and_level_rows(gpu_ctx, rmat_graph):
+    """A 1 MiB level budget with 256 MiB of host memory for swapped levels:
+    levels move out and back, and every query's count and per-level rows
+    equal the oracle's vertex-level walk along the same order."""
+    gpu_ctx.load_data(rmat_graph)
+    qs = _queries(rmat_graph, 10, [5, 6, 7], 11501)
+    cap = 10 ** 8
+    vos = [oracle.vertex_order_from_edges(q, oracle.count(rmat_graph, q, cap=cap)["order"]) for q in qs]
+    gpu_ctx.set_level_swap(256 << 20)
+    try:
+        got = gpu_ctx.match_batch(qs, vertex_orders=vos, cap=cap, level_budget=1 << 20)
+        st = gpu_ctx.swap_stats()
+    finally:
+        gpu_ctx.set_level_swap(0)
+    print("swap stats", st, flush=True)
+    assert st["levels_out"] > 0 and st["levels_in"] > 0 and 0 < st["peak_host_bytes"] <= 256 << 20
+    for q, vo, r in zip(qs, vos, got):
+        ref = oracle.vcount(rmat_graph, q, vo, cap=cap)
+        assert (r["count"], r["saturated"]) == (ref["count"], ref["saturated"])
+        if not ref["saturated"]:
+            assert [int(x) for x in r["level_rows"]] == [int(x) for x in ref["level_rows"]]
+
+
 def test_batch_invariance(gpu_ctx, rmat_graph):
     """Group size, level budget and vertex order do not change any count."""
     gpu_ctx.load_data(rmat_graph)

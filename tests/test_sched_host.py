@@ -110,3 +110,21 @@ def test_scheduler_continuous_admission(sim):
         assert p["pending"] == 0
         assert p["saturated"] + p["work_limited"] + p["finished"] == p["queries"]
         assert p["pool_in_use_after"] == 0 and p["held_after_trim"] == 0
+
+
+def test_scheduler_swaps_levels_to_host(sim):
+    """Swapping on (1 GiB of host memory, a 64 MiB device budget, 160 queries,
+    16 running): levels move out and back, the host bytes stay within their
+    bound and are all released, every query ends, no level is measured or
+    extended while in host memory (the simulator exits if one is), a level
+    comes back with its own rows (size checked), and the device-resident peak
+    is no higher than without swapping."""
+    budget, swap = 64 << 20, 1 << 30
+    (p,) = _run(sim, "run", 160, budget, 1, 11, 16, swap)
+    (q,) = _run(sim, "run", 160, budget, 1, 11, 16, 0)
+    assert p["swaps_out"] > 0 and p["swaps_in"] > 0 and q["swaps_out"] == 0
+    assert 0 < p["peak_host"] <= swap and p["backend_host_peak"] == p["peak_host"]
+    assert p["host_left"] == 0
+    assert p["pending"] == 0 and p["saturated"] + p["work_limited"] + p["finished"] == p["queries"]
+    assert p["pool_in_use_after"] == 0
+    assert p["peak_charged"] <= q["peak_charged"]
