@@ -44,6 +44,9 @@ DESCR = {
     "ooc24": "R-MAT 2^24 vertices, avg degree 64, 64 Zipf labels; 32 sparse random-walk 12-vertex queries; "
              "data graph's neighbours in pinned host memory, edge tables streamed from pinned host memory "
              "through a device cache; count capped at 1e9",
+    "ooc26": "R-MAT 2^26 vertices, avg degree 64, 64 Zipf labels; 32 sparse random-walk 12-vertex queries; "
+             "data graph's neighbours in pinned host memory, edge tables streamed from pinned host memory "
+             "through a device cache; count capped at 1e9",
 }
 
 
@@ -156,7 +159,7 @@ def main():
     ap.add_argument("--cpu-work-limit", type=float, default=2e9,
                     help="intermediate rows after which one oracle query is abandoned")
     ap.add_argument("--stream-cache", default=None,
-                    help="stream edge tables through a device cache of this size (e.g. 8G); default: 8G for ooc24, "
+                    help="stream edge tables through a device cache of this size (e.g. 8G); default: 8G for ooc24 / ooc26, "
                          "tables in HBM otherwise")
     ap.add_argument("--stream-slot", default="1M", help="cache slot size for streamed tables")
     ap.add_argument("--no-racing", action="store_true", help="no twins racing the batch's stragglers")
@@ -165,7 +168,7 @@ def main():
     args = ap.parse_args()
     graph_host = args.graph_host == "on" or (args.graph_host == "auto" and args.workload.startswith("ooc"))
     if args.stream_cache is None:
-        args.stream_cache = "8G" if args.workload == "ooc24" else "0"
+        args.stream_cache = "8G" if args.workload.startswith("ooc") else "0"
     stream_cache = parse_bytes(args.stream_cache)
 
     if args.gpus != 1:

@@ -18,6 +18,9 @@ the parity tests.
          (average degree < 3) random-walk 12-vertex queries: the out-of-core
          shape (streamed edge tables) at a size one box generates in about a
          minute (2^28 vertices take over ten).
+  ooc26  the same at 2^26 vertices (4.3e9 directed slots, 17 GB of
+         neighbours): the largest out-of-core shape one command generates and
+         runs within its time limit.
 """
 import numpy as np
 
@@ -31,6 +34,7 @@ WORKLOADS = {
     "rmat20_train": dict(kind="rmat", scale=20, degree=16.0, labels=16, seed=1002, qseed=2002),
     "rmat22_learn": dict(kind="rmat", scale=22, degree=16.0, labels=32, seed=1003, qseed=2003),
     "ooc24": dict(kind="rmat", scale=24, degree=64.0, labels=64, seed=1005),
+    "ooc26": dict(kind="rmat", scale=26, degree=64.0, labels=64, seed=1006),
 }
 
 CAP = 10 ** 9  # enumeration cap for the R-MAT workloads
@@ -70,7 +74,7 @@ def make_queries(name, graph, count=None):
         for d, s in zip(dense, sparse):
             qs += [d, s]
         return qs
-    if name == "ooc24":
+    if name in ("ooc24", "ooc26"):
         n = count or 32
         i = 0
         while len(qs) < n:
