@@ -87,6 +87,10 @@ int gem_hip_create(int device, gem_ctx **out) {
 void gem_hip_destroy(gem_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
+  if (c->prep) {
+    gem_hip_destroy(c->prep);
+    c->prep = nullptr;
+  }
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (gem_mlp *m : c->mlps) gemd::mlp_detach(m);
   c->mlps.clear();

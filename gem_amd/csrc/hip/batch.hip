@@ -2,9 +2,16 @@
 // edge tables for each query, a vertex plan per query (given, or derived from
 // GEM's edge order), then one VBatchRunner over the whole group.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
+#include <exception>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
 
 #include "ctx.h"
 #include "tcache.h"
@@ -42,11 +49,11 @@ static gemh::QueryStats query_stats(const gem_ctx *c, const QueryDev &Qd) {
   for (uint32_t d = 0; d < 2 * Q.ne; ++d) s.tsize[d] = Qd.tab[d].present ? Qd.tab[d].size : 0;
   s.label_freq.assign(Q.nv, 0);
   for (uint32_t u = 0; u < Q.nv; ++u) {
-    auto it = c->g.label_id.find(Q.lab[u]);
-    if (it != c->g.label_id.end()) s.label_freq[u] = c->g.label_freq[it->second];
+    auto it = graph(c).label_id.find(Q.lab[u]);
+    if (it != graph(c).label_id.end()) s.label_freq[u] = graph(c).label_freq[it->second];
   }
-  s.data_v = c->g.n;
-  s.data_e = c->g.slots / 2;
+  s.data_v = graph(c).n;
+  s.data_e = graph(c).slots / 2;
   return s;
 }
 
@@ -184,142 +191,266 @@ static gemh::QueryStats item_stats(const gem_ctx *c, const BatchItem &it) {
   return s;
 }
 
-// Candidate orders, model ranking and one VBatchRunner over `items`; results
-// (and level_rows, chosen) indexed like items.  Returns the seconds spent on
-// candidate orders and features.
-static double run_items(gem_ctx *c, const std::vector<BatchItem> &items, gem_mlp *mlp, uint32_t max_orders,
-                        const gem_batch_opts &o, bool streamed, gem_join_result *results, uint64_t *level_rows,
-                        uint32_t *chosen) {
-  using clk = std::chrono::steady_clock;
-  const auto t0 = clk::now();
-  const size_t n = items.size();
-  std::vector<VQueryState> st(n);
-  std::vector<VQueryState *> ptrs;
-  std::vector<std::vector<std::vector<uint32_t>>> cand(n);
-  std::vector<int32_t> feats;
-  std::vector<uint32_t> owner;
-  for (size_t i = 0; i < n; ++i) {
-    const gemh::QueryInfo &Q = item_query(items[i]);
-    const gemh::QueryStats qsx = item_stats(c, items[i]);
+// Upper bounds of a query's launch needs (see VQueryState::pending): the most
+// backward lists of a level (the largest vertex degree) and equal-label
+// columns (the most vertices of one label, less one).
+static void launch_bounds_of(const gem_query_view &v, uint32_t *nb, uint32_t *nd) {
+  std::vector<uint32_t> deg(v.num_v, 0);
+  for (uint32_t e = 0; e < v.num_e; ++e) {
+    deg[v.edges[2 * e]]++;
+    deg[v.edges[2 * e + 1]]++;
+  }
+  std::map<uint32_t, uint32_t> per_label;
+  uint32_t most = 0;
+  for (uint32_t u = 0; u < v.num_v; ++u) most = std::max(most, ++per_label[v.labels[u]]);
+  *nb = std::max<uint32_t>(1, deg.empty() ? 1 : *std::max_element(deg.begin(), deg.end()));
+  *nd = most ? most - 1 : 0;
+}
+
+// One batched learned run over `items`: each query's candidate orders and
+// features (once its filter and tables exist), their ranking and the plan of
+// the first ranked order.  Queries prepared before the run are completed
+// before it starts; the others are prepared on a second stream while the run
+// goes on (mark_prepared) and completed when they start (LazyStates).
+class ItemRun : public LazyStates {
+ public:
+  ItemRun(gem_ctx *c, std::vector<BatchItem> &items, gem_mlp *mlp, uint32_t max_orders, const gem_batch_opts &o)
+      : c_(c), items_(items), mlp_(mlp), max_orders_(max_orders), o_(o), n_(items.size()), st_(items.size()),
+        cand_(items.size()), feats_(items.size()), prepared_(items.size()) {
+    for (auto &p : prepared_) p.store(false);
+  }
+
+  // Host part of query i's preparation, after its tables exist; cx is the
+  // context that prepared it (any thread).
+  void candidates(const gem_ctx *cx, size_t i) {
+    const BatchItem &it = items_[i];
+    const gemh::QueryInfo &Q = item_query(it);
+    const gemh::QueryStats qsx = item_stats(cx, it);
     // without a model, candidates beyond GEM's order matter only to restarts
-    const uint32_t norders = (mlp || o.first_attempt) ? max_orders : 1;
-    gemh::candidate_orders(Q, qsx, item_gem_vertex_order(items[i]), norders, query_seed(Q), &cand[i]);
-    if (mlp) {
-      for (auto &vo : cand[i]) {
-        const size_t off = feats.size();
-        feats.resize(off + gemh::FEAT_DIM);
-        gemh::order_features(Q, qsx, vo, feats.data() + off);
-        owner.push_back((uint32_t)i);
-      }
+    const uint32_t norders = (mlp_ || o_.first_attempt) ? max_orders_ : 1;
+    gemh::candidate_orders(Q, qsx, item_gem_vertex_order(it), norders, query_seed(Q), &cand_[i]);
+    feats_[i].clear();
+    if (mlp_) {
+      feats_[i].resize(cand_[i].size() * gemh::FEAT_DIM);
+      for (size_t j = 0; j < cand_[i].size(); ++j)
+        gemh::order_features(Q, qsx, cand_[i][j], feats_[i].data() + j * gemh::FEAT_DIM);
     }
   }
-  // Ranking of each query's candidates: without a model, as generated (GEM's
-  // order first); with one, by predicted cost, except that GEM's order
-  // (candidate 0) stays first unless the cheapest prediction beats it by
-  // more than the model's margin.  The first ranked order runs; restarts
-  // take the next ones.
-  std::vector<std::vector<uint32_t>> rank(n);
-  for (size_t i = 0; i < n; ++i) {
-    rank[i].resize(cand[i].size());
-    for (uint32_t j = 0; j < cand[i].size(); ++j) rank[i][j] = j;
-  }
-  if (mlp && !owner.empty()) {
-    std::vector<float> pred(owner.size());
-    const int rc = gem_mlp_predict(mlp, feats.data(), (uint32_t)owner.size(), pred.data());
-    if (rc) throw Error{rc, "order model: " + c->err};
-    const float margin = mlp_margin(mlp);
-    size_t k = 0;
-    for (size_t i = 0; i < n; ++i) {
-      const float *p = pred.data() + k;
-      k += cand[i].size();
-      std::vector<uint32_t> &R = rank[i];
+
+  // Ranking of query i's candidates and the plan of the first (the run's
+  // stream, the calling thread).  Without a model, as generated (GEM's order
+  // first); with one, by predicted cost, except that GEM's order (candidate
+  // 0) stays first unless the cheapest prediction beats it by more than the
+  // model's margin.  The first ranked order runs; restarts take the next ones.
+  void finish(size_t i) {
+    std::vector<uint32_t> R(cand_[i].size());
+    for (uint32_t j = 0; j < R.size(); ++j) R[j] = j;
+    if (mlp_ && !R.empty()) {
+      std::vector<float> p(R.size());
+      const int rc = gem_mlp_predict(mlp_, feats_[i].data(), (uint32_t)R.size(), p.data());
+      if (rc) throw Error{rc, "order model: " + c_->err};
       std::stable_sort(R.begin(), R.end(), [&](uint32_t a, uint32_t b) { return p[a] < p[b]; });
-      if (!(p[R[0]] < p[0] - margin)) {
+      if (!(p[R[0]] < p[0] - mlp_margin(mlp_))) {
         R.erase(std::find(R.begin(), R.end(), 0u));
         R.insert(R.begin(), 0u);
       }
     }
-  }
-  for (size_t i = 0; i < n; ++i) {
-    VQueryState &V = st[i];
-    V.qd = items[i].qd;
-    V.Qp = items[i].Qp;
-    V.edge_map = items[i].edge_map;
-    V.first_attempt = o.first_attempt;
-    V.growth = o.restart_growth ? o.restart_growth : 4;
-    V.width = o.restart_width ? o.restart_width : 1;
-    V.max_attempts = o.max_attempts;
-    for (uint32_t j : rank[i]) {
-      V.ranked.push_back(cand[i][j]);
+    VQueryState &V = st_[i];
+    V.first_attempt = o_.first_attempt;
+    V.growth = o_.restart_growth ? o_.restart_growth : 4;
+    V.width = o_.restart_width ? o_.restart_width : 1;
+    V.max_attempts = o_.max_attempts;
+    V.ranked.clear();
+    V.ranked_idx.clear();
+    for (uint32_t j : R) {
+      V.ranked.push_back(cand_[i][j]);
       V.ranked_idx.push_back(j);
     }
     // size the launch buffers for every order a restart may run
     if (attempt_limit(V, 0))
       for (size_t j = 1; j < V.ranked.size(); ++j) {
         gemh::VertexPlan P;
-        if (gemh::make_vertex_plan(item_query(items[i]), V.ranked[j], &P))
+        if (gemh::make_vertex_plan(item_query(items_[i]), V.ranked[j], &P))
           throw Error{GEM_E_STATE, "invalid candidate vertex order"};
         for (size_t l = 1; l < P.levels.size(); ++l) {
           V.mnb_all = std::max(V.mnb_all, P.levels[l].nb);
           V.mnd_all = std::max(V.mnd_all, P.levels[l].ndup);
         }
       }
-    replan(c, V, V.ranked[0]);
+    replan(c_, V, V.ranked[0]);
     V.cur = 0;
-    ptrs.push_back(&V);
   }
-  // Racing: each query with restarts gets a twin on the same candidates and
-  // tables; the twins start, in the same order, once every query of the batch
-  // has started, when a running query ends and leaves room (the tail, where
-  // the few queries left would leave most of the device idle), and whichever
-  // of a pair ends first with a count cancels the other.  Counts do not
-  // depend on the order, so either gives the query's result.
-  std::vector<VQueryState> twins;
-  std::vector<int64_t> twin_of(n, -1);
-  if (c->racing && !streamed && o.first_attempt) {
-    twins.resize(n);
-    for (size_t i = 0; i < n; ++i) {
-      VQueryState &P = st[i];
-      if (P.ranked.size() < 2) continue;
-      VQueryState &T = twins[i];
-      T.qd = P.qd;
-      T.Qp = P.Qp;
-      T.edge_map = P.edge_map;
-      T.first_attempt = P.first_attempt;
-      T.growth = P.growth;
-      T.width = P.width;
-      T.max_attempts = P.max_attempts;
-      T.ranked = P.ranked;
-      T.ranked_idx = P.ranked_idx;
-      T.mnb_all = P.mnb_all;
-      T.mnd_all = P.mnd_all;
-      T.twin = true;
-      replan(c, T, T.ranked[1]);
-      twin_of[i] = (int64_t)ptrs.size();
-      T.s.partner = (int64_t)i;
-      P.s.partner = twin_of[i];
-      ptrs.push_back(&T);
+
+  void mark_prepared(size_t i) {
+    std::lock_guard<std::mutex> lk(mu_);
+    prepared_[i].store(true);
+    cv_.notify_all();
+  }
+  void fail(std::exception_ptr e) {
+    std::lock_guard<std::mutex> lk(mu_);
+    error_ = e;
+    cv_.notify_all();
+  }
+  bool ready(uint32_t q) override { return q < n_ && prepared_[q].load(); }
+  void complete(uint32_t q) override {
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [&]() { return prepared_[q].load() || error_; });
+      if (!prepared_[q].load() && error_) std::rethrow_exception(error_);
+    }
+    if (st_[q].pending) finish(q);
+  }
+
+  // Runs the batch: queries [0, eager) are prepared; the rest are pending
+  // until mark_prepared.  Results (and level_rows, chosen) indexed like items.
+  void run(size_t eager, bool streamed, const std::vector<const gem_query_view *> &views,
+           gem_join_result *results, uint64_t *level_rows, uint32_t *chosen) {
+    std::vector<VQueryState *> ptrs;
+    for (size_t i = 0; i < n_; ++i) {
+      VQueryState &V = st_[i];
+      V.qd = items_[i].qd;
+      V.Qp = items_[i].Qp;
+      V.edge_map = items_[i].edge_map;
+      if (i < eager) {
+        finish(i);
+      } else {
+        uint32_t nb = 1, nd = 0;
+        launch_bounds_of(*views[i], &nb, &nd);
+        V.pending = true;
+        V.pending_nv = views[i]->num_v;
+        V.max_nb = {nb};
+        V.max_ndup = {nd};
+      }
+      ptrs.push_back(&V);
+    }
+    // Racing: each query with restarts gets a twin on the same candidates and
+    // tables; the twins start, in the same order, once every query of the
+    // batch has started, when a running query ends and leaves room (the tail,
+    // where the few queries left would leave most of the device idle), and
+    // whichever of a pair ends first with a count cancels the other.  Counts
+    // do not depend on the order, so either gives the query's result.  A twin
+    // takes its orders from its query when it starts.
+    std::vector<VQueryState> twins;
+    std::vector<int64_t> twin_of(n_, -1);
+    if (c_->racing && !streamed && o_.first_attempt && max_orders_ > 1) {
+      twins.resize(n_);
+      for (size_t i = 0; i < n_; ++i) {
+        VQueryState &P = st_[i];
+        if (!P.pending && P.ranked.size() < 2) continue;
+        VQueryState &T = twins[i];
+        T.qd = P.qd;
+        T.Qp = P.Qp;
+        T.edge_map = P.edge_map;
+        T.twin = true;
+        T.pending = true;
+        T.pending_nv = views[i]->num_v;
+        if (P.pending) {
+          T.max_nb = P.max_nb;
+          T.max_ndup = P.max_ndup;
+        } else {
+          T.max_nb = {P.mnb_all};
+          T.max_ndup = {P.mnd_all};
+        }
+        twin_of[i] = (int64_t)ptrs.size();
+        T.s.partner = (int64_t)i;
+        P.s.partner = twin_of[i];
+        ptrs.push_back(&T);
+      }
+    }
+    VBatchRunner runner(c_, o_.cap, o_.work_limit, o_.level_budget, streamed);
+    if (!twins.empty()) runner.set_primaries((uint32_t)n_);
+    runner.set_lazy(this);
+    runner.run(ptrs);
+    for (size_t i = 0; i < n_; ++i) {
+      const VQueryState *W = &st_[i];
+      uint64_t twin_work = 0;
+      if (twin_of[i] >= 0) {
+        const VQueryState &T = twins[i];
+        twin_work = T.s.work;
+        // the twin's result when it ended with a count and the query did not
+        if (!T.pending && T.s.done && !T.s.cancelled && !T.s.work_limited &&
+            (st_[i].s.cancelled || st_[i].s.work_limited))
+          W = &T;
+      }
+      fill_result(*W, o_.cap, &results[i], level_rows ? level_rows + i * GEM_MAX_QV : nullptr);
+      results[i].work = st_[i].s.work + twin_work;
+      if (chosen) chosen[i] = W->ranked.empty() ? 0 : W->ranked_idx[W->cur];
     }
   }
-  const double t_orders = std::chrono::duration<double>(clk::now() - t0).count();
-  VBatchRunner runner(c, o.cap, o.work_limit, o.level_budget, streamed);
-  if (!twins.empty()) runner.set_primaries((uint32_t)n);
-  runner.run(ptrs);
-  for (size_t i = 0; i < n; ++i) {
-    const VQueryState *W = &st[i];
-    uint64_t twin_work = 0;
-    if (twin_of[i] >= 0) {
-      const VQueryState &T = twins[i];
-      twin_work = T.s.work;
-      // the twin's result when it ended with a count and the query did not
-      if (T.s.done && !T.s.cancelled && !T.s.work_limited && (st[i].s.cancelled || st[i].s.work_limited)) W = &T;
-    }
-    fill_result(*W, o.cap, &results[i], level_rows ? level_rows + i * GEM_MAX_QV : nullptr);
-    results[i].work = st[i].s.work + twin_work;
-    if (chosen) chosen[i] = W->ranked_idx[W->cur];
+
+ private:
+  gem_ctx *c_;
+  std::vector<BatchItem> &items_;
+  gem_mlp *mlp_;
+  uint32_t max_orders_;
+  gem_batch_opts o_;
+  size_t n_;
+  std::vector<VQueryState> st_;
+  std::vector<std::vector<std::vector<uint32_t>>> cand_;
+  std::vector<std::vector<int32_t>> feats_;
+  std::vector<std::atomic<bool>> prepared_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::exception_ptr error_;
+};
+
+// The context that prepares queries for c on a stream of its own (stream,
+// events, staging), reading c's data graph; made at its first use.  The
+// queries it prepares live in its stream's part of the device pool.
+static gem_ctx *prep_ctx(gem_ctx *c) {
+  if (!c->prep) {
+    gem_ctx *p = nullptr;
+    const int st = gem_hip_create(c->device, &p);
+    if (st) throw Error{st, "cannot create the preparing context"};
+    p->gsrc = &c->g;
+    c->prep = p;
   }
-  return t_orders;
+  return c->prep;
 }
+
+// Prepares queries [begin, end) of a batch (filter, tables, candidate orders)
+// on the preparing context in a thread of its own while the batch runs; stops
+// after the query at hand and joins when destroyed (before the queries it
+// writes are destroyed).
+class Preparer {
+ public:
+  Preparer(gem_ctx *c, gem_ctx *shadow, const std::vector<const gem_query_view *> &views, std::vector<QueryDev> &qd,
+           ItemRun &run, size_t begin, size_t end)
+      : c_(c), shadow_(shadow) {
+    th_ = std::thread([this, &views, &qd, &run, begin, end]() {
+      try {
+        GEM_HIP_CHECK(hipSetDevice(c_->device));
+        using clk = std::chrono::steady_clock;
+        for (size_t i = begin; i < end && !stop_.load(); ++i) {
+          const auto ta = clk::now();
+          run_filter(shadow_, qd[i], views[i], 0);
+          const auto tb = clk::now();
+          run_tables(shadow_, qd[i]);
+          t_filter += std::chrono::duration<double>(tb - ta).count();
+          t_tables += std::chrono::duration<double>(clk::now() - tb).count();
+          run.candidates(shadow_, i);
+          run.mark_prepared(i);
+        }
+      } catch (...) {
+        run.fail(std::current_exception());
+      }
+    });
+  }
+  ~Preparer() {
+    stop_.store(true);
+    join();
+  }
+  void join() {
+    if (th_.joinable()) th_.join();
+  }
+  double t_filter = 0, t_tables = 0;  // read after the thread ended
+
+ private:
+  gem_ctx *c_;
+  gem_ctx *shadow_ = nullptr;
+  std::atomic<bool> stop_{false};
+  std::thread th_;
+};
 
 static bool verbose() {
   const char *v = std::getenv("GEM_VERBOSE");
@@ -359,32 +490,61 @@ void run_match_batch_learned(gem_ctx *c, const gem_query_view *qs, uint32_t nq, 
   auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
   for (uint32_t g0 = 0; g0 < nq; g0 += group) {
     const uint32_t gn = std::min(group, nq - g0);
+    // Queries the run starts with are prepared here; with continuous
+    // admission (at most max_active running) the later ones are prepared on a
+    // second stream while the run goes on, and start as they become ready.
+    // (Not while profiling: kernel statistics are the batch context's.)
+    const uint32_t eager = (!streamed && c->prep_overlap && c->max_active && !c->prof.on)
+                               ? std::min<uint32_t>(gn, c->max_active + 16)
+                               : gn;
+    std::vector<const gem_query_view *> views(gn);
+    for (uint32_t i = 0; i < gn; ++i) views[i] = &qs[g0 + i];
+    gem_ctx *shadow = eager < gn ? prep_ctx(c) : nullptr;
     std::vector<QueryDev> qd(gn);
+    // queries prepared on the second stream go back to its pool: the batch
+    // stream's reads of them end first, also when the run throws
+    struct SyncFirst {
+      hipStream_t s;
+      bool on;
+      ~SyncFirst() {
+        if (on) (void)hipStreamSynchronize(s);
+      }
+    } sync_first{c->stream, eager < gn};
     std::vector<BatchItem> items(gn);
+    for (uint32_t i = 0; i < gn; ++i) items[i].qd = &qd[i];
+    ItemRun run(c, items, mlp, max_orders, o);
     double t_filter = 0, t_tables = 0;
     const auto t_prep = clk::now();
-    for (uint32_t i = 0; i < gn; ++i) {
+    for (uint32_t i = 0; i < eager; ++i) {
       const auto ta = clk::now();
-      run_filter(c, qd[i], &qs[g0 + i], 0);
+      run_filter(c, qd[i], views[i], 0);
       const auto tb = clk::now();
       run_tables(c, qd[i]);
       if (streamed) spill_tables(c, qd[i], c->stream_slot_bytes);
       t_filter += secs(ta, tb);
       t_tables += secs(tb, clk::now());
-      items[i].qd = &qd[i];
+      run.candidates(c, i);
     }
-    if (verbose()) {
-      std::fprintf(stderr, "[batch] %u queries: filter %.2f s, edge tables %.2f s\n", gn, t_filter, t_tables);
-      std::fflush(stderr);
+    {
+      std::unique_ptr<Preparer> prep;
+      if (eager < gn) prep = std::make_unique<Preparer>(c, shadow, views, qd, run, eager, gn);
+      if (verbose()) {
+        std::fprintf(stderr, "[batch] %u queries: %u prepared first (filter %.2f s, edge tables %.2f s)%s\n", gn,
+                     eager, t_filter, t_tables, eager < gn ? ", the rest during the run on a second stream" : "");
+        std::fflush(stderr);
+      }
+      run.run(eager, streamed, views, results + g0, level_rows ? level_rows + (uint64_t)g0 * GEM_MAX_QV : nullptr,
+              chosen ? chosen + g0 : nullptr);
+      if (prep) {
+        prep->join();  // done: every query has started, so every one was prepared
+        t_filter += prep->t_filter;
+        t_tables += prep->t_tables;
+      }
     }
     sum_filter += t_filter;
     sum_tables += t_tables;
-    const double t_orders = run_items(c, items, mlp, max_orders, o, streamed, results + g0,
-                                      level_rows ? level_rows + (uint64_t)g0 * GEM_MAX_QV : nullptr,
-                                      chosen ? chosen + g0 : nullptr);
     if (verbose()) {
-      std::fprintf(stderr, "[batch] %u queries done in %.2f s (candidate orders, features and plans %.2f s)\n", gn,
-                   secs(t_prep, clk::now()), t_orders);
+      std::fprintf(stderr, "[batch] %u queries done in %.2f s\n", gn, secs(t_prep, clk::now()));
       std::fflush(stderr);
     }
   }
@@ -438,7 +598,11 @@ void run_match_relaxed(gem_ctx *c, const gem_query_view *q, uint32_t threshold,
     const uint32_t gn = std::min(group, n - g0);
     std::vector<BatchItem> items(gn);
     for (uint32_t i = 0; i < gn; ++i) items[i] = BatchItem{&parent, &Qi[g0 + i], &emap[g0 + i]};
-    run_items(c, items, mlp, max_orders, o, false, results + g0, nullptr, nullptr);
+    std::vector<const gem_query_view *> vv(gn);
+    for (uint32_t i = 0; i < gn; ++i) vv[i] = &views[g0 + i];
+    ItemRun run(c, items, mlp, max_orders, o);
+    for (uint32_t i = 0; i < gn; ++i) run.candidates(c, i);
+    run.run(gn, false, vv, results + g0, nullptr, nullptr);
   }
   if (verbose()) {
     std::fprintf(stderr, "[relaxed] %u relaxed queries sharing one filter (%.2f s) and one table build (%.2f s)\n",

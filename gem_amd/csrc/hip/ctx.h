@@ -270,6 +270,13 @@ struct gem_ctx {
   int graph_host = 0;                       // next data graph: neighbours stay in pinned host memory
   bool relaxed_share = true;                // relaxed queries share one filter and one table build
   bool racing = true;                       // batched learned path: twins race stragglers
+  bool prep_overlap = true;                 // batched learned path: later queries prepared during the join
+  // A context that prepares queries on its own stream for another one reads
+  // that context's data graph (gsrc) instead of its own; `prep` is that
+  // context of this one (made at its first use, destroyed with this one, so
+  // that its stream's pool memory is reused from run to run).
+  const gemd::GraphDev *gsrc = nullptr;
+  gem_ctx *prep = nullptr;
   uint64_t swap_bytes = 0;                  // batched paths: host memory pending levels may move to
   bool use_graph = false;                   // replay the batched extend step as a hipGraph
   // batched path with streamed edge tables: values in pinned host memory,
@@ -364,6 +371,9 @@ T *pinned(gem_ctx *c, size_t n) {
 }
 
 // ---------------------------------------------------------------- phases
+// The data graph a context's filter and table kernels read.
+inline const GraphDev &graph(const gem_ctx *c) { return c->gsrc ? *c->gsrc : c->g; }
+
 void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T);
 void run_tables(gem_ctx *c);
 // the same phases on an explicit per-query state (several queries resident)

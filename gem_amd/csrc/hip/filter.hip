@@ -375,7 +375,7 @@ __global__ void k_zero_u32(uint32_t *p, uint64_t n) {
 // the rank scan, and the compaction of Can(u).  No allocation; the counters
 // are cleared by a kernel (no memset node when a caller captures the stream).
 static void filter_launch(gem_ctx *c, uint32_t nv, uint32_t W, const FilterBufs &B) {
-  const GraphDev &G = c->g;
+  const GraphDev &G = graph(c);
   hipStream_t s = c->stream;
   k_zero_u32<<<1, 64, 0, s>>>(B.heavy_n, 1);
   k_zero_u32<<<1, 64, 0, s>>>(B.hub_n, 1);
@@ -445,13 +445,13 @@ static void filter_launch(gem_ctx *c, uint32_t nv, uint32_t W, const FilterBufs 
 void run_filter(gem_ctx *c, const gem_query_view *qv, uint32_t T) { run_filter(c, c->q, qv, T); }
 
 void run_filter(gem_ctx *c, QueryDev &Qd, const gem_query_view *qv, uint32_t T) {
-  if (!c->g.loaded) throw Error{GEM_E_STATE, "no data graph loaded"};
+  if (!graph(c).loaded) throw Error{GEM_E_STATE, "no data graph loaded"};
   Qd = QueryDev();
   std::string why;
   int st = gemh::analyse_query(qv, &Qd.Q, &why);
   if (st) throw Error{st, why};
   const gemh::QueryInfo &Q = Qd.Q;
-  const GraphDev &G = c->g;
+  const GraphDev &G = graph(c);
   const uint32_t nv = Q.nv;
   const uint32_t W = (G.n + 63) / 64;
   Qd.W = W;
@@ -554,7 +554,7 @@ uint64_t filter_scratch_bytes(const GraphDev &G, uint32_t nv) {
 
 void run_filter_noalloc(gem_ctx *c, const gem_query_view *qv, uint32_t T, uint32_t *d_num_can, uint32_t *d_can,
                         uint64_t can_pitch) {
-  if (!c->g.loaded) throw Error{GEM_E_STATE, "no data graph loaded"};
+  if (!graph(c).loaded) throw Error{GEM_E_STATE, "no data graph loaded"};
   if (!d_num_can) throw Error{GEM_E_INVALID, "d_num_can is null"};
   if (c->device < 0 || c->device >= SCRATCH_MAX_DEVICES || !g_scratch_ptr[c->device])
     throw Error{GEM_E_STATE, "scratch not initialised on this device"};
@@ -562,7 +562,7 @@ void run_filter_noalloc(gem_ctx *c, const gem_query_view *qv, uint32_t T, uint32
   std::string why;
   int st = gemh::analyse_query(qv, &Q, &why);
   if (st) throw Error{st, why};
-  const GraphDev &G = c->g;
+  const GraphDev &G = graph(c);
   const uint32_t nv = Q.nv;
   const uint32_t W = (G.n + 63) / 64;
   const uint64_t need = filter_scratch_bytes(G, nv);

@@ -296,7 +296,7 @@ void run_tables(gem_ctx *c) { run_tables(c, c->q); }
 void run_tables(gem_ctx *c, QueryDev &Qd) {
   if (!Qd.filtered) throw Error{GEM_E_STATE, "filter has not run"};
   const gemh::QueryInfo &Q = Qd.Q;
-  const GraphDev &G = c->g;
+  const GraphDev &G = graph(c);
   hipStream_t s = c->stream;
   GEM_HIP_CHECK(hipEventRecord(c->ev[2], s));
   Qd.tab.clear();

@@ -81,6 +81,23 @@ struct VQueryState {
   // at its partner's current limit.
   bool twin = false;
   uint32_t shift = 0;
+  // Not yet complete when the run starts (ranked orders and plan missing):
+  // a query prepared on another stream during the run, or a twin; completed
+  // when it starts.  max_nb / max_ndup then hold one upper bound each (a
+  // level has at most the degree of its vertex in backward lists and one
+  // equal-label column less than the vertices of its label), which sizes the
+  // run's launch buffers.
+  bool pending = false;
+  uint32_t pending_nv = 0;
+};
+
+// Completion of pending primaries (see VQueryState::pending): ready() says
+// whether query q's filter, tables and candidate orders exist; complete()
+// waits for them, then ranks its orders and plans it.
+struct LazyStates {
+  virtual bool ready(uint32_t q) = 0;
+  virtual void complete(uint32_t q) = 0;
+  virtual ~LazyStates() = default;
 };
 
 void build_vlevels(gem_ctx *c, VQueryState &S);
@@ -98,6 +115,7 @@ class VBatchRunner {
   // queries [0, n) are the batch, later ones race a partner among them
   // (SQuery::partner, VQueryState::twin); 0 = no racing
   void set_primaries(uint32_t n) { primaries_ = n; }
+  void set_lazy(LazyStates *lazy) { lazy_ = lazy; }
   void run(std::vector<VQueryState *> &qs);
   const gemh::SchedStats &stats() const { return stats_; }
 
@@ -106,6 +124,7 @@ class VBatchRunner {
   uint64_t cap_, work_limit_, budget_bytes_;
   bool streamed_;
   uint32_t primaries_ = 0;
+  LazyStates *lazy_ = nullptr;
   gemh::SchedStats stats_;
 };
 

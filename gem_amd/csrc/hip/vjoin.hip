@@ -933,6 +933,7 @@ struct HipSchedBackend {
   TableCache *tc;
   bool verbose;
   uint64_t cap_;
+  LazyStates *lazy = nullptr;
   using clk = std::chrono::steady_clock;
   clk::time_point last_print = clk::now();
   double t_len = 0, t_ext = 0;
@@ -1096,8 +1097,55 @@ struct HipSchedBackend {
     return b;
   }
 
+  bool ready(uint32_t qi) {
+    const VQueryState &V = *qs[qi];
+    return !V.pending || V.twin || !lazy || lazy->ready(qi);
+  }
+  void wait_ready(uint32_t qi) {
+    const VQueryState &V = *qs[qi];
+    if (V.pending && !V.twin && lazy) lazy->complete(qi);
+  }
+
+  // A pending state is completed when it starts: a primary by its preparer
+  // (ranked orders, plan), a twin from its partner (which started before it).
+  // Returns false when there is nothing to run (a twin of a query with one
+  // candidate order).
+  bool complete(uint32_t qi) {
+    VQueryState &V = *qs[qi];
+    if (!V.pending) return true;
+    if (V.twin) {
+      const VQueryState &P = *qs[(size_t)V.s.partner];
+      if (P.pending || P.ranked.size() < 2) return false;
+      V.first_attempt = P.first_attempt;
+      V.growth = P.growth;
+      V.width = P.width;
+      V.max_attempts = P.max_attempts;
+      V.ranked = P.ranked;
+      V.ranked_idx = P.ranked_idx;
+      V.mnb_all = P.mnb_all;
+      V.mnd_all = P.mnd_all;
+    } else {
+      if (!lazy) throw Error{GEM_E_STATE, "a pending query without a preparer"};
+      lazy->complete(qi);
+    }
+    if (V.mnb_all > slot.max_nb || V.mnd_all > slot.max_ndup)
+      throw Error{GEM_E_STATE, "a query needs more launch LDS than the run was sized for"};
+    V.pending = false;
+    return true;
+  }
+
   void admit(uint32_t qi) {
     VQueryState &V = *qs[qi];
+    if (!complete(qi)) {
+      V.s.done = true;
+      V.s.cancelled = true;
+      return;
+    }
+    if (!V.twin) {
+      // a primary completed now: its levels' list counts and first limit
+      V.s.nb = V.max_nb;
+      if (!V.s.attempts) V.s.attempt_limit = attempt_limit(V, 0);
+    }
     if (V.twin && V.s.partner >= 0) {
       // a racing twin starts at its partner's current limit, on the order
       // after the one its partner runs
@@ -1107,6 +1155,7 @@ struct HipSchedBackend {
       V.shift = (P.cur + 1 + K - a % K) % K;
       V.cur = (a + V.shift) % K;
       replan(c, V, V.ranked[V.cur]);
+      V.s.nb = V.max_nb;
       if (V.mnb_all > slot.max_nb || V.mnd_all > slot.max_ndup)
         throw Error{GEM_E_STATE, "racing order needs more launch LDS than the run was sized for"};
       V.s.attempts = a;
@@ -1256,15 +1305,17 @@ void VBatchRunner::run(std::vector<VQueryState *> &qs) {
     const int64_t partner = Q.partner;
     Q = gemh::SQuery<DevBuf>{};
     Q.partner = partner;
-    Q.nv = S->qd->Q.nv;
-    Q.nb = S->max_nb;
+    // (a pending state's query may still be being prepared: its vertex count
+    // was recorded when the state was made)
+    Q.nv = S->pending ? S->pending_nv : S->qd->Q.nv;
+    Q.nb = S->pending ? std::vector<uint32_t>(Q.nv, S->max_nb.empty() ? 1u : S->max_nb[0]) : S->max_nb;
     Q.level_rows.assign(Q.nv, 0);
-    Q.attempt_limit = attempt_limit(*S, 0);
+    Q.attempt_limit = S->pending ? 0 : attempt_limit(*S, 0);
     sq.push_back(&Q);
   }
   const char *verbose_env = std::getenv("GEM_VERBOSE");
   const bool verbose = verbose_env && verbose_env[0] == '1';
-  HipSchedBackend be{c_, qs, slot, tc.get(), verbose, cap_};
+  HipSchedBackend be{c_, qs, slot, tc.get(), verbose, cap_, lazy_};
   gemh::SchedConfig cfg;
   cfg.cap = cap_;
   cfg.work_limit = work_limit_;

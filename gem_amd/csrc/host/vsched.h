@@ -227,6 +227,10 @@ inline int plan_chunks(const std::vector<ChunkCand> &cand, uint64_t left, uint64
 //   // query q's attempt went past its limit, its levels still pending: raise
 //   // its attempt_limit and return true to let it go on, or return false
 //   bool prolong(uint32_t q);
+//   // is query q ready to start (its candidates and tables prepared)?  If not,
+//   // the schedule starts it later, or waits for it when nothing else runs
+//   bool ready(uint32_t q);
+//   void wait_ready(uint32_t q);
 //   // swapping only: copy `bytes` of a device buffer to new host memory, and
 //   // back to a new device buffer (copies ordered on the backend's stream)
 //   std::shared_ptr<void> swap_out(const Buf &rows, uint64_t bytes);
@@ -269,7 +273,14 @@ void run_schedule(BE &be, std::vector<SQuery<typename BE::Buf> *> &qs, const Sch
       for (uint32_t i = 0; i < qs.size(); ++i)
         if (started[i] && !qs[i]->done && !qs[i]->stack.empty()) ++running;
       while (next < qs.size() && (!cfg.max_active || running < cfg.max_active)) {
-        const uint32_t qi = (uint32_t)next++;
+        const uint32_t qi = (uint32_t)next;
+        // a query prepared elsewhere (another stream) starts once it is
+        // ready: later, while others run, or now, after waiting for it
+        if (!qs[qi]->done && !be.ready(qi)) {
+          if (running) break;
+          be.wait_ready(qi);
+        }
+        ++next;
         started[qi] = 1;
         if (qs[qi]->done) continue;
         if (qs[qi]->stack.empty()) be.admit(qi);

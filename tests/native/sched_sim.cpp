@@ -142,6 +142,8 @@ struct SimBackend {
   void on_iteration(uint64_t, size_t, uint64_t, uint64_t) {}
   bool restart(uint32_t) { return false; }
   bool prolong(uint32_t) { return false; }
+  bool ready(uint32_t) { return true; }
+  void wait_ready(uint32_t) {}
   void admit(uint32_t) {}
   // swapping: a host copy keeps the rows' identity (buffer id, rows)
   struct HostRows {
