@@ -99,6 +99,7 @@ SIGNATURES = [
     ("gem_hip_set_graph_residency", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_set_relaxed_sharing", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_set_racing", C.c_int, [C.c_void_p, C.c_int]),
+    ("gem_hip_set_prep_overlap", C.c_int, [C.c_void_p, C.c_int]),
     ("gem_hip_set_level_swap", C.c_int, [C.c_void_p, C.c_uint64]),
     ("gem_hip_swap_stats", C.c_int, [C.c_void_p, C.c_void_p]),
     ("gem_export_rows", C.c_int, [u32p, C.c_uint64, C.c_uint32, C.c_char_p, C.c_uint64, C.c_void_p]),

@@ -155,6 +155,12 @@ int gem_hip_set_racing(gem_ctx *c, int on) {
   return GEM_OK;
 }
 
+int gem_hip_set_prep_overlap(gem_ctx *c, int on) {
+  if (!c) return GEM_E_INVALID;
+  c->prep_overlap = on != 0;
+  return GEM_OK;
+}
+
 int gem_hip_set_relaxed_sharing(gem_ctx *c, int on) {
   if (!c) return GEM_E_INVALID;
   c->relaxed_share = on != 0;

@@ -178,6 +178,11 @@ class GemContext:
         """Twins race the batched learned path's stragglers (default on)."""
         self._check(self._lib.gem_hip_set_racing(self._h, int(bool(on))), "gem_hip_set_racing")
 
+    def set_prep_overlap(self, on=True):
+        """The batched learned path prepares the queries past its first
+        max_active + 16 on a second stream while the join runs (default on)."""
+        self._check(self._lib.gem_hip_set_prep_overlap(self._h, int(bool(on))), "gem_hip_set_prep_overlap")
+
     def set_relaxed_sharing(self, on=True):
         """Relaxed queries share one filter at the threshold and one table per
         query edge (default), or are filtered and tabled one by one."""

@@ -383,6 +383,14 @@ int gem_hip_set_max_active(gem_ctx *ctx, uint32_t queries);
  * pair ends first with a count cancels the other.  Results are the same; a
  * query's `work` includes its twin's.  Not used with streamed tables. */
 int gem_hip_set_racing(gem_ctx *ctx, int on);
+/* on != 0 (default): in the batched learned path without streamed tables,
+ * with continuous admission (max_active != 0) and profiling off, the first
+ * max_active + 16 queries of a group are filtered and tabled before the join
+ * starts and the rest by a second context of this one (its own stream, made at
+ * the first such run, destroyed with this one) in a host thread while the join
+ * runs; a query starts once it is prepared, or the join waits for it when
+ * nothing else runs.  Results are the same. */
+int gem_hip_set_prep_overlap(gem_ctx *ctx, int on);
 /* Swapping of pending join levels (the reference's out-of-core LRU swap of
  * intermediate results, out-of-core/lib/structure/src/memory_manager.cu:748-822
  * with LRUEvictPolicy, policy.cpp:106-172, hooked into the join at

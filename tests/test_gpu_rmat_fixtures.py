@@ -139,3 +139,26 @@ def test_streamed_counts_match_oracle_fixtures(gpu_ctx):
     bad = [(e["index"], r["count"], r["saturated"], r["work_limited"]) for (e, _), r in zip(picked, res)
            if r["work_limited"] or (r["count"], r["saturated"]) != (e["count"], e["saturated"])]
     assert not bad, bad
+
+
+@pytest.mark.parametrize("overlap", [True, False])
+def test_prep_overlap_matches_oracle_fixtures(gpu_ctx, overlap):
+    """With 8 queries running at once, the R-MAT 2^20 workload's queries past
+    the first 24 are prepared on the second stream while the join runs (or,
+    with the overlap off, all before it): every count equals the fixture."""
+    fx, g, picked = _setup(gpu_ctx, "rmat20")
+    qs = [q for _, q in picked]
+    assert len(qs) > 24
+    model = _model(gpu_ctx, "rmat20")
+    gpu_ctx.set_max_active(8)
+    gpu_ctx.set_prep_overlap(overlap)
+    try:
+        res = match_batch_learned(gpu_ctx, qs, model, max_orders=8, cap=fx["cap"], first_attempt=FIRST_ATTEMPT)
+    finally:
+        gpu_ctx.set_prep_overlap(True)
+        gpu_ctx.set_max_active(64)
+        if model is not None:
+            model.close()
+    bad = [(e["index"], r["count"], r["saturated"], r["work_limited"]) for (e, _), r in zip(picked, res)
+           if r["work_limited"] or (r["count"], r["saturated"]) != (e["count"], e["saturated"])]
+    assert not bad, bad
