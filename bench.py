@@ -163,6 +163,8 @@ def main():
                          "tables in HBM otherwise")
     ap.add_argument("--stream-slot", default="1M", help="cache slot size for streamed tables")
     ap.add_argument("--no-racing", action="store_true", help="no twins racing the batch's stragglers")
+    ap.add_argument("--no-prep-overlap", action="store_true",
+                    help="prepare every query before the join starts (no second stream)")
     ap.add_argument("--graph-host", choices=["auto", "on", "off"], default="auto",
                     help="keep the data graph's neighbour array in pinned host memory (auto: on for ooc24)")
     args = ap.parse_args()
@@ -215,6 +217,7 @@ def main():
         ctx.set_table_stream(stream_cache, parse_bytes(args.stream_slot))
         log(f"[bench] edge tables streamed from host memory through a {args.stream_cache} device cache")
     ctx.set_racing(not args.no_racing)
+    ctx.set_prep_overlap(not args.no_prep_overlap)
 
     # ---- batched path with the learned order (the measured value)
     from gem_amd.learned import OrderModel, match_batch_learned
@@ -417,6 +420,7 @@ def main():
                    "cap": cap, "restarts": {"first_attempt_work": first, "growth": args.restart_growth,
                                             "width": args.restart_width,
                                             "racing_twins": not args.no_racing and not stream_cache},
+                   "prep_overlap": not args.no_prep_overlap and not stream_cache,
                    "work_limit": wl,
                    "tuning": tuning or None},
         "embeddings_per_s": round(eps, 1),
