@@ -73,6 +73,7 @@ SIGNATURES = [
     ("gem_graph_generate_rmat", C.c_int, [C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
                                           C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
     ("gem_graph_from_csr", C.c_int, [C.c_uint32, u64p, u32p, u32p, C.POINTER(C.c_void_p)]),
+    ("gem_graph_from_csr32", C.c_int, [C.c_uint32, u32p, u32p, u32p, C.POINTER(C.c_void_p)]),
     ("gem_graph_load", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     ("gem_graph_save", C.c_int, [C.c_void_p, C.c_char_p]),
     ("gem_graph_save_binary", C.c_int, [C.c_void_p, C.c_char_p]),

@@ -9,6 +9,7 @@
 // searches (in-core/lib/join/src/join_trie_kernel.cu:148); this reader sorts
 // each list, which equals the reference's layout whenever the file is valid
 // input for it.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -142,6 +143,31 @@ int gem_graph_from_csr(uint32_t n, const uint64_t *offsets, const uint32_t *neig
   g->lab.assign(labels, labels + n);
   std::string why;
   if (!validate_csr(*g, &why)) {
+    delete g;
+    return GEM_E_INVALID;
+  }
+  *out = g;
+  return GEM_OK;
+}
+
+int gem_graph_from_csr32(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors, const uint32_t *labels,
+                         gem_graph **out) {
+  if (!out || !offsets || !labels || (offsets[n] && !neighbors)) return GEM_E_INVALID;
+  *out = nullptr;
+  if (offsets[0] != 0) return GEM_E_INVALID;
+  for (uint32_t v = 0; v < n; ++v)
+    if (offsets[v + 1] < offsets[v]) return GEM_E_INVALID;
+  auto *g = new gem_graph();
+  g->n = n;
+  g->off.assign(offsets, offsets + n + 1);
+  g->nbr.assign(neighbors, neighbors + offsets[n]);
+  g->lab.assign(labels, labels + n);
+  // the reference's loader appends neighbours in edge-line order
+  gemh::parallel_chunks(n, [&](uint64_t b, uint64_t e) {
+    for (uint64_t v = b; v < e; ++v) std::sort(g->nbr.begin() + g->off[v], g->nbr.begin() + g->off[v + 1]);
+  });
+  std::string why;
+  if (!gemh::validate_csr(*g, &why)) {
     delete g;
     return GEM_E_INVALID;
   }

@@ -69,6 +69,20 @@ class Graph:
         return cls(h)
 
     @classmethod
+    def from_csr32(cls, offsets, neighbors, labels):
+        """The reference loader's host CSR (u32 offsets, adjacency in edge-line
+        order): offsets widened and every list sorted."""
+        off = np.ascontiguousarray(offsets, dtype=np.uint32)
+        nbr = np.ascontiguousarray(neighbors, dtype=np.uint32)
+        lab = np.ascontiguousarray(labels, dtype=np.uint32)
+        h = C.c_void_p()
+        _check(load().gem_graph_from_csr32(len(lab), off.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           nbr.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           lab.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(h)),
+               "gem_graph_from_csr32")
+        return cls(h)
+
+    @classmethod
     def from_edges(cls, n, edges, labels):
         """Build from an undirected edge list (pairs); duplicates are an error."""
         e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)

@@ -76,6 +76,17 @@ int gem_graph_generate_rmat(uint32_t scale, double avg_degree, double a,
 int gem_graph_from_csr(uint32_t n, const uint64_t *offsets,
                        const uint32_t *neighbors, const uint32_t *labels,
                        gem_graph **out);
+/* The reference's host CSR as its loader leaves it: cpuGraph::offsets_ (u32,
+ * [n+1]), neighbors_ (file order per vertex) and vLabels_
+ * (in-core/lib/graph/include/cpuGraph.h:23-31, filled by
+ * readGraphToCPUStandard, in-core/lib/io/src/io.cu:69-199).  Offsets are
+ * widened, each adjacency list is sorted; self-loops, repeated neighbours,
+ * asymmetric edges or neighbours out of range give GEM_E_INVALID.  Labels are
+ * taken as given (see the integration notes for the reference's per-query
+ * label ids). */
+int gem_graph_from_csr32(uint32_t n, const uint32_t *offsets,
+                         const uint32_t *neighbors, const uint32_t *labels,
+                         gem_graph **out);
 /* Reference text format: "t N M" / "v id label degree" / "e src dst"
  * (in-core/lib/io/src/io.cu:45-220).  Declared degrees must match. */
 int gem_graph_load(const char *path, gem_graph **out);

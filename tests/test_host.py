@@ -227,3 +227,54 @@ def test_binary_csr_cache(tmp_path):
     (tmp_path / "short.bin").write_bytes(raw[:-4])
     with pytest.raises(GemError):
         Graph.load(tmp_path / "short.bin")
+
+
+def _reference_loader_csr(n, edges):
+    """The host CSR the reference's loader leaves for an edge list: u32 offsets
+    from the degrees, and each edge line appended to both endpoints' lists in
+    file order (in-core/lib/io/src/io.cu:169-199)."""
+    deg = np.zeros(n, np.int64)
+    for s, d in edges:
+        deg[s] += 1
+        deg[d] += 1
+    off = np.zeros(n + 1, np.uint32)
+    off[1:] = np.cumsum(deg)
+    nbr = np.zeros(int(off[-1]), np.uint32)
+    fill = np.zeros(n, np.int64)
+    for s, d in edges:
+        nbr[off[s] + fill[s]] = d
+        fill[s] += 1
+        nbr[off[d] + fill[d]] = s
+        fill[d] += 1
+    return off, nbr
+
+
+def test_graph_from_reference_host_csr():
+    """The reference loader's file-order CSR becomes the sorted CSR of the same
+    graph; a repeated edge, a self-loop, an asymmetric list and a neighbour out
+    of range are refused."""
+    rng = np.random.default_rng(7)
+    g = Graph.erdos_renyi(400, 1500, 6, 3)
+    off, nbr = g.offsets, g.neighbors
+    edges = [(v, int(w)) for v in range(400) for w in nbr[off[v]:off[v + 1]] if v < w]
+    edges = [edges[i] for i in rng.permutation(len(edges))]
+    edges = [(d, s) if rng.random() < 0.5 else (s, d) for s, d in edges]
+    ro, rn = _reference_loader_csr(400, edges)
+    assert any(np.any(np.diff(rn[ro[v]:ro[v + 1]].astype(np.int64)) < 0) for v in range(400))  # unsorted input
+    h = Graph.from_csr32(ro, rn, g.labels)
+    assert np.array_equal(h.offsets, g.offsets) and np.array_equal(h.neighbors, g.neighbors)
+    assert np.array_equal(h.labels, g.labels) and h.digest() == g.digest()
+    lab = np.zeros(4, np.uint32)
+    for bad_edges in ([(0, 1), (1, 2), (0, 1)], [(0, 1), (2, 2)]):
+        ro, rn = _reference_loader_csr(4, bad_edges)
+        with pytest.raises(GemError):
+            Graph.from_csr32(ro, rn, lab)
+    ro, rn = _reference_loader_csr(4, [(0, 1), (1, 2)])
+    rn_asym = rn.copy()
+    rn_asym[np.where(rn_asym == 2)[0][0]] = 3  # 1 lists 3, 3 does not list 1
+    with pytest.raises(GemError):
+        Graph.from_csr32(ro, rn_asym, lab)
+    rn_far = rn.copy()
+    rn_far[0] = 9
+    with pytest.raises(GemError):
+        Graph.from_csr32(ro, rn_far, lab)
