@@ -9,7 +9,12 @@
 // several passes on one pool and prints one JSON object per pass.
 //
 //   sched_sim run <queries> <budget_bytes> <passes> <seed> [max_active] [swap_bytes]
+//                 [eager] [every]
+// eager/every: queries from `eager` on are prepared elsewhere, in order, one
+// per `every` extend steps (0 = all ready at the start), as a batch whose later
+// queries are prepared on a second stream while the join runs.
 //   sched_sim plan      (fixed plan_chunks cases, one JSON line)
+#include <algorithm>
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
@@ -64,6 +69,24 @@ struct SimBackend {
   uint64_t seed;
   uint64_t next_id = 1;
   uint64_t length_items = 0, extend_calls = 0;
+  // lazy preparation: query q >= eager is ready once every * (q - eager + 1)
+  // extend steps have run, or once the schedule waited for it (the preparer
+  // works in order, so a wait for q readies every query up to q)
+  const std::vector<gemh::SQuery<SimBuf> *> *qs = nullptr;
+  uint32_t eager = ~0u;
+  uint64_t every = 0;
+  uint32_t waited_up_to = 0;  // queries below it are ready
+  std::vector<char> started;
+  uint64_t ready_waits = 0, not_ready = 0;
+  bool is_ready(uint32_t q) const {
+    return q < eager || q < waited_up_to || extend_calls >= every * (uint64_t)(q - eager + 1);
+  }
+  void check_ready(uint32_t q, const char *what) const {
+    if (!is_ready(q)) {
+      std::fprintf(stderr, "query %u %s before it was ready\n", q, what);
+      std::exit(6);
+    }
+  }
 
   static uint64_t buf_bytes(const SimBuf &b) { return b.bytes; }
 
@@ -99,6 +122,7 @@ struct SimBackend {
     prefix->resize(items.size());
     for (size_t t = 0; t < items.size(); ++t) {
       const auto &x = items[t];
+      check_ready(x.q, "was measured");
       if (!x.rows) {
         std::fprintf(stderr, "a level in host memory was measured\n");
         std::exit(5);
@@ -119,6 +143,7 @@ struct SimBackend {
     got_fin->assign(fin.size(), 0);
     for (size_t t = 0; t < wr.size(); ++t) {
       const auto &x = wr[t];
+      check_ready(x.q, "was extended");
       if (!x.rows) {
         std::fprintf(stderr, "a level in host memory was extended\n");
         std::exit(5);
@@ -134,6 +159,7 @@ struct SimBackend {
     }
     for (size_t t = 0; t < fin.size(); ++t) {
       const auto &x = fin[t];
+      check_ready(x.q, "was counted");
       const uint64_t nrows = level_rows_of(x);
       for (uint64_t u = x.row_begin / gemh::SCHED_UNIT; u * gemh::SCHED_UNIT < x.row_end; ++u)
         (*got_fin)[t] += survivors(x.rows->id, x.level, u, nrows, true);
@@ -142,8 +168,26 @@ struct SimBackend {
   void on_iteration(uint64_t, size_t, uint64_t, uint64_t) {}
   bool restart(uint32_t) { return false; }
   bool prolong(uint32_t) { return false; }
-  bool ready(uint32_t) { return true; }
-  void wait_ready(uint32_t) {}
+  // a query the schedule is told is ready starts at once
+  bool ready(uint32_t q) {
+    if (!is_ready(q)) {
+      ++not_ready;
+      return false;
+    }
+    started[q] = 1;
+    return true;
+  }
+  // waiting is for when nothing runs: no started query has levels pending
+  void wait_ready(uint32_t q) {
+    for (size_t i = 0; i < qs->size(); ++i)
+      if (started[i] && !(*qs)[i]->done && !(*qs)[i]->stack.empty()) {
+        std::fprintf(stderr, "waited for query %u while query %zu runs\n", q, i);
+        std::exit(7);
+      }
+    ++ready_waits;
+    waited_up_to = std::max(waited_up_to, q + 1);
+    started[q] = 1;
+  }
   void admit(uint32_t) {}
   // swapping: a host copy keeps the rows' identity (buffer id, rows)
   struct HostRows {
@@ -172,7 +216,8 @@ struct SimBackend {
   }
 };
 
-int run(int nq, uint64_t budget, int passes, uint64_t seed, uint32_t max_active, uint64_t swap_bytes) {
+int run(int nq, uint64_t budget, int passes, uint64_t seed, uint32_t max_active, uint64_t swap_bytes, uint32_t eager,
+        uint64_t every) {
   FakeDevice dev;
   gemh::PoolCore pool(gemh::PoolBacking{fake_alloc, fake_release, fake_sync, &dev});
   for (int pass = 0; pass < passes; ++pass) {
@@ -199,6 +244,12 @@ int run(int nq, uint64_t budget, int passes, uint64_t seed, uint32_t max_active,
       Q.stack.push_back(std::move(L));
       ptrs.push_back(&Q);
     }
+    if (every) {
+      be.qs = &ptrs;
+      be.eager = eager;
+      be.every = every;
+    }
+    be.started.assign(nq, 0);
     gemh::SchedConfig cfg;
     cfg.cap = 1000000000ull;
     cfg.work_limit = 2000000000ull;
@@ -231,11 +282,13 @@ int run(int nq, uint64_t budget, int passes, uint64_t seed, uint32_t max_active,
         ", \"work_limited\": %" PRIu64 ", \"finished\": %" PRIu64 ", \"pending\": %" PRIu64
         ", \"length_items\": %" PRIu64 ", \"extend_calls\": %" PRIu64 ", \"swaps_out\": %" PRIu64
         ", \"swaps_in\": %" PRIu64 ", \"swap_bytes_out\": %" PRIu64 ", \"peak_host\": %" PRIu64
-        ", \"backend_host_peak\": %" PRIu64 ", \"host_left\": %" PRIu64 "}\n",
+        ", \"backend_host_peak\": %" PRIu64 ", \"host_left\": %" PRIu64 ", \"ready_waits\": %" PRIu64
+        ", \"not_ready\": %" PRIu64 ", \"started\": %" PRIu64 "}\n",
         pass, nq, max_nv, budget, force_chunk, st.iterations, st.forced, st.waits, st.peak_charged,
         st.max_overshoot, st.max_unit_bytes, before.in_use, after.peak_in_use, after.in_use, after.held, trimmed,
         trimmed_st.held, saturated, limited, finished, pending, be.length_items, be.extend_calls, st.swaps_out,
-        st.swaps_in, st.swap_bytes_out, st.peak_host, be.host_peak, be.host_live);
+        st.swaps_in, st.swap_bytes_out, st.peak_host, be.host_peak, be.host_live, be.ready_waits, be.not_ready,
+        (uint64_t)std::count(be.started.begin(), be.started.end(), 1));
     std::fflush(stdout);
   }
   return 0;
@@ -336,7 +389,8 @@ int main(int argc, char **argv) {
   if (argc >= 6 && !std::strcmp(argv[1], "run"))
     return run(std::atoi(argv[2]), std::strtoull(argv[3], nullptr, 10), std::atoi(argv[4]),
                std::strtoull(argv[5], nullptr, 10), argc >= 7 ? (uint32_t)std::atoi(argv[6]) : 0u,
-               argc >= 8 ? std::strtoull(argv[7], nullptr, 10) : 0ull);
+               argc >= 8 ? std::strtoull(argv[7], nullptr, 10) : 0ull,
+               argc >= 9 ? (uint32_t)std::atoi(argv[8]) : 0u, argc >= 10 ? std::strtoull(argv[9], nullptr, 10) : 0ull);
   std::fprintf(stderr, "usage: sched_sim run <queries> <budget> <passes> <seed> [max_active] | sched_sim plan\n");
   return 2;
 }

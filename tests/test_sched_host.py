@@ -128,3 +128,20 @@ def test_scheduler_swaps_levels_to_host(sim):
     assert p["pending"] == 0 and p["saturated"] + p["work_limited"] + p["finished"] == p["queries"]
     assert p["pool_in_use_after"] == 0
     assert p["peak_charged"] <= q["peak_charged"]
+
+
+def test_scheduler_lazy_admission(sim):
+    """160 queries, 16 running, the first 24 ready at the start and the rest
+    prepared elsewhere one per 5,000 extend steps: no level of a query is
+    measured, extended or counted before it is ready (the simulator exits if
+    one is), the schedule skips a query that is not ready while others run and
+    waits for it only when nothing runs (it exits otherwise), every query
+    starts and ends, and the charged bytes keep their bound."""
+    budget = 256 << 20
+    (p,) = _run(sim, "run", 160, budget, 1, 13, 16, 0, 24, 5000)
+    assert p["not_ready"] > 0 and p["ready_waits"] > 0
+    assert p["started"] == p["queries"] == 160
+    assert p["pending"] == 0 and p["saturated"] + p["work_limited"] + p["finished"] == p["queries"]
+    headroom = (p["max_nv"] - 1) * max(p["force_chunk"], p["max_unit_bytes"])
+    assert p["peak_charged"] <= budget + headroom
+    assert p["pool_in_use_after"] == 0 and p["held_after_trim"] == 0
