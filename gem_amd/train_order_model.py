@@ -72,7 +72,7 @@ def main():
     ap.add_argument("--eval-every", type=int, default=50)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--first-attempt", type=float, default=1e8, help="restart schedule the model is judged with")
+    ap.add_argument("--first-attempt", type=float, default=1e8, help="restart schedule the model is evaluated with")
     ap.add_argument("--growth", type=int, default=4)
     ap.add_argument("--width", type=int, default=1)
     ap.add_argument("--margin", type=float, default=0.0,

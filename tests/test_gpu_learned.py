@@ -74,7 +74,7 @@ def _schedule(T, ranking, first, limit):
 
 
 def test_learned_orders_gain_and_keep_counts(gpu_ctx, rmat_graph):
-    """Trained here on 48 queries of a 2^14-vertex R-MAT graph, judged on 24
+    """Trained here on 48 queries of a 2^14-vertex R-MAT graph, evaluated on 24
     others: the schedule's work with the model's ranking is printed, and the
     batched path's counts with the model equal those without it."""
     gpu_ctx.load_data(rmat_graph)

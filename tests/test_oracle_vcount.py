@@ -1,5 +1,5 @@
-"""Pins for the oracle's vertex-level walk (orc_vcount), which judges the
-batched GPU path's per-level row counts.
+"""Pins for the oracle's vertex-level walk (orc_vcount), against which the
+batched GPU path's per-level row counts are checked.
 
 level_rows[i] must equal the number of injective maps of the query vertices
 vorder[0..i] into their candidate sets that preserve every query edge among
