@@ -290,7 +290,10 @@ int main(int argc, char **argv) {
     else if ((a == "-g" || a == "--gpu") && (v = next())) gpu = std::atoi(v);
     else if ((a == "-t" || a == "--threshold") && (v = next())) threshold = std::atol(v);
     else if (a == "--cap" && (v = next())) cap = std::strtoull(v, nullptr, 10);
-    else if (a == "--eviction" && (v = next())) { /* accepted; tables are streamed, not evicted */ }
+    else if (a == "--eviction" && (v = next())) {
+      // accepted and ignored, as the reference's command line does
+      // (out-of-core/src/main.cu:128-130, 154: always LRU)
+    }
     else if (a == "--no-fork") fork_each = false;
     else if (a == "--export" && (v = next())) g_export = v;
     else {
