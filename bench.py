@@ -71,9 +71,9 @@ def cpu_model():
 
 
 def profiled(key, kernel, workload):
-    """A per-launch figure of `kernel` on `workload` recorded in the committed
-    rocprofv3 summaries under profiles/ as a line `<key> <kernel> <workload> N`,
-    or None when no summary covers it."""
+    """A figure of `kernel` on `workload` (per launch or per pass) recorded in
+    the committed rocprofv3 summaries under profiles/ as a line
+    `<key> <kernel> <workload> N`, or None when no summary covers it."""
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
         return None
@@ -322,12 +322,19 @@ def main():
         dom_name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
         achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else 0.0
         prof_us = profiled("avg_launch_us", dom_name, args.workload)
+        prof_total = profiled("total_us_per_pass", dom_name, args.workload)
+        prof_launches = profiled("launches_per_pass", dom_name, args.workload)
         roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": profiled_traffic(dom_name, args.workload),
                 "algorithmic_bytes_per_launch": round(dom["bytes"] / max(dom["launches"], 1)),
                 "launches": dom["launches"], "avg_launch_ms": round(dom["ms"] / max(dom["launches"], 1), 3),
-                "profiled_avg_launch_ms": round(prof_us / 1000, 3) if prof_us is not None else None}
+                "profiled_avg_launch_ms": round(prof_us / 1000, 3) if prof_us is not None else None,
+                # per pass: launches follow the pass's scheduler iterations,
+                # which vary from pass to pass, and so does the time per launch
+                "ms_per_pass": round(dom["ms"], 1),
+                "profiled_ms_per_pass": round(prof_total / 1000, 1) if prof_total is not None else None,
+                "profiled_launches": int(prof_launches) if prof_launches is not None else None}
 
     # ---- plain path, one pass over a systematic sample (every k-th query)
     plain = None

@@ -175,6 +175,9 @@ def test_profiles_cover_the_walk_kernels():
     for k in ("k_vext<write>", "k_vext<final>", "k_vext_heavy<write>", "k_vext_heavy<final>"):
         assert bench.profiled_traffic(k, "rmat22") > 0
         assert bench.profiled("avg_launch_us", k, "rmat22") > 0
+        # the per-pass total is launches x average, as rocprofv3 summed it
+        n, tot = bench.profiled("launches_per_pass", k, "rmat22"), bench.profiled("total_us_per_pass", k, "rmat22")
+        assert abs(tot / n - bench.profiled("avg_launch_us", k, "rmat22")) < 0.1
     assert bench.profiled_traffic("k_vext<write>", "no-such-workload") is None
 
 
